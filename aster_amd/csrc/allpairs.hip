@@ -1,0 +1,337 @@
+// All-pairs gravitational accelerations on CDNA4.
+//
+// FP32 paths: one target body per lane; source bodies are read with
+// wave-uniform addresses, so they arrive through the scalar cache into SGPRs
+// and feed VALU instructions directly (no LDS staging, no barriers). The
+// source range is split into slices (grid.y) so that a 64K-body problem puts
+// several waves on every SIMD; slice partials are combined in FP64 in a fixed
+// order, so results are deterministic. Per interaction the plain FP32 loop
+// issues 3 sub + 3 fma (r^2) + 1 rsq + 3 mul + 3 fma = 12 VALU + 1
+// transcendental; the mixed method forms the separation in FP64 and rounds it
+// (3 add_f64 + 3 cvt instead of 3 sub: 15 VALU + 1 transcendental) and
+// accumulates with Kahan compensation across 64-source tiles.
+//
+// FP64 path: the reference's operation order (Parallelized::compute_forces,
+// src/utils/builder_endpoint.cpp:729-757, with compute_gravitational,
+// src/physics/tool_chain_impl.cpp:120-137), one target per lane, sequential
+// sources, IEEE division and square root, no contraction: bit-identical to the
+// CPU path.
+#include "common.hpp"
+
+namespace aster {
+
+// ---------------------------------------------------------------- FP32 ----
+
+enum { ACC_PLAIN = 0, ACC_TILE_KAHAN = 1, ACC_FULL_KAHAN = 2 };
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float t) {
+  const float y = t - c;
+  const float u = s + y;
+  c = (u - s) - y;
+  s = u;
+}
+
+// Sources, padded to npad with zero-mass bodies at the origin.
+//   k_src32: FP32 {x, y, z, m} (positions rounded to FP32 once).
+//   k_src64: FP64 {x, y, z, m}; the kernel forms x_j - x_i in FP64 and rounds
+//            the separation to FP32, so the pair force is FP32 but a close pair
+//            does not lose ~|x|/r ulps to position rounding.
+__global__ __launch_bounds__(256) void k_src32(int n, int npad, const double* __restrict__ x,
+                                               const double* __restrict__ m, float4* __restrict__ src) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npad) return;
+  src[i] = i < n ? make_float4((float)x[i], (float)x[n + i], (float)x[2 * n + i], (float)m[i])
+                 : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// The w slot carries the FP32 mass in its low 32 bits, so the kernel reads it
+// straight from the low SGPR of the pair without a conversion.
+__device__ __forceinline__ double pack_mass(float m) {
+  return __longlong_as_double((long long)(unsigned long long)__float_as_uint(m));
+}
+__device__ __forceinline__ float unpack_mass(double w) {
+  return __uint_as_float((unsigned)(unsigned long long)__double_as_longlong(w));
+}
+
+__global__ __launch_bounds__(256) void k_src64(int n, int npad, const double* __restrict__ x,
+                                               const double* __restrict__ m, double4* __restrict__ src) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npad) return;
+  src[i] = i < n ? make_double4(x[i], x[n + i], x[2 * n + i], pack_mass((float)m[i]))
+                 : make_double4(0.0, 0.0, 0.0, pack_mass(0.f));
+}
+
+__device__ __forceinline__ float src_mass(const float4& q) { return q.w; }
+__device__ __forceinline__ float src_mass(const double4& q) { return unpack_mass(q.w); }
+
+template <bool SRC64>
+struct SrcT;
+template <>
+struct SrcT<false> {
+  using type = float4;
+};
+template <>
+struct SrcT<true> {
+  using type = double4;
+};
+
+// One target per lane (natural order), sources [s*slice_len, ...) of the
+// padded source array; out: [S][3][n] doubles.
+template <int ACC, bool MASK, bool SRC64>
+__global__ __launch_bounds__(256) void k_allpairs_f32(const typename SrcT<SRC64>::type* __restrict__ src,
+                                                      const double* __restrict__ x, int n, int npad, int slice_len,
+                                                      float eps2, double* __restrict__ out) {
+  const int i = blockIdx.x * kApBlock + threadIdx.x;
+  const int ii = i < n ? i : n - 1;
+  const int s = blockIdx.y;
+  const int jb = s * slice_len;
+  const int je = min(jb + slice_len, npad);
+  const double px = x[ii], py = x[n + ii], pz = x[2 * n + ii];
+  const float fx = (float)px, fy = (float)py, fz = (float)pz;
+  float sx = 0.f, sy = 0.f, sz = 0.f;
+  float cx = 0.f, cy = 0.f, cz = 0.f;
+  for (int t = jb; t < je; t += kApTile) {
+    float tx = 0.f, ty = 0.f, tz = 0.f;
+#pragma unroll 16
+    for (int k = 0; k < kApTile; ++k) {
+      const auto q = src[t + k];
+      float dx, dy, dz;
+      if (SRC64) {
+        dx = (float)(q.x - px);
+        dy = (float)(q.y - py);
+        dz = (float)(q.z - pz);
+      } else {
+        dx = q.x - fx;
+        dy = q.y - fy;
+        dz = q.z - fz;
+      }
+      float r2 = fmaf(dx, dx, eps2);
+      r2 = fmaf(dy, dy, r2);
+      r2 = fmaf(dz, dz, r2);
+      float inv = __builtin_amdgcn_rsqf(r2);
+      if (MASK) inv = (r2 > 0.f) ? inv : 0.f;
+      const float inv2 = inv * inv;
+      const float f = (src_mass(q) * inv) * inv2;
+      if (ACC == ACC_FULL_KAHAN) {
+        kahan_add(sx, cx, f * dx);
+        kahan_add(sy, cy, f * dy);
+        kahan_add(sz, cz, f * dz);
+      } else if (ACC == ACC_TILE_KAHAN) {
+        tx = fmaf(f, dx, tx);
+        ty = fmaf(f, dy, ty);
+        tz = fmaf(f, dz, tz);
+      } else {
+        sx = fmaf(f, dx, sx);
+        sy = fmaf(f, dy, sy);
+        sz = fmaf(f, dz, sz);
+      }
+    }
+    if (ACC == ACC_TILE_KAHAN) {
+      kahan_add(sx, cx, tx);
+      kahan_add(sy, cy, ty);
+      kahan_add(sz, cz, tz);
+    }
+  }
+  if (i < n) {
+    double* o = out + (size_t)s * 3 * n;
+    o[i] = (double)sx - (double)cx;
+    o[n + i] = (double)sy - (double)cy;
+    o[2 * n + i] = (double)sz - (double)cz;
+  }
+}
+
+// Sum S slice partials (and the optional FP64 1PN term) in slice order, then
+// scale by G: a[c][i] = G * sum_s part[s][c][i] + pn[c][i].
+__global__ void k_combine(int n, int S, const double* __restrict__ part, const double* __restrict__ pn, double G,
+                          double* __restrict__ a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double s = 0.0;
+    for (int k = 0; k < S; ++k) s += part[((size_t)k * 3 + c) * n + i];
+    double r = G * s;
+    if (pn) r += pn[(size_t)c * n + i];
+    a[(size_t)c * n + i] = r;
+  }
+}
+
+// ---------------------------------------------------------------- FP64 ----
+
+// get_pn1 (tool_chain_impl.cpp:110-118) with the caller's -norm (:133).
+__device__ __forceinline__ void pn1_term(double m1, double m2, double v1x, double v1y, double v1z, double v2x,
+                                         double v2y, double v2z, double inv_r, double nx, double ny, double nz,
+                                         double G, double& ax, double& ay, double& az) {
+#pragma clang fp contract(off)
+  const double v2_sq = v2x * v2x + v2y * v2y + v2z * v2z;
+  const double v1_sq = v1x * v1x + v1y * v1y + v1z * v1z;
+  const double dot4 = (4.0 * v1x) * v2x + (4.0 * v1y) * v2y + (4.0 * v1z) * v2z;
+  const double nv2 = nx * v2x + ny * v2y + nz * v2z;
+  double s = -v1_sq - 2.0 * v2_sq;
+  s = s + dot4;
+  s = s + 1.5 * (nv2 * nv2);
+  s = s + 5.0 * G * m1 * inv_r;
+  s = s + 4.0 * G * m2 * inv_r;
+  ax = nx * s;
+  ay = ny * s;
+  az = nz * s;
+  const double n4v1 = (nx * 4.0) * v1x + (ny * 4.0) * v1y + (nz * 4.0) * v1z;
+  const double n3v2 = (nx * 3.0) * v2x + (ny * 3.0) * v2y + (nz * 3.0) * v2z;
+  const double t = n4v1 - n3v2;
+  ax = ax + (v1x - v2x) * t;
+  ay = ay + (v1y - v2y) * t;
+  az = az + (v1z - v2z) * t;
+  ax = ax * G * m2 * inv_r * inv_r;
+  ay = ay * G * m2 * inv_r * inv_r;
+  az = az * G * m2 * inv_r * inv_r;
+}
+
+// MODE 0: Newton (+1PN if PN); MODE 1: only the 1PN term (added to an FP32
+// Newtonian result). x, v: SoA planes [3][n].
+template <bool PN, int MODE>
+__global__ __launch_bounds__(256) void k_allpairs_f64(int n, int lower, int upper, const double* __restrict__ x,
+                                                      const double* __restrict__ v, const double* __restrict__ m,
+                                                      double G, double ic2, double soft, double* __restrict__ a) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ii = i < n ? i : n - 1;
+  const double p1x = x[ii], p1y = x[n + ii], p1z = x[2 * n + ii];
+  const double m1 = m[ii];
+  double v1x = 0.0, v1y = 0.0, v1z = 0.0;
+  if (PN) {
+    v1x = v[ii];
+    v1y = v[n + ii];
+    v1z = v[2 * n + ii];
+  }
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  for (int j = lower; j < upper; ++j) {
+    const double p2x = x[j], p2y = x[n + j], p2z = x[2 * n + j];
+    const double m2 = m[j];
+    if (j == ii) continue;
+    const double dx = p2x - p1x, dy = p2y - p1y, dz = p2z - p1z;
+    const double r2 = dx * dx + dy * dy + dz * dz + soft;
+    const double inv_r = 1.0 / sqrt(r2);
+    const double inv_r2 = inv_r * inv_r;
+    const double nx = dx * inv_r, ny = dy * inv_r, nz = dz * inv_r;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (MODE == 0) {
+      const double scal = G * m2 * m1 * inv_r2;
+      fx = nx * scal;
+      fy = ny * scal;
+      fz = nz * scal;
+    }
+    if (PN) {
+      const double v2x = v[j], v2y = v[n + j], v2z = v[2 * n + j];
+      double px, py, pz;
+      pn1_term(m1, m2, v1x, v1y, v1z, v2x, v2y, v2z, inv_r, -nx, -ny, -nz, G, px, py, pz);
+      fx = fx + (px * ic2) * m1;
+      fy = fy + (py * ic2) * m1;
+      fz = fz + (pz * ic2) * m1;
+    }
+    ax += fx / m1;
+    ay += fy / m1;
+    az += fz / m1;
+  }
+  if (i < n) {
+    a[i] = ax;
+    a[n + i] = ay;
+    a[2 * n + i] = az;
+  }
+}
+
+// --------------------------------------------------------------- launch ----
+
+int choose_split(aster_ctx* c) {
+  if (c->P.split > 0) return c->P.split;
+  if (c->tuned_split > 0) return c->tuned_split;
+  // Aim for about 8 waves per SIMD (256 CUs x 4 SIMDs) while keeping each
+  // slice at least 1024 sources.
+  const int64_t waves = c->npad / 64;
+  int64_t S = (8 * 1024 + waves - 1) / waves;
+  const int64_t max_by_len = c->npad / 1024;
+  if (S > max_by_len) S = max_by_len;
+  if (S < 1) S = 1;
+  if (S > 64) S = 64;
+  return (int)S;
+}
+
+int launch_allpairs(aster_ctx* c, int method) {
+  const int n = (int)c->n;
+  const int npad = (int)c->npad;
+  const int lower = c->P.lower < 0 ? 0 : c->P.lower;
+  const int upper = (c->P.upper < 0 || c->P.upper > n) ? n : c->P.upper;
+  if (lower > upper) return fail(c, ASTER_ERR_INVALID, "source bounds: lower > upper");
+  const bool pn = (c->P.pn_flags & ASTER_PN1) != 0;
+  const double ic2 = (1.0 / c->P.c) * (1.0 / c->P.c);
+  if (method == ASTER_ALLPAIRS_FP64) {
+    dim3 grid((n + 255) / 256);
+    phase_begin(c, ASTER_PHASE_ALLPAIRS);
+    if (pn)
+      hipLaunchKernelGGL((k_allpairs_f64<true, 0>), grid, dim3(256), 0, c->stream, n, lower, upper, c->x.p, c->v.p,
+                         c->m.p, c->P.G, ic2, c->P.softening, c->a.p);
+    else
+      hipLaunchKernelGGL((k_allpairs_f64<false, 0>), grid, dim3(256), 0, c->stream, n, lower, upper, c->x.p, c->v.p,
+                         c->m.p, c->P.G, ic2, c->P.softening, c->a.p);
+    ASTER_HIP(c, hipGetLastError());
+    phase_end(c, ASTER_PHASE_ALLPAIRS);
+    return ASTER_OK;
+  }
+  if (method != ASTER_ALLPAIRS_FP32 && method != ASTER_ALLPAIRS_FP32_KAHAN)
+    return fail(c, ASTER_ERR_INVALID, "unknown all-pairs method");
+  if (lower != 0 || upper != n)
+    return fail(c, ASTER_ERR_INVALID, "FP32 all-pairs supports only the full source range; use ASTER_ALLPAIRS_FP64");
+  // Padded source array: FP32 for the plain method, FP64 for the mixed one.
+  const bool src64 = method == ASTER_ALLPAIRS_FP32_KAHAN;
+  phase_begin(c, ASTER_PHASE_TILES);
+  if (src64) {
+    ASTER_HIP(c, c->src64.reserve((size_t)npad));
+    hipLaunchKernelGGL(k_src64, dim3(npad / 256), dim3(256), 0, c->stream, n, npad, c->x.p, c->m.p, c->src64.p);
+  } else {
+    hipLaunchKernelGGL(k_src32, dim3(npad / 256), dim3(256), 0, c->stream, n, npad, c->x.p, c->m.p, c->sh.p);
+  }
+  ASTER_HIP(c, hipGetLastError());
+  phase_end(c, ASTER_PHASE_TILES);
+  // Slices over the padded source range (padding bodies have zero mass).
+  int S = choose_split(c);
+  int64_t slice = round_up((npad + S - 1) / S, kApTile);
+  S = (int)((npad + slice - 1) / slice);
+  ASTER_HIP(c, c->apart.reserve((size_t)S * 3 * n));
+  const float eps2 = (float)c->P.softening;
+  const bool mask = !(c->P.softening > 0.0);
+  dim3 grid(npad / kApBlock, S);
+  double* out = c->apart.p;
+  phase_begin(c, ASTER_PHASE_ALLPAIRS);
+  if (src64) {
+    if (mask)
+      hipLaunchKernelGGL((k_allpairs_f32<ACC_TILE_KAHAN, true, true>), grid, dim3(kApBlock), 0, c->stream,
+                         c->src64.p, c->x.p, n, npad, (int)slice, eps2, out);
+    else
+      hipLaunchKernelGGL((k_allpairs_f32<ACC_TILE_KAHAN, false, true>), grid, dim3(kApBlock), 0, c->stream,
+                         c->src64.p, c->x.p, n, npad, (int)slice, eps2, out);
+  } else {
+    if (mask)
+      hipLaunchKernelGGL((k_allpairs_f32<ACC_PLAIN, true, false>), grid, dim3(kApBlock), 0, c->stream, c->sh.p,
+                         c->x.p, n, npad, (int)slice, eps2, out);
+    else
+      hipLaunchKernelGGL((k_allpairs_f32<ACC_PLAIN, false, false>), grid, dim3(kApBlock), 0, c->stream, c->sh.p,
+                         c->x.p, n, npad, (int)slice, eps2, out);
+  }
+  ASTER_HIP(c, hipGetLastError());
+  phase_end(c, ASTER_PHASE_ALLPAIRS);
+  const double* pnp = nullptr;
+  if (pn) {
+    ASTER_HIP(c, c->apn.reserve((size_t)3 * n));
+    hipLaunchKernelGGL((k_allpairs_f64<true, 1>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, 0, n, c->x.p,
+                       c->v.p, c->m.p, c->P.G, ic2, c->P.softening, c->apn.p);
+    ASTER_HIP(c, hipGetLastError());
+    pnp = c->apn.p;
+  }
+  phase_begin(c, ASTER_PHASE_COMBINE);
+  hipLaunchKernelGGL(k_combine, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, S, c->apart.p, pnp, c->P.G,
+                     c->a.p);
+  ASTER_HIP(c, hipGetLastError());
+  phase_end(c, ASTER_PHASE_COMBINE);
+  return ASTER_OK;
+}
+
+}  // namespace aster

@@ -1,0 +1,496 @@
+// C ABI of libaster_hip: context lifetime, state transfer, dispatch of force
+// methods and integrators, diagnostics and timers.
+//
+// Replaces the reference's OpenCL glue (src/utils/tc_impl_cl.cpp,
+// src/utils/kernel_uploaders.cpp) and the global device state of
+// include/Aster/building-api/GPU_endpoint.h:25-31 with one context object per
+// stream; errors are returned, never exit()ed.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace aster {
+
+int fail(aster_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_fail(aster_ctx* c, hipError_t e, const char* what) {
+  if (c) c->err = std::string("HIP error: ") + hipGetErrorString(e) + " (" + what + ")";
+  return ASTER_ERR_HIP;
+}
+
+static thread_local std::string g_create_err = "no error";
+
+static int check_flags(aster_ctx* c) {
+  uint32_t f[4] = {0, 0, 0, 0};
+  ASTER_HIP(c, hipMemcpyAsync(f, c->flags.p, sizeof(f), hipMemcpyDeviceToHost, c->stream));
+  ASTER_HIP(c, hipStreamSynchronize(c->stream));
+  if (f[0] == 0) return ASTER_OK;
+  int rc = launch_reset_flags(c);
+  if (rc) return rc;
+  if (f[0] & FLAG_TREE) return fail(c, ASTER_ERR_STATE, "tree traversal exceeded its bound (corrupt tree)");
+  if (f[0] & FLAG_OUT_OF_BOX)
+    return fail(c, ASTER_ERR_OUT_OF_BOX, "a body lies outside the Morton box (set aster_params.box or use the auto box)");
+  return fail(c, ASTER_ERR_NONFINITE, "non-finite position or velocity produced");
+}
+
+static bool is_allpairs(int m) {
+  return m == ASTER_ALLPAIRS_FP32 || m == ASTER_ALLPAIRS_FP32_KAHAN || m == ASTER_ALLPAIRS_FP64;
+}
+static bool is_bh(int m) { return m == ASTER_BH_PARITY || m == ASTER_BH_GEOMETRIC || m == ASTER_BH_GROUP; }
+
+int compute_forces(aster_ctx* c, int method) {
+  int rc;
+  if (is_allpairs(method))
+    rc = launch_allpairs(c, method);
+  else if (is_bh(method))
+    rc = bh_forces(c, method);
+  else
+    return fail(c, ASTER_ERR_INVALID, "unknown force method");
+  if (rc) return rc;
+  c->acc_valid = true;
+  c->acc_method = method;
+  return ASTER_OK;
+}
+
+static void soa_to_aos(int64_t n, const std::vector<double>& soa, double* aos) {
+  for (int64_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) aos[3 * i + k] = soa[(size_t)k * n + i];
+}
+
+static int download3(aster_ctx* c, const double* dev, double* host_aos) {
+  const int64_t n = c->n;
+  std::vector<double> tmp((size_t)3 * n);
+  ASTER_HIP(c, hipMemcpyAsync(tmp.data(), dev, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  ASTER_HIP(c, hipStreamSynchronize(c->stream));
+  soa_to_aos(n, tmp, host_aos);
+  return ASTER_OK;
+}
+
+}  // namespace aster
+
+using namespace aster;
+
+extern "C" {
+
+int aster_version(void) { return 1 * 10000 + 0 * 100 + 0; }
+
+int aster_create(aster_ctx** out, int device) {
+  if (!out) {
+    g_create_err = "out pointer is NULL";
+    return ASTER_ERR_INVALID;
+  }
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count < 1) {
+    g_create_err = std::string("no HIP device: ") + hipGetErrorString(e);
+    return ASTER_ERR_HIP;
+  }
+  if (device < 0 || device >= count) {
+    g_create_err = "device index out of range";
+    return ASTER_ERR_INVALID;
+  }
+  e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    g_create_err = std::string("hipSetDevice: ") + hipGetErrorString(e);
+    return ASTER_ERR_HIP;
+  }
+  aster_ctx* c = new aster_ctx();
+  c->device = device;
+  c->P.G = 6e-11;
+  c->P.c = 299792458.0;
+  c->P.softening = 1e-14;
+  c->P.theta = 0.2;
+  c->P.pn_flags = ASTER_NEWTON;
+  c->P.precision = ASTER_FP32;
+  c->P.lower = 0;
+  c->P.upper = -1;
+  c->P.split = 0;
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  for (int k = 0; k < 16 && e == hipSuccess; ++k) e = hipEventCreate(&c->ev[k]);
+  if (e == hipSuccess) e = c->flags.reserve(4);
+  if (e == hipSuccess) e = c->boxd.reserve(8);
+  if (e == hipSuccess) e = hipMemsetAsync(c->flags.p, 0, 4 * sizeof(uint32_t), c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(c->boxd.p, 0, 8 * sizeof(double), c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    g_create_err = std::string("context setup: ") + hipGetErrorString(e);
+    aster_destroy(c);
+    return ASTER_ERR_HIP;
+  }
+  c->err = "no error";
+  *out = c;
+  return ASTER_OK;
+}
+
+int aster_destroy(aster_ctx* c) {
+  if (!c) return ASTER_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->batch.exec) (void)hipGraphExecDestroy(c->batch.exec);
+  if (c->batch.graph) (void)hipGraphDestroy(c->batch.graph);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->prof_pool)
+    if (e) (void)hipEventDestroy(e);
+  c->x.release(); c->v.release(); c->m.release(); c->a.release(); c->sh.release(); c->src64.release();
+  c->apart.release(); c->apn.release(); c->ework.release(); c->eout.release(); c->flags.release();
+  c->boxbits.release(); c->boxd.release();
+  c->codes.release(); c->idx.release(); c->codes2.release(); c->idx2.release(); c->hist.release();
+  c->left.release(); c->right.release(); c->parent.release(); c->escape.release(); c->counter.release();
+  c->depth.release(); c->nmass.release(); c->ncom.release(); c->nvel.release(); c->nbox.release();
+  c->nrec.release(); c->nlink.release(); c->bh_counters.release(); c->gsh.release();
+  c->batch.offsets.release(); c->batch.x.release(); c->batch.v.release(); c->batch.m.release();
+  c->batch.e.release(); c->batch.L.release(); c->batch.order.release();
+  auto& A = c->adj;
+  A.ckpt_x.release(); A.ckpt_v.release(); A.x0.release(); A.v0.release(); A.lx.release(); A.lv.release();
+  A.lm.release(); A.w.release(); A.snapg.release(); A.aprev.release(); A.part.release();
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return ASTER_OK;
+}
+
+const char* aster_last_error(const aster_ctx* c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+int aster_set_params(aster_ctx* c, const aster_params* p) {
+  if (!c || !p) return ASTER_ERR_INVALID;
+  if (!(p->softening >= 0.0) || !(p->theta >= 0.0) || !(p->c > 0.0) || !std::isfinite(p->G))
+    return fail(c, ASTER_ERR_INVALID, "invalid G, c, softening or theta");
+  if (p->precision != ASTER_FP32 && p->precision != ASTER_FP64)
+    return fail(c, ASTER_ERR_INVALID, "precision must be ASTER_FP32 or ASTER_FP64");
+  if ((p->pn_flags & ~(ASTER_NEWTON | ASTER_PN1)) != 0)
+    return fail(c, ASTER_ERR_INVALID, "only ASTER_NEWTON and ASTER_PN1 force terms are provided");
+  const bool any = p->box[0] != 0.0 || p->box[1] != 0.0 || p->box[2] != 0.0;
+  if (any && !(p->box[0] > 0.0 && p->box[1] > 0.0 && p->box[2] > 0.0))
+    return fail(c, ASTER_ERR_INVALID, "box half-extents must all be > 0 (or all 0 for the automatic box)");
+  if (p->split < 0 || p->split > 1024) return fail(c, ASTER_ERR_INVALID, "split out of range");
+  ASTER_HIP(c, hipSetDevice(c->device));
+  c->P = *p;
+  if (any) {
+    double ub[3] = {p->box[0], p->box[1], p->box[2]};
+    ASTER_HIP(c, hipMemcpyAsync(c->boxd.p + 4, ub, sizeof(ub), hipMemcpyHostToDevice, c->stream));
+    ASTER_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  c->acc_valid = false;
+  c->tree_valid = false;
+  return ASTER_OK;
+}
+
+int aster_get_params(const aster_ctx* c, aster_params* p) {
+  if (!c || !p) return ASTER_ERR_INVALID;
+  *p = c->P;
+  return ASTER_OK;
+}
+
+int aster_set_state(aster_ctx* c, int64_t n, const double* pos, const double* vel, const double* mass) {
+  if (!c) return ASTER_ERR_INVALID;
+  if (n < 1 || !pos || !mass) return fail(c, ASTER_ERR_INVALID, "need n >= 1, positions and masses");
+  if (n > (int64_t(1) << 28)) return fail(c, ASTER_ERR_CAPACITY, "at most 2^28 bodies per context");
+  for (int64_t i = 0; i < n; ++i) {
+    if (!(mass[i] > 0.0) || !std::isfinite(mass[i]))
+      return fail(c, ASTER_ERR_INVALID, "masses must be finite and > 0 (accelerations divide by m_i)");
+    for (int k = 0; k < 3; ++k)
+      if (!std::isfinite(pos[3 * i + k]) || (vel && !std::isfinite(vel[3 * i + k])))
+        return fail(c, ASTER_ERR_NONFINITE, "non-finite position or velocity in input");
+  }
+  ASTER_HIP(c, hipSetDevice(c->device));
+  const int64_t npad = round_up(n, kApBlock);
+  ASTER_HIP(c, c->x.reserve(3 * n));
+  ASTER_HIP(c, c->v.reserve(3 * n));
+  ASTER_HIP(c, c->a.reserve(3 * n));
+  ASTER_HIP(c, c->m.reserve(n));
+  ASTER_HIP(c, c->sh.reserve(npad));
+  std::vector<double> xs((size_t)3 * n), vs((size_t)3 * n, 0.0);
+  for (int64_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      xs[(size_t)k * n + i] = pos[3 * i + k];
+      if (vel) vs[(size_t)k * n + i] = vel[3 * i + k];
+    }
+  ASTER_HIP(c, hipStreamSynchronize(c->stream));
+  ASTER_HIP(c, hipMemcpyAsync(c->x.p, xs.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  ASTER_HIP(c, hipMemcpyAsync(c->v.p, vs.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  ASTER_HIP(c, hipMemcpyAsync(c->m.p, mass, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  ASTER_HIP(c, hipMemsetAsync(c->a.p, 0, 3 * n * sizeof(double), c->stream));
+  c->n = n;
+  c->npad = npad;
+  int rc = launch_reset_flags(c);
+  if (rc) return rc;
+  ASTER_HIP(c, hipStreamSynchronize(c->stream));
+  c->have_state = true;
+  c->acc_valid = false;
+  c->tree_valid = false;
+  c->adj.ready = false;
+  return ASTER_OK;
+}
+
+int aster_get_state(aster_ctx* c, double* pos, double* vel) {
+  if (!c) return ASTER_ERR_INVALID;
+  if (!c->have_state) return fail(c, ASTER_ERR_STATE, "no state loaded");
+  ASTER_HIP(c, hipSetDevice(c->device));
+  int rc;
+  if (pos && (rc = download3(c, c->x.p, pos))) return rc;
+  if (vel && (rc = download3(c, c->v.p, vel))) return rc;
+  return check_flags(c);
+}
+
+int64_t aster_num_bodies(const aster_ctx* c) { return c ? c->n : 0; }
+
+int aster_forces(aster_ctx* c, int method, double* acc) {
+  if (!c) return ASTER_ERR_INVALID;
+  if (!c->have_state) return fail(c, ASTER_ERR_STATE, "no state loaded");
+  ASTER_HIP(c, hipSetDevice(c->device));
+  int rc = compute_forces(c, method);
+  if (rc) return rc;
+  if (acc) {
+    if ((rc = download3(c, c->a.p, acc))) return rc;
+    return check_flags(c);
+  }
+  return ASTER_OK;
+}
+
+int aster_get_acc(aster_ctx* c, double* acc) {
+  if (!c || !acc) return ASTER_ERR_INVALID;
+  if (!c->acc_valid) return fail(c, ASTER_ERR_STATE, "no accelerations computed yet");
+  ASTER_HIP(c, hipSetDevice(c->device));
+  return download3(c, c->a.p, acc);
+}
+
+int aster_yoshida_weights(double* w) {
+  if (!w) return ASTER_ERR_INVALID;
+  // Yoshida (1990): w1 = 1 / (2 - 2^(1/3)), w0 = 1 - 2 w1, evaluated in FP64
+  // from the correctly rounded cube root 0x1.428a2f98d728bp+0. Literals, so
+  // that no libm cbrt (whose last bit differs between toolchains) enters.
+  w[0] = 0x1.59e8b6eb96339p+0;
+  w[1] = -0x1.b3d16dd72c672p+0;
+  w[2] = 0x1.59e8b6eb96339p+0;
+  return ASTER_OK;
+}
+
+int aster_step(aster_ctx* c, int method, int integrator, double dt, int64_t nsteps) {
+  if (!c) return ASTER_ERR_INVALID;
+  if (!c->have_state) return fail(c, ASTER_ERR_STATE, "no state loaded");
+  if (integrator != ASTER_LEAPFROG && integrator != ASTER_YOSHIDA4)
+    return fail(c, ASTER_ERR_INVALID, "integrator must be ASTER_LEAPFROG or ASTER_YOSHIDA4");
+  if (!std::isfinite(dt) || nsteps < 0) return fail(c, ASTER_ERR_INVALID, "invalid dt or nsteps");
+  if (nsteps == 0) return ASTER_OK;
+  ASTER_HIP(c, hipSetDevice(c->device));
+  double w[3] = {1.0, 0.0, 0.0};
+  int nsub = 1;
+  if (integrator == ASTER_YOSHIDA4) {
+    aster_yoshida_weights(w);
+    nsub = 3;
+  }
+  int rc;
+  if (!c->acc_valid || c->acc_method != method)
+    if ((rc = compute_forces(c, method))) return rc;
+  double pending = 0.0;
+  for (int64_t s = 0; s < nsteps; ++s)
+    for (int k = 0; k < nsub; ++k) {
+      const double h = (integrator == ASTER_YOSHIDA4) ? w[k] * dt : dt;
+      if ((rc = launch_kick_drift(c, pending, h))) return rc;
+      if ((rc = compute_forces(c, method))) return rc;
+      pending = h;
+    }
+  if ((rc = launch_kick(c, pending))) return rc;
+  c->adj.ready = false;
+  return ASTER_OK;
+}
+
+int aster_energy(aster_ctx* c, double* e_ref, double* e_plummer, double* L) {
+  if (!c) return ASTER_ERR_INVALID;
+  if (!c->have_state) return fail(c, ASTER_ERR_STATE, "no state loaded");
+  ASTER_HIP(c, hipSetDevice(c->device));
+  double s[6];
+  int rc = launch_energy(c, s);
+  if (rc) return rc;
+  if (e_ref) *e_ref = s[0] - 0.5 * s[1];
+  if (e_plummer) *e_plummer = s[0] - 0.5 * s[2];
+  if (L) {
+    L[0] = s[3];
+    L[1] = s[4];
+    L[2] = s[5];
+  }
+  return check_flags(c);
+}
+
+int aster_tree_dump(aster_ctx* c, uint32_t* codes, uint32_t* perm, int32_t* left, int32_t* right, double* node_mass,
+                    double* node_com) {
+  if (!c) return ASTER_ERR_INVALID;
+  if (!c->tree_valid) return fail(c, ASTER_ERR_STATE, "no Barnes-Hut tree built yet");
+  ASTER_HIP(c, hipSetDevice(c->device));
+  const int64_t n = c->n, nn = 2 * n - 1;
+  auto cp = [&](void* dst, const void* src, size_t bytes) -> int {
+    ASTER_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    return ASTER_OK;
+  };
+  int rc = 0;
+  if (codes && (rc = cp(codes, c->codes.p, n * sizeof(uint32_t)))) return rc;
+  if (perm && (rc = cp(perm, c->idx.p, n * sizeof(uint32_t)))) return rc;
+  if (left && (rc = cp(left, c->left.p, nn * sizeof(int32_t)))) return rc;
+  if (right && (rc = cp(right, c->right.p, nn * sizeof(int32_t)))) return rc;
+  if (node_mass && (rc = cp(node_mass, c->nmass.p, nn * sizeof(double)))) return rc;
+  std::vector<double> com;
+  if (node_com) {
+    com.resize((size_t)3 * nn);
+    if ((rc = cp(com.data(), c->ncom.p, 3 * nn * sizeof(double)))) return rc;
+  }
+  ASTER_HIP(c, hipStreamSynchronize(c->stream));
+  if (node_com)
+    for (int64_t k = 0; k < nn; ++k)
+      for (int d = 0; d < 3; ++d) node_com[3 * k + d] = com[(size_t)d * nn + k];
+  return ASTER_OK;
+}
+
+int aster_morton(aster_ctx* c, uint32_t* codes, double* box_out) {
+  if (!c || !codes) return ASTER_ERR_INVALID;
+  if (!c->have_state) return fail(c, ASTER_ERR_STATE, "no state loaded");
+  ASTER_HIP(c, hipSetDevice(c->device));
+  int rc = bh_morton_only(c);
+  if (rc) return rc;
+  ASTER_HIP(c, hipMemcpyAsync(codes, c->codes.p, c->n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  double b[4];
+  ASTER_HIP(c, hipMemcpyAsync(b, c->boxd.p, sizeof(b), hipMemcpyDeviceToHost, c->stream));
+  ASTER_HIP(c, hipStreamSynchronize(c->stream));
+  if (box_out) {
+    box_out[0] = b[0];
+    box_out[1] = b[1];
+    box_out[2] = b[2];
+  }
+  c->tree_valid = false;  // the sort buffers now hold unsorted codes
+  return check_flags(c);
+}
+
+int aster_bh_stats(aster_ctx* c, double* interactions, double* node_visits) {
+  if (!c) return ASTER_ERR_INVALID;
+  if (!c->bh_counters.p) return fail(c, ASTER_ERR_STATE, "no Barnes-Hut evaluation yet");
+  ASTER_HIP(c, hipSetDevice(c->device));
+  unsigned long long v[BHC_COUNT];
+  ASTER_HIP(c, hipMemcpyAsync(v, c->bh_counters.p, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+  ASTER_HIP(c, hipMemsetAsync(c->bh_counters.p, 0, sizeof(v), c->stream));
+  ASTER_HIP(c, hipStreamSynchronize(c->stream));
+  if (interactions) *interactions = (double)v[BHC_INTERACTIONS];
+  if (node_visits) *node_visits = (double)v[BHC_VISITS];
+  return ASTER_OK;
+}
+
+int aster_sync(aster_ctx* c) {
+  if (!c) return ASTER_ERR_INVALID;
+  ASTER_HIP(c, hipSetDevice(c->device));
+  ASTER_HIP(c, hipStreamSynchronize(c->stream));
+  return check_flags(c);
+}
+
+int aster_timer_mark(aster_ctx* c, int slot) {
+  if (!c || slot < 0 || slot >= 16) return ASTER_ERR_INVALID;
+  ASTER_HIP(c, hipEventRecord(c->ev[slot], c->stream));
+  return ASTER_OK;
+}
+
+int aster_timer_elapsed(aster_ctx* c, int a, int b, float* ms) {
+  if (!c || !ms || a < 0 || a >= 16 || b < 0 || b >= 16) return ASTER_ERR_INVALID;
+  ASTER_HIP(c, hipEventSynchronize(c->ev[b]));
+  ASTER_HIP(c, hipEventElapsedTime(ms, c->ev[a], c->ev[b]));
+  return ASTER_OK;
+}
+
+int aster_batch_set(aster_ctx* c, int64_t nsys, const int64_t* offsets, const double* pos, const double* vel,
+                    const double* mass);
+int aster_batch_get(aster_ctx* c, double* pos, double* vel);
+
+int aster_batch_step(aster_ctx* c, int integrator, double dt, int64_t nsteps, int32_t spl, int32_t use_graph) {
+  if (!c) return ASTER_ERR_INVALID;
+  ASTER_HIP(c, hipSetDevice(c->device));
+  return batch_step(c, integrator, dt, nsteps, spl, use_graph);
+}
+
+int aster_batch_energy(aster_ctx* c, double* e, double* L) {
+  if (!c) return ASTER_ERR_INVALID;
+  ASTER_HIP(c, hipSetDevice(c->device));
+  return batch_energy(c, e, L);
+}
+
+int aster_batch_autotune(aster_ctx* c, int integrator, double dt, int32_t* spb, int32_t* spl) {
+  if (!c) return ASTER_ERR_INVALID;
+  ASTER_HIP(c, hipSetDevice(c->device));
+  return batch_autotune(c, integrator, dt, spb, spl);
+}
+
+int aster_traj_forward(aster_ctx* c, double dt, int64_t nsteps, int64_t ckpt_every, int32_t nsnap,
+                       const int64_t* snap_steps, double* snap_pos) {
+  if (!c) return ASTER_ERR_INVALID;
+  if (!c->have_state) return fail(c, ASTER_ERR_STATE, "no state loaded");
+  ASTER_HIP(c, hipSetDevice(c->device));
+  return traj_forward(c, dt, nsteps, ckpt_every, nsnap, snap_steps, snap_pos);
+}
+
+int aster_traj_backward(aster_ctx* c, const double* grad_snap_pos, double* g_pos, double* g_vel, double* g_mass) {
+  if (!c) return ASTER_ERR_INVALID;
+  ASTER_HIP(c, hipSetDevice(c->device));
+  return traj_backward(c, grad_snap_pos, g_pos, g_vel, g_mass);
+}
+
+}  // extern "C"
+
+namespace aster {
+
+constexpr size_t kProfPairs = 8192;
+
+void phase_begin(aster_ctx* c, int phase) {
+  if (!c->profiling || c->prof_used >= kProfPairs) return;
+  if (c->prof_pool.size() < 2 * (c->prof_used + 1)) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    c->prof_pool.push_back(a);
+    c->prof_pool.push_back(b);
+  }
+  if (hipEventRecord(c->prof_pool[2 * c->prof_used], c->stream) != hipSuccess) return;
+  c->prof_open = phase;
+}
+
+void phase_end(aster_ctx* c, int phase) {
+  if (!c->profiling || c->prof_open != phase) return;
+  if (hipEventRecord(c->prof_pool[2 * c->prof_used + 1], c->stream) != hipSuccess) return;
+  if (c->prof_phase.size() <= c->prof_used) c->prof_phase.resize(c->prof_used + 1);
+  c->prof_phase[c->prof_used] = phase;
+  c->prof_used++;
+  c->prof_open = -1;
+}
+
+}  // namespace aster
+
+extern "C" {
+
+int aster_profile(aster_ctx* c, int enable) {
+  if (!c) return ASTER_ERR_INVALID;
+  c->profiling = enable != 0;
+  c->prof_used = 0;
+  c->prof_open = -1;
+  return ASTER_OK;
+}
+
+int aster_phase_times(aster_ctx* c, double* ms, int64_t* counts) {
+  if (!c || !ms) return ASTER_ERR_INVALID;
+  ASTER_HIP(c, hipSetDevice(c->device));
+  ASTER_HIP(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < ASTER_NUM_PHASES; ++k) {
+    ms[k] = 0.0;
+    if (counts) counts[k] = 0;
+  }
+  for (size_t i = 0; i < c->prof_used; ++i) {
+    float t = 0.f;
+    ASTER_HIP(c, hipEventElapsedTime(&t, c->prof_pool[2 * i], c->prof_pool[2 * i + 1]));
+    const int ph = c->prof_phase[i];
+    ms[ph] += t;
+    if (counts) counts[ph] += 1;
+  }
+  c->prof_used = 0;
+  return ASTER_OK;
+}
+
+}  // extern "C"

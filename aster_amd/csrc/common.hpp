@@ -1,0 +1,169 @@
+// Internal declarations shared by the HIP translation units of libaster_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/aster_hip.h"
+
+namespace aster {
+
+// Growable device allocation. Contents are not preserved on growth.
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t cap = 0;  // elements
+  hipError_t reserve(size_t n) {
+    if (n <= cap) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      if (e != hipSuccess) return e;
+      p = nullptr;
+      cap = 0;
+    }
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e == hipSuccess) cap = n;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+// Device-side error word bits (ctx->flags[0]).
+enum : uint32_t { FLAG_OUT_OF_BOX = 1u, FLAG_NONFINITE = 2u, FLAG_TREE = 4u };
+
+// Sizes of the Barnes-Hut counters block (ctx->bh_counters, u64).
+enum { BHC_INTERACTIONS = 0, BHC_VISITS = 1, BHC_COUNT = 4 };
+
+struct BatchState {
+  int64_t nsys = 0, nbodies = 0;
+  DBuf<int64_t> offsets;   // nsys+1
+  DBuf<double> x, v, m;    // AoS-d3 over bodies (x, v), m per body
+  DBuf<double> e, L;       // per-system energy / angular momentum
+  DBuf<int32_t> order;     // systems sorted by size (descending) for packing
+  int systems_per_block = 4;
+  int steps_per_launch = 0;
+  // Captured launch sequence.
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_steps = 0, graph_integrator = -1, graph_spl = 0, graph_spb = 0;
+  double graph_dt = 0.0;
+  int graph_pn = -1;
+  double graph_G = 0.0, graph_c = 0.0, graph_soft = -1.0;
+};
+
+struct AdjointState {
+  int64_t nsteps = 0, ckpt_every = 0;
+  int32_t nsnap = 0;
+  double dt = 0.0;
+  std::vector<int64_t> snap_steps;
+  DBuf<double> ckpt_x, ckpt_v;  // (nsteps/ckpt_every + 1) * 3n each
+  DBuf<double> x0, v0;          // initial state (3n each)
+  DBuf<double> lx, lv, lm;      // adjoint accumulators (3n, 3n, n)
+  DBuf<double> w;               // weights for the fused VJP (3n)
+  DBuf<double> snapg;           // snapshot gradients (nsnap * 3n)
+  DBuf<double> aprev;           // a(x_{s+1}) during the reverse sweep (3n)
+  DBuf<double> part;            // partial VJP outputs: slices * 7 * n
+  bool ready = false;
+};
+
+}  // namespace aster
+
+struct aster_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  aster_params P{};
+  int64_t n = 0;
+  int64_t npad = 0;  // n rounded up to the all-pairs block
+  bool have_state = false;
+  bool acc_valid = false;
+  int acc_method = -1;
+
+  // State, device resident. x, v, a: SoA planes [3][n]; m: [n].
+  aster::DBuf<double> x, v, m, a;
+  aster::DBuf<float4> sh;       // FP32 sources {x, y, z, m}, npad entries
+  aster::DBuf<double4> src64;   // FP64 sources {x, y, z, m}, npad entries
+  aster::DBuf<double> apart;    // all-pairs slice partials [S][3][n]
+  aster::DBuf<double> apn;      // FP64 1PN correction [3][n]
+  aster::DBuf<double> ework;    // energy work [6][n] + reduction scratch
+  aster::DBuf<double> eout;     // 8 doubles
+  aster::DBuf<uint32_t> flags;  // device error word(s), 4 words
+  aster::DBuf<unsigned long long> boxbits;  // auto box as ordered bit patterns (4)
+  aster::DBuf<double> boxd;     // box half-extent used (3) + |box| (1)
+
+  // Barnes-Hut / LBVH.
+  aster::DBuf<uint32_t> codes, idx, codes2, idx2;  // sort ping-pong, n
+  aster::DBuf<uint32_t> hist;                      // per-block digit counts
+  aster::DBuf<int32_t> left, right, parent, escape;// 2n-1
+  aster::DBuf<int32_t> counter;                    // n-1 arrival counters
+  aster::DBuf<int32_t> depth;                      // 2n-1
+  aster::DBuf<double> nmass, ncom, nvel;           // FP64 moments: 2n-1, [3][2n-1], [3][2n-1]
+  aster::DBuf<float> nbox;                         // FP32 AABB [6][2n-1]
+  aster::DBuf<float4> nrec;                        // FP32 node record: com, mass (2n-1)
+  aster::DBuf<int4> nlink;                         // right, escape, crit2 bits, leaf body (2n-1)
+  aster::DBuf<unsigned long long> bh_counters;     // BHC_COUNT
+  aster::DBuf<float4> gsh;                         // bodies in sorted order (FP32), n
+  bool tree_valid = false;
+  int tree_mac = -1;
+
+  // Timers.
+  hipEvent_t ev[16] = {};
+  // Phase profiling: event pairs recorded around launch phases.
+  bool profiling = false;
+  std::vector<hipEvent_t> prof_pool;       // created lazily, reused
+  std::vector<int> prof_phase;             // phase of each recorded pair
+  size_t prof_used = 0;                    // pairs in use
+  int prof_open = -1;                      // phase currently open
+
+  aster::BatchState batch;
+  aster::AdjointState adj;
+  int tuned_split = 0;
+};
+
+namespace aster {
+
+int fail(aster_ctx* c, int code, const std::string& msg);
+int hip_fail(aster_ctx* c, hipError_t e, const char* what);
+
+#define ASTER_HIP(ctx, expr)                              \
+  do {                                                    \
+    hipError_t _e = (expr);                               \
+    if (_e != hipSuccess) return ::aster::hip_fail((ctx), _e, #expr); \
+  } while (0)
+
+constexpr int kApBlock = 256;  // all-pairs targets per block
+constexpr int kApTile = 64;    // sources per accumulation tile
+
+inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+// Launchers (each returns an aster_status).
+int launch_allpairs(aster_ctx* c, int method);           // writes c->a
+int launch_kick_drift(aster_ctx* c, double dt_prev, double dt);
+int launch_kick(aster_ctx* c, double dt);
+int launch_shadow(aster_ctx* c);
+int launch_energy(aster_ctx* c, double* out6_host);
+int launch_reset_flags(aster_ctx* c);
+int launch_box(aster_ctx* c, bool user_box);             // fills c->boxd
+int bh_forces(aster_ctx* c, int method);                 // writes c->a
+int bh_morton_only(aster_ctx* c);
+int morton_sort(aster_ctx* c, bool user_box);            // c->codes sorted, c->idx = perm
+int batch_step(aster_ctx* c, int integrator, double dt, int64_t nsteps, int spl, int use_graph);
+int batch_energy(aster_ctx* c, double* e, double* L);
+int batch_autotune(aster_ctx* c, int integrator, double dt, int32_t* spb, int32_t* spl);
+int traj_forward(aster_ctx* c, double dt, int64_t nsteps, int64_t ckpt, int32_t nsnap, const int64_t* snaps,
+                 double* snap_pos);
+int traj_backward(aster_ctx* c, const double* gsnap, double* gx, double* gv, double* gm);
+int choose_split(aster_ctx* c);
+void phase_begin(aster_ctx* c, int phase);
+void phase_end(aster_ctx* c, int phase);
+
+}  // namespace aster

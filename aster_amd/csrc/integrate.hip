@@ -1,0 +1,212 @@
+// Kick/drift updates on the FP64 state, the FP32 shadow, energy and angular
+// momentum reductions, the automatic Morton box and device error flags.
+//
+// The kick/drift arithmetic keeps the operation order of update_leapfrog
+// (src/physics/tool_chain_impl.cpp:169-185):
+//   x += v*dt + ((0.5*a)*dt)*dt ;  v += (0.5*a)*dt
+// with contraction disabled, so given identical accelerations the device
+// trajectory is bit-identical to the CPU path. The second half-kick of a step
+// and the first half-kick of the next use the same accelerations, so they are
+// applied in one pass (k_kick_drift with dt_prev != 0).
+#include "common.hpp"
+
+namespace aster {
+
+__global__ __launch_bounds__(256) void k_kick_drift(int n, double* __restrict__ x, double* __restrict__ v,
+                                                    const double* __restrict__ a, double dt_prev, double dt,
+                                                    uint32_t* __restrict__ flags) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  bool finite = true;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t k = (size_t)c * n + i;
+    const double ac = a[k];
+    double vc = v[k];
+    if (dt_prev != 0.0) vc = vc + 0.5 * ac * dt_prev;
+    const double xc = x[k] + (vc * dt + 0.5 * ac * dt * dt);
+    vc = vc + 0.5 * ac * dt;
+    x[k] = xc;
+    v[k] = vc;
+    finite = finite && isfinite(xc) && isfinite(vc);
+  }
+  if (!finite) atomicOr(flags, (uint32_t)FLAG_NONFINITE);
+}
+
+__global__ __launch_bounds__(256) void k_kick(int n, double* __restrict__ v, const double* __restrict__ a,
+                                              double dt) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t k = (size_t)c * n + i;
+    v[k] = v[k] + 0.5 * a[k] * dt;
+  }
+}
+
+// FP32 shadow {x,y,z,m} over npad entries; padding bodies have zero mass.
+__global__ void k_shadow(int n, int npad, const double* __restrict__ x, const double* __restrict__ m,
+                         float4* __restrict__ sh) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npad) return;
+  if (i < n)
+    sh[i] = make_float4((float)x[i], (float)x[n + i], (float)x[2 * n + i], (float)m[i]);
+  else
+    sh[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Per-body energy terms: out[0][i] = kinetic, out[1][i] = sum_j G m_i m_j /
+// (|r| + soft), out[2][i] = sum_j G m_i m_j / sqrt(r^2 + soft), out[3..5][i] =
+// m x cross v. Pair sums run over all j != i (each pair counted twice).
+__global__ __launch_bounds__(256) void k_energy_terms(int n, const double* __restrict__ x,
+                                                      const double* __restrict__ v, const double* __restrict__ m,
+                                                      double G, double soft, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ii = i < n ? i : n - 1;
+  const double px = x[ii], py = x[n + ii], pz = x[2 * n + ii];
+  const double mi = m[ii];
+  double pref = 0.0, ppl = 0.0;
+  for (int j = 0; j < n; ++j) {
+    const double dx = x[j] - px, dy = x[n + j] - py, dz = x[2 * n + j] - pz;
+    const double mj = m[j];
+    if (j == ii) continue;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    pref += mj / (sqrt(r2) + soft);
+    ppl += mj / sqrt(r2 + soft);
+  }
+  if (i < n) {
+    const double vx = v[i], vy = v[n + i], vz = v[2 * n + i];
+    out[i] = 0.5 * mi * (vx * vx + vy * vy + vz * vz);
+    out[(size_t)n + i] = G * mi * pref;
+    out[(size_t)2 * n + i] = G * mi * ppl;
+    out[(size_t)3 * n + i] = mi * (py * vz - pz * vy);
+    out[(size_t)4 * n + i] = mi * (pz * vx - px * vz);
+    out[(size_t)5 * n + i] = mi * (px * vy - py * vx);
+  }
+}
+
+// Deterministic reduction of `rows` rows of length n: one block, fixed
+// strided order, fixed tree.
+__global__ __launch_bounds__(1024) void k_reduce_rows(int n, int rows, const double* __restrict__ in,
+                                                      double* __restrict__ out) {
+  __shared__ double sm[1024];
+  for (int r = 0; r < rows; ++r) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 1024) s += in[(size_t)r * n + i];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[r] = sm[0];
+    __syncthreads();
+  }
+}
+
+// Symmetric box: max |coordinate| per axis. Non-negative doubles order like
+// their bit patterns, so an integer atomicMax on the bits is exact and
+// order-independent.
+__global__ __launch_bounds__(256) void k_box(int n, const double* __restrict__ x,
+                                             unsigned long long* __restrict__ bits) {
+  double mx[3] = {0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) mx[c] = fmax(mx[c], fabs(x[(size_t)c * n + i]));
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double r = mx[c];
+    for (int off = 32; off > 0; off >>= 1) r = fmax(r, __shfl_xor(r, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(&bits[c], (unsigned long long)__double_as_longlong(r));
+  }
+}
+
+__global__ void k_box_finish(const unsigned long long* __restrict__ bits, const double* __restrict__ user,
+                             int use_user, double* __restrict__ box) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0) return;
+  double b[3];
+  for (int c = 0; c < 3; ++c) b[c] = use_user ? user[c] : __longlong_as_double((long long)bits[c]);
+  box[0] = b[0];
+  box[1] = b[1];
+  box[2] = b[2];
+  // bd_size = |bounding_box| (barnes_hut.cpp:405; vec3::magnitude, vectors.h:93-99)
+  box[3] = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+}
+
+__global__ void k_zero_u32(uint32_t* p, int k) {
+  if ((int)threadIdx.x < k) p[threadIdx.x] = 0u;
+}
+__global__ void k_zero_u64(unsigned long long* p, int k) {
+  if ((int)threadIdx.x < k) p[threadIdx.x] = 0ull;
+}
+
+// --------------------------------------------------------------- launch ----
+
+int launch_reset_flags(aster_ctx* c) {
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, c->flags.p, 4);
+  ASTER_HIP(c, hipGetLastError());
+  return ASTER_OK;
+}
+
+int launch_kick_drift(aster_ctx* c, double dt_prev, double dt) {
+  const int n = (int)c->n;
+  phase_begin(c, ASTER_PHASE_KICKDRIFT);
+  hipLaunchKernelGGL(k_kick_drift, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->x.p, c->v.p, c->a.p,
+                     dt_prev, dt, c->flags.p);
+  ASTER_HIP(c, hipGetLastError());
+  phase_end(c, ASTER_PHASE_KICKDRIFT);
+  return ASTER_OK;
+}
+
+int launch_kick(aster_ctx* c, double dt) {
+  const int n = (int)c->n;
+  hipLaunchKernelGGL(k_kick, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->v.p, c->a.p, dt);
+  ASTER_HIP(c, hipGetLastError());
+  return ASTER_OK;
+}
+
+int launch_shadow(aster_ctx* c) {
+  const int npad = (int)c->npad;
+  hipLaunchKernelGGL(k_shadow, dim3((npad + 255) / 256), dim3(256), 0, c->stream, (int)c->n, npad, c->x.p, c->m.p,
+                     c->sh.p);
+  ASTER_HIP(c, hipGetLastError());
+  return ASTER_OK;
+}
+
+int launch_energy(aster_ctx* c, double* out6) {
+  const int n = (int)c->n;
+  ASTER_HIP(c, c->ework.reserve((size_t)6 * n));
+  ASTER_HIP(c, c->eout.reserve(8));
+  hipLaunchKernelGGL(k_energy_terms, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->x.p, c->v.p, c->m.p,
+                     c->P.G, c->P.softening, c->ework.p);
+  ASTER_HIP(c, hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(1024), 0, c->stream, n, 6, c->ework.p, c->eout.p);
+  ASTER_HIP(c, hipGetLastError());
+  ASTER_HIP(c, hipMemcpyAsync(out6, c->eout.p, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  ASTER_HIP(c, hipStreamSynchronize(c->stream));
+  return ASTER_OK;
+}
+
+int launch_box(aster_ctx* c, bool user_box) {
+  const int n = (int)c->n;
+  ASTER_HIP(c, c->boxbits.reserve(4));
+  ASTER_HIP(c, c->boxd.reserve(8));
+  // A user box was uploaded to boxd[4..6] by aster_set_params.
+  const bool user = user_box && (c->P.box[0] != 0.0 || c->P.box[1] != 0.0 || c->P.box[2] != 0.0);
+  if (!user) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, c->boxbits.p, 4);
+    int blocks = (n + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(k_box, dim3(blocks), dim3(256), 0, c->stream, n, c->x.p, c->boxbits.p);
+    ASTER_HIP(c, hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_box_finish, dim3(1), dim3(64), 0, c->stream, c->boxbits.p, c->boxd.p + 4, user ? 1 : 0,
+                     c->boxd.p);
+  ASTER_HIP(c, hipGetLastError());
+  return ASTER_OK;
+}
+
+}  // namespace aster

@@ -1,0 +1,692 @@
+// LBVH Barnes-Hut on CDNA4: Morton codes -> LDS radix sort -> Karras radix
+// tree -> atomic bottom-up moments -> escape links -> stackless traversal.
+//
+// Tree layout follows the reference's NodesArray indexing
+// (src/solvers/barnes_hut.cpp:309-400): leaf k is the body at sorted position
+// k, the internal node of split i is n + i, the root is n.
+//
+// Parity notes:
+//  * Morton codes use the reference's FP64 normalisation and clamp
+//    (src/solvers/bh_specialization.cpp:38-46): bit-identical.
+//  * The sort is a stable LSD radix sort, i.e. the (code, index) order.
+//  * The Karras split uses the reference's tie-break on sorted positions
+//    (barnes_hut.cpp:340-342) and its search sequence: identical topology.
+//  * Moments: M = m_R + m_L, COM = (c_R m_R + c_L m_L) / M in FP64 without
+//    contraction (barnes_hut.cpp:145-150). Each node depends only on its two
+//    children, so the bottom-up atomic order cannot change the result: node
+//    masses and centres of mass are bit-identical to the recursive unite().
+//  * Traversal: the reference visits right children first (it pushes left,
+//    then right, barnes_hut.cpp:241-256). Opening goes to the right child and
+//    every accepted/skipped node continues at its escape link, which is the
+//    next node of that same order, so the FP64 traversal accumulates in the
+//    reference's order and is bit-identical to get_node_body (:202-261).
+#include "common.hpp"
+
+namespace aster {
+
+constexpr int kRsThreads = 256;
+constexpr int kRsTile = 2048;  // keys per block (8 per thread)
+
+// ------------------------------------------------------------ Morton -------
+
+__device__ __forceinline__ uint32_t part1by2(uint32_t v) {
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t cell_of(double p, double b, uint32_t* bad) {
+#pragma clang fp contract(off)
+  const double q = (p + b) / (2 * b);
+  const double s = q * (1 << 10);
+  // The reference converts without a lower clamp (undefined for q < 0): such
+  // inputs are reported instead of guessed.
+  if (!(q >= 0.0) || !(s < 4294967296.0)) *bad = 1u;
+  const uint32_t c = (uint32_t)s;
+  return c < 1023u ? c : 1023u;
+}
+
+__global__ __launch_bounds__(256) void k_morton(int n, const double* __restrict__ x, const double* __restrict__ box,
+                                                uint32_t* __restrict__ codes, uint32_t* __restrict__ idx,
+                                                uint32_t* __restrict__ flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t bad = 0;
+  const uint32_t ix = cell_of(x[i], box[0], &bad);
+  const uint32_t iy = cell_of(x[n + i], box[1], &bad);
+  const uint32_t iz = cell_of(x[2 * n + i], box[2], &bad);
+  codes[i] = (part1by2(iz) << 2) | (part1by2(iy) << 1) | part1by2(ix);
+  idx[i] = (uint32_t)i;
+  if (bad) atomicOr(flags, (uint32_t)FLAG_OUT_OF_BOX);
+}
+
+// -------------------------------------------------------- radix sort ------
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint32_t* __restrict__ keys, int n, int shift,
+                                                        uint32_t* __restrict__ hist, int nblocks) {
+  __shared__ uint32_t cnt[256];
+  const int t = threadIdx.x;
+  cnt[t] = 0;
+  __syncthreads();
+  const int base = blockIdx.x * kRsTile;
+#pragma unroll
+  for (int r = 0; r < kRsTile / kRsThreads; ++r) {
+    const int p = base + r * kRsThreads + t;
+    if (p < n) atomicAdd(&cnt[(keys[p] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(size_t)t * nblocks + blockIdx.x] = cnt[t];
+}
+
+// Exclusive prefix sum over `total` counts, one block.
+__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ hist, int total) {
+  __shared__ uint32_t part[1024];
+  const int t = threadIdx.x;
+  const int per = (total + 1023) / 1024;
+  const int b = t * per, e = min(b + per, total);
+  uint32_t s = 0;
+  for (int k = b; k < e; ++k) s += hist[k];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    uint32_t v = (t >= off) ? part[t - off] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[t] - s;  // exclusive
+  for (int k = b; k < e; ++k) {
+    const uint32_t v = hist[k];
+    hist[k] = run;
+    run += v;
+  }
+}
+
+// Stable scatter: rank of a key among equal digits = earlier rounds (run[]),
+// earlier waves of this round (wc[][]), earlier lanes of its wave (ballot
+// match mask).
+__global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t* __restrict__ kin,
+                                                           const uint32_t* __restrict__ vin,
+                                                           uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                           int n, int shift, const uint32_t* __restrict__ hist,
+                                                           int nblocks) {
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t gbase[256];
+  __shared__ uint32_t wc[4][256];
+  const int t = threadIdx.x;
+  const int lane = t & 63, w = t >> 6;
+  run[t] = 0;
+  gbase[t] = hist[(size_t)t * nblocks + blockIdx.x];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) wc[q][t] = 0;
+  __syncthreads();
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int base = blockIdx.x * kRsTile;
+  for (int r = 0; r < kRsTile / kRsThreads; ++r) {
+    const int p = base + r * kRsThreads + t;
+    const bool valid = p < n;
+    const uint32_t key = valid ? kin[p] : 0u;
+    const uint32_t val = valid ? vin[p] : 0u;
+    const uint32_t d = (key >> shift) & 255u;
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bal = __ballot(bit);
+      m &= bit ? bal : ~bal;
+    }
+    const uint32_t rank = (uint32_t)__popcll(m & lt);
+    if (valid && rank == 0) wc[w][d] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (valid) {
+      uint32_t off = run[d];
+      for (int q = 0; q < w; ++q) off += wc[q][d];
+      const uint32_t pos = gbase[d] + off + rank;
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    __syncthreads();
+    run[t] += wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) wc[q][t] = 0;
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------- Karras build ------
+
+__device__ __forceinline__ int kdelta(const uint32_t* __restrict__ codes, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  const uint32_t a = codes[i], b = codes[j];
+  if (a == b) return 32 + __clz((uint32_t)(i ^ j));
+  return __clz(a ^ b);
+}
+
+// Thread k initialises leaf k (body perm[k]) and, for k < n-1, the internal
+// node n+k. Also resets arrival counters and writes parent links.
+__global__ __launch_bounds__(256) void k_build(int n, const uint32_t* __restrict__ codes,
+                                               const uint32_t* __restrict__ perm, const double* __restrict__ x,
+                                               const double* __restrict__ v, const double* __restrict__ m,
+                                               int32_t* __restrict__ left, int32_t* __restrict__ right,
+                                               int32_t* __restrict__ parent, int32_t* __restrict__ counter,
+                                               double* __restrict__ nmass, double* __restrict__ ncom,
+                                               double* __restrict__ nvel, float* __restrict__ nbox,
+                                               float4* __restrict__ nrec, float4* __restrict__ gsh,
+                                               uint32_t* __restrict__ flags) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int nn = 2 * n - 1;
+  {
+    const uint32_t b = perm[k];
+    const double px = x[b], py = x[n + b], pz = x[2 * n + b], mb = m[b];
+    nmass[k] = mb;
+    ncom[k] = px;
+    ncom[nn + k] = py;
+    ncom[2 * nn + k] = pz;
+    nvel[k] = v[b];
+    nvel[nn + k] = v[n + b];
+    nvel[2 * nn + k] = v[2 * n + b];
+    const float fx = (float)px, fy = (float)py, fz = (float)pz;
+    nbox[k] = fx;
+    nbox[nn + k] = fy;
+    nbox[2 * nn + k] = fz;
+    nbox[3 * nn + k] = fx;
+    nbox[4 * nn + k] = fy;
+    nbox[5 * nn + k] = fz;
+    const float4 r = make_float4(fx, fy, fz, (float)mb);
+    nrec[k] = r;
+    gsh[k] = r;
+    left[k] = -1;
+    right[k] = -1;
+  }
+  if (k == 0) parent[n] = -1;
+  if (k >= n - 1) return;
+  const int i = k;
+  const int d_left = (i == 0) ? -1 : kdelta(codes, n, i, i - 1);
+  const int d_right = kdelta(codes, n, i, i + 1);
+  const int d = (d_right > d_left) ? 1 : -1;
+  const int d_min = (d == 1) ? d_left : d_right;
+  int l_max = 2;
+  while (kdelta(codes, n, i, i + l_max * d) > d_min) l_max *= 2;
+  int l = 0;
+  for (int t = l_max / 2; t >= 1; t /= 2)
+    if (kdelta(codes, n, i, i + (l + t) * d) > d_min) l += t;
+  const int j = i + l * d;
+  const int d_node = kdelta(codes, n, i, j);
+  int s = 0;
+  for (int t = (l + 1) / 2; t >= 1; t = (t + 1) / 2) {
+    if (kdelta(codes, n, i, i + (s + t) * d) > d_node) s += t;
+    if (t == 1) break;
+  }
+  const int gamma = i + s * d + min(d, 0);
+  const int lo = min(i, j), hi = max(i, j);
+  const int L = (lo == gamma) ? gamma : n + gamma;
+  const int R = (hi == gamma + 1) ? gamma + 1 : n + gamma + 1;
+  if (gamma < 0 || gamma > n - 2 || R >= nn) {  // cannot happen for a sorted key array
+    atomicOr(flags, (uint32_t)FLAG_TREE);
+    return;
+  }
+  left[n + i] = L;
+  right[n + i] = R;
+  parent[L] = n + i;
+  parent[R] = n + i;
+  counter[i] = 0;
+}
+
+// Bottom-up moments: each leaf climbs; the second arrival at a node combines
+// both children (acquire/release on the arrival counter orders the data).
+__global__ __launch_bounds__(256) void k_moments(int n, const int32_t* __restrict__ left,
+                                                 const int32_t* __restrict__ right, const int32_t* __restrict__ parent,
+                                                 int32_t* __restrict__ counter, double* __restrict__ nmass,
+                                                 double* __restrict__ ncom, double* __restrict__ nvel,
+                                                 float* __restrict__ nbox, float4* __restrict__ nrec,
+                                                 uint32_t* __restrict__ flags) {
+#pragma clang fp contract(off)
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n || n < 2) return;
+  const int nn = 2 * n - 1;
+  int node = parent[k];
+  for (int guard = 0; node >= 0; ++guard) {
+    if (guard > nn) {
+      atomicOr(flags, (uint32_t)FLAG_TREE);
+      return;
+    }
+    const int old = __hip_atomic_fetch_add(&counter[node - n], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0) return;
+    const int L = left[node], R = right[node];
+    const double mR = nmass[R], mL = nmass[L];
+    const double total = mR + mL;
+    double com[3], vel[3];
+    double mass;
+    if (total <= 0) {
+      mass = 0.0;
+      com[0] = com[1] = com[2] = 0.0;
+      vel[0] = vel[1] = vel[2] = 0.0;
+    } else {
+      mass = total;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        com[c] = (ncom[(size_t)c * nn + R] * mR + ncom[(size_t)c * nn + L] * mL) / mass;
+        vel[c] = (nvel[(size_t)c * nn + R] * mR + nvel[(size_t)c * nn + L] * mL) / mass;
+      }
+    }
+    nmass[node] = mass;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      ncom[(size_t)c * nn + node] = com[c];
+      nvel[(size_t)c * nn + node] = vel[c];
+      nbox[(size_t)c * nn + node] = fminf(nbox[(size_t)c * nn + L], nbox[(size_t)c * nn + R]);
+      nbox[(size_t)(3 + c) * nn + node] = fmaxf(nbox[(size_t)(3 + c) * nn + L], nbox[(size_t)(3 + c) * nn + R]);
+    }
+    nrec[node] = make_float4((float)com[0], (float)com[1], (float)com[2], (float)mass);
+    node = parent[node];
+  }
+}
+
+// Per node: depth, escape link of the right-first preorder and the opening
+// threshold crit2 (accept when d^2 > crit2).
+// mac 0 (parity): crit2 = (s / theta)^2 with s = |box| 2^-depth.
+// mac 1 (geometric): crit2 = (l / theta + delta)^2, l = longest AABB side,
+//                    delta = |COM - AABB centre|.
+__global__ __launch_bounds__(256) void k_links(int n, const int32_t* __restrict__ left,
+                                               const int32_t* __restrict__ right, const int32_t* __restrict__ parent,
+                                               const float* __restrict__ nbox, const float4* __restrict__ nrec,
+                                               const double* __restrict__ boxd, double theta, int mac,
+                                               int32_t* __restrict__ depth, int4* __restrict__ nlink,
+                                               const uint32_t* __restrict__ perm, uint32_t* __restrict__ flags) {
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nn = 2 * n - 1;
+  if (node >= nn) return;
+  int esc = -1, dep = 0;
+  int cur = node;
+  for (int guard = 0;; ++guard) {
+    const int p = parent[cur];
+    if (p < 0) break;
+    if (guard > nn) {
+      atomicOr(flags, (uint32_t)FLAG_TREE);
+      break;
+    }
+    if (esc < 0 && right[p] == cur) esc = left[p];
+    ++dep;
+    cur = p;
+  }
+  depth[node] = dep;
+  float crit2;
+  if (mac == 0) {
+    const double s = ldexp(boxd[3], -dep);
+    const double q = s / theta;
+    crit2 = (float)(q * q);
+  } else {
+    const float lx = nbox[(size_t)3 * nn + node] - nbox[node];
+    const float ly = nbox[(size_t)4 * nn + node] - nbox[(size_t)nn + node];
+    const float lz = nbox[(size_t)5 * nn + node] - nbox[(size_t)2 * nn + node];
+    const float l = fmaxf(lx, fmaxf(ly, lz));
+    const float4 r = nrec[node];
+    const float cx = 0.5f * (nbox[(size_t)3 * nn + node] + nbox[node]);
+    const float cy = 0.5f * (nbox[(size_t)4 * nn + node] + nbox[(size_t)nn + node]);
+    const float cz = 0.5f * (nbox[(size_t)5 * nn + node] + nbox[(size_t)2 * nn + node]);
+    const float ex = r.x - cx, ey = r.y - cy, ez = r.z - cz;
+    const float delta = sqrtf(ex * ex + ey * ey + ez * ez);
+    const float q = l / (float)theta + delta;
+    crit2 = q * q;
+  }
+  const int body = node < n ? (int)perm[node] : -1;
+  nlink[node] = make_int4(node < n ? -1 : right[node], esc, __float_as_int(crit2), body);
+}
+
+// --------------------------------------------------------- traversals ------
+
+// FP32 per-body stackless traversal (lanes = consecutive sorted bodies).
+template <bool MASK>
+__global__ __launch_bounds__(256) void k_bh_f32(int n, const float4* __restrict__ gsh, const float4* __restrict__ nrec,
+                                                const int4* __restrict__ nlink, float eps2, double G,
+                                                double* __restrict__ a, unsigned long long* __restrict__ counters,
+                                                uint32_t* __restrict__ flags) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = k < n;
+  const float4 me = gsh[live ? k : n - 1];
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  uint32_t inter = 0, visits = 0;
+  int node = live ? n : -1;
+  const uint32_t limit = 4u * (uint32_t)n + 16u;
+  while (node >= 0) {
+    if (visits > limit) {
+      atomicOr(flags, (uint32_t)FLAG_TREE);
+      break;
+    }
+    const float4 r = nrec[node];
+    const int4 L = nlink[node];
+    ++visits;
+    const float dx = r.x - me.x, dy = r.y - me.y, dz = r.z - me.z;
+    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    if (d2 == 0.f) {
+      node = L.y;
+      continue;
+    }
+    if (node < n || d2 > __int_as_float(L.z)) {
+      const float r2 = d2 + eps2;
+      float inv = __builtin_amdgcn_rsqf(r2);
+      if (MASK) inv = (r2 > 0.f) ? inv : 0.f;
+      const float f = (r.w * inv) * (inv * inv);
+      ax = fmaf(f, dx, ax);
+      ay = fmaf(f, dy, ay);
+      az = fmaf(f, dz, az);
+      ++inter;
+      node = L.y;
+    } else {
+      node = L.x;
+    }
+  }
+  if (live) {
+    const int b = nlink[k].w;
+    a[b] = G * (double)ax;
+    a[n + b] = G * (double)ay;
+    a[2 * n + b] = G * (double)az;
+  }
+  unsigned long long ci = inter, cv = visits;
+  for (int off = 32; off > 0; off >>= 1) {
+    ci += __shfl_xor(ci, off, 64);
+    cv += __shfl_xor(cv, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&counters[BHC_INTERACTIONS], ci);
+    atomicAdd(&counters[BHC_VISITS], cv);
+  }
+}
+
+// compute_gravitational (tool_chain_impl.cpp:120-137) in FP64, reference order.
+__device__ __forceinline__ void grav64(double m1, double m2, double v1x, double v1y, double v1z, double v2x,
+                                       double v2y, double v2z, double p1x, double p1y, double p1z, double p2x,
+                                       double p2y, double p2z, double G, double ic2, double soft, bool pn,
+                                       double& fx, double& fy, double& fz);
+
+__device__ __forceinline__ void pn1_term64(double m1, double m2, double v1x, double v1y, double v1z, double v2x,
+                                           double v2y, double v2z, double inv_r, double nx, double ny, double nz,
+                                           double G, double& ax, double& ay, double& az) {
+#pragma clang fp contract(off)
+  const double v2_sq = v2x * v2x + v2y * v2y + v2z * v2z;
+  const double v1_sq = v1x * v1x + v1y * v1y + v1z * v1z;
+  const double dot4 = (4.0 * v1x) * v2x + (4.0 * v1y) * v2y + (4.0 * v1z) * v2z;
+  const double nv2 = nx * v2x + ny * v2y + nz * v2z;
+  double s = -v1_sq - 2.0 * v2_sq;
+  s = s + dot4;
+  s = s + 1.5 * (nv2 * nv2);
+  s = s + 5.0 * G * m1 * inv_r;
+  s = s + 4.0 * G * m2 * inv_r;
+  ax = nx * s;
+  ay = ny * s;
+  az = nz * s;
+  const double n4v1 = (nx * 4.0) * v1x + (ny * 4.0) * v1y + (nz * 4.0) * v1z;
+  const double n3v2 = (nx * 3.0) * v2x + (ny * 3.0) * v2y + (nz * 3.0) * v2z;
+  const double t = n4v1 - n3v2;
+  ax = ax + (v1x - v2x) * t;
+  ay = ay + (v1y - v2y) * t;
+  az = az + (v1z - v2z) * t;
+  ax = ax * G * m2 * inv_r * inv_r;
+  ay = ay * G * m2 * inv_r * inv_r;
+  az = az * G * m2 * inv_r * inv_r;
+}
+
+__device__ __forceinline__ void grav64(double m1, double m2, double v1x, double v1y, double v1z, double v2x,
+                                       double v2y, double v2z, double p1x, double p1y, double p1z, double p2x,
+                                       double p2y, double p2z, double G, double ic2, double soft, bool pn,
+                                       double& fx, double& fy, double& fz) {
+#pragma clang fp contract(off)
+  const double dx = p2x - p1x, dy = p2y - p1y, dz = p2z - p1z;
+  const double r2 = dx * dx + dy * dy + dz * dz + soft;
+  const double inv_r = 1.0 / sqrt(r2);
+  const double inv_r2 = inv_r * inv_r;
+  const double nx = dx * inv_r, ny = dy * inv_r, nz = dz * inv_r;
+  const double scal = G * m2 * m1 * inv_r2;
+  fx = nx * scal;
+  fy = ny * scal;
+  fz = nz * scal;
+  if (pn) {
+    double px, py, pz;
+    pn1_term64(m1, m2, v1x, v1y, v1z, v2x, v2y, v2z, inv_r, -nx, -ny, -nz, G, px, py, pz);
+    fx = fx + (px * ic2) * m1;
+    fy = fy + (py * ic2) * m1;
+    fz = fz + (pz * ic2) * m1;
+  }
+}
+
+// FP64 per-body traversal in the reference's order and arithmetic
+// (get_node_body, barnes_hut.cpp:202-261).
+template <bool PN>
+__global__ __launch_bounds__(256) void k_bh_f64(int n, const double* __restrict__ x, const double* __restrict__ v,
+                                                const double* __restrict__ m, const double* __restrict__ nmass,
+                                                const double* __restrict__ ncom, const double* __restrict__ nvel,
+                                                const int4* __restrict__ nlink, const int32_t* __restrict__ depth,
+                                                const double* __restrict__ boxd, double theta, double G, double ic2,
+                                                double soft, double* __restrict__ a,
+                                                unsigned long long* __restrict__ counters,
+                                                uint32_t* __restrict__ flags) {
+#pragma clang fp contract(off)
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = k < n;
+  const int nn = 2 * n - 1;
+  const int b = nlink[live ? k : n - 1].w;
+  const double pix = x[b], piy = x[n + b], piz = x[2 * n + b];
+  double vix = 0.0, viy = 0.0, viz = 0.0;
+  if (PN) {
+    vix = v[b];
+    viy = v[n + b];
+    viz = v[2 * n + b];
+  }
+  const double mi = m[b];
+  const double inv_m = 1.0 / mi;
+  const double theta2 = theta * theta;
+  const double bd = boxd[3];
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  uint32_t inter = 0, visits = 0;
+  int node = live ? n : -1;
+  const uint32_t limit = 4u * (uint32_t)n + 16u;
+  while (node >= 0) {
+    if (visits > limit) {
+      atomicOr(flags, (uint32_t)FLAG_TREE);
+      break;
+    }
+    const int4 L = nlink[node];
+    ++visits;
+    const double cx = ncom[node], cy = ncom[nn + node], cz = ncom[2 * nn + node];
+    const double dx = pix - cx, dy = piy - cy, dz = piz - cz;
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 == 0.0) {
+      node = L.y;
+      continue;
+    }
+    const double s = ldexp(bd, -depth[node]);
+    if ((s * s < d2 * theta2) || node < n) {
+      double vnx = 0.0, vny = 0.0, vnz = 0.0;
+      if (PN) {
+        vnx = nvel[node];
+        vny = nvel[nn + node];
+        vnz = nvel[2 * nn + node];
+      }
+      double fx, fy, fz;
+      grav64(mi, nmass[node], vix, viy, viz, vnx, vny, vnz, pix, piy, piz, cx, cy, cz, G, ic2, soft, PN, fx, fy, fz);
+      ax += fx * inv_m;
+      ay += fy * inv_m;
+      az += fz * inv_m;
+      ++inter;
+      node = L.y;
+    } else {
+      node = L.x;
+    }
+  }
+  if (live) {
+    a[b] = ax;
+    a[n + b] = ay;
+    a[2 * n + b] = az;
+  }
+  unsigned long long ci = inter, cv = visits;
+  for (int off = 32; off > 0; off >>= 1) {
+    ci += __shfl_xor(ci, off, 64);
+    cv += __shfl_xor(cv, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&counters[BHC_INTERACTIONS], ci);
+    atomicAdd(&counters[BHC_VISITS], cv);
+  }
+}
+
+__global__ void k_zero_counters(unsigned long long* p) {
+  if (threadIdx.x < BHC_COUNT) p[threadIdx.x] = 0ull;
+}
+
+// --------------------------------------------------------------- launch ----
+
+static int reserve_sort(aster_ctx* c) {
+  const size_t n = (size_t)c->n;
+  const size_t nblocks = (n + kRsTile - 1) / kRsTile;
+  ASTER_HIP(c, c->codes.reserve(n));
+  ASTER_HIP(c, c->idx.reserve(n));
+  ASTER_HIP(c, c->codes2.reserve(n));
+  ASTER_HIP(c, c->idx2.reserve(n));
+  ASTER_HIP(c, c->hist.reserve(256 * nblocks));
+  return ASTER_OK;
+}
+
+static int reserve_tree(aster_ctx* c) {
+  const size_t n = (size_t)c->n, nn = 2 * n - 1;
+  int rc = reserve_sort(c);
+  if (rc) return rc;
+  ASTER_HIP(c, c->left.reserve(nn));
+  ASTER_HIP(c, c->right.reserve(nn));
+  ASTER_HIP(c, c->parent.reserve(nn));
+  ASTER_HIP(c, c->escape.reserve(nn));
+  ASTER_HIP(c, c->depth.reserve(nn));
+  ASTER_HIP(c, c->counter.reserve(n));
+  ASTER_HIP(c, c->nmass.reserve(nn));
+  ASTER_HIP(c, c->ncom.reserve(3 * nn));
+  ASTER_HIP(c, c->nvel.reserve(3 * nn));
+  ASTER_HIP(c, c->nbox.reserve(6 * nn));
+  ASTER_HIP(c, c->nrec.reserve(nn));
+  ASTER_HIP(c, c->nlink.reserve(nn));
+  ASTER_HIP(c, c->gsh.reserve(n));
+  if (!c->bh_counters.p) {
+    // Counters accumulate over evaluations until aster_bh_stats reads them.
+    ASTER_HIP(c, c->bh_counters.reserve(BHC_COUNT));
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, c->stream, c->bh_counters.p);
+    ASTER_HIP(c, hipGetLastError());
+  }
+  return ASTER_OK;
+}
+
+static int morton_codes(aster_ctx* c, bool user_box) {
+  phase_begin(c, ASTER_PHASE_MORTON);
+  int rc = launch_box(c, user_box);
+  if (rc) return rc;
+  const int n = (int)c->n;
+  hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->x.p, c->boxd.p, c->codes.p,
+                     c->idx.p, c->flags.p);
+  ASTER_HIP(c, hipGetLastError());
+  phase_end(c, ASTER_PHASE_MORTON);
+  c->tree_valid = false;
+  return ASTER_OK;
+}
+
+int bh_morton_only(aster_ctx* c) {
+  int rc = reserve_sort(c);
+  if (rc) return rc;
+  return morton_codes(c, true);
+}
+
+// Morton codes of the current state, stably sorted: c->codes holds the sorted
+// codes and c->idx the body at each sorted position. user_box = false always
+// uses the symmetric box of the data.
+int morton_sort(aster_ctx* c, bool user_box) {
+  int rc = reserve_sort(c);
+  if (rc) return rc;
+  if ((rc = morton_codes(c, user_box))) return rc;
+  const int n = (int)c->n;
+  const int nblocks = (n + kRsTile - 1) / kRsTile;
+  uint32_t *kin = c->codes.p, *vin = c->idx.p, *kout = c->codes2.p, *vout = c->idx2.p;
+  phase_begin(c, ASTER_PHASE_SORT);
+  for (int shift = 0; shift < 32; shift += 8) {
+    hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(kRsThreads), 0, c->stream, kin, n, shift, c->hist.p, nblocks);
+    hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, c->stream, c->hist.p, 256 * nblocks);
+    hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(kRsThreads), 0, c->stream, kin, vin, kout, vout, n, shift,
+                       c->hist.p, nblocks);
+    std::swap(kin, kout);
+    std::swap(vin, vout);
+  }
+  ASTER_HIP(c, hipGetLastError());
+  phase_end(c, ASTER_PHASE_SORT);
+  // Four passes: sorted codes are back in c->codes, permutation in c->idx.
+  return ASTER_OK;
+}
+
+static int build_tree(aster_ctx* c, int mac) {
+  const int n = (int)c->n;
+  int rc = reserve_tree(c);
+  if (rc) return rc;
+  if ((rc = morton_sort(c, true))) return rc;
+  phase_begin(c, ASTER_PHASE_BUILD);
+  hipLaunchKernelGGL(k_build, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->codes.p, c->idx.p, c->x.p,
+                     c->v.p, c->m.p, c->left.p, c->right.p, c->parent.p, c->counter.p, c->nmass.p, c->ncom.p,
+                     c->nvel.p, c->nbox.p, c->nrec.p, c->gsh.p, c->flags.p);
+  ASTER_HIP(c, hipGetLastError());
+  phase_end(c, ASTER_PHASE_BUILD);
+  phase_begin(c, ASTER_PHASE_MOMENTS);
+  hipLaunchKernelGGL(k_moments, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->left.p, c->right.p,
+                     c->parent.p, c->counter.p, c->nmass.p, c->ncom.p, c->nvel.p, c->nbox.p, c->nrec.p, c->flags.p);
+  ASTER_HIP(c, hipGetLastError());
+  phase_end(c, ASTER_PHASE_MOMENTS);
+  const int nn = 2 * n - 1;
+  phase_begin(c, ASTER_PHASE_LINKS);
+  hipLaunchKernelGGL(k_links, dim3((nn + 255) / 256), dim3(256), 0, c->stream, n, c->left.p, c->right.p,
+                     c->parent.p, c->nbox.p, c->nrec.p, c->boxd.p, c->P.theta, mac, c->depth.p, c->nlink.p, c->idx.p,
+                     c->flags.p);
+  ASTER_HIP(c, hipGetLastError());
+  phase_end(c, ASTER_PHASE_LINKS);
+  c->tree_valid = true;
+  c->tree_mac = mac;
+  return ASTER_OK;
+}
+
+int bh_forces(aster_ctx* c, int method) {
+  const int n = (int)c->n;
+  const bool pn = (c->P.pn_flags & ASTER_PN1) != 0;
+  const bool f64 = method == ASTER_BH_PARITY && c->P.precision == ASTER_FP64;
+  if (pn && !f64)
+    return fail(c, ASTER_ERR_INVALID, "Barnes-Hut with 1PN needs ASTER_BH_PARITY with precision ASTER_FP64");
+  if (!(c->P.theta >= 0.0)) return fail(c, ASTER_ERR_INVALID, "theta must be >= 0");
+  if (method == ASTER_BH_GROUP) return fail(c, ASTER_ERR_INVALID, "ASTER_BH_GROUP is not available in this build");
+  if (c->P.lower > 0 || (c->P.upper >= 0 && c->P.upper < n))
+    return fail(c, ASTER_ERR_INVALID, "Barnes-Hut evaluates the full body range only");
+  if (n == 1) {
+    ASTER_HIP(c, hipMemsetAsync(c->a.p, 0, 3 * sizeof(double), c->stream));
+    return ASTER_OK;
+  }
+  const int mac = (method == ASTER_BH_PARITY) ? 0 : 1;
+  int rc = build_tree(c, mac);
+  if (rc) return rc;
+  phase_begin(c, ASTER_PHASE_TRAVERSE);
+  const double ic2 = (1.0 / c->P.c) * (1.0 / c->P.c);
+  if (f64) {
+    if (pn)
+      hipLaunchKernelGGL((k_bh_f64<true>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->x.p, c->v.p, c->m.p,
+                         c->nmass.p, c->ncom.p, c->nvel.p, c->nlink.p, c->depth.p, c->boxd.p, c->P.theta, c->P.G, ic2,
+                         c->P.softening, c->a.p, c->bh_counters.p, c->flags.p);
+    else
+      hipLaunchKernelGGL((k_bh_f64<false>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->x.p, c->v.p, c->m.p,
+                         c->nmass.p, c->ncom.p, c->nvel.p, c->nlink.p, c->depth.p, c->boxd.p, c->P.theta, c->P.G, ic2,
+                         c->P.softening, c->a.p, c->bh_counters.p, c->flags.p);
+  } else {
+    const float eps2 = (float)c->P.softening;
+    if (c->P.softening > 0.0)
+      hipLaunchKernelGGL((k_bh_f32<false>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->gsh.p, c->nrec.p,
+                         c->nlink.p, eps2, c->P.G, c->a.p, c->bh_counters.p, c->flags.p);
+    else
+      hipLaunchKernelGGL((k_bh_f32<true>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->gsh.p, c->nrec.p,
+                         c->nlink.p, eps2, c->P.G, c->a.p, c->bh_counters.p, c->flags.p);
+  }
+  ASTER_HIP(c, hipGetLastError());
+  phase_end(c, ASTER_PHASE_TRAVERSE);
+  return ASTER_OK;
+}
+
+}  // namespace aster

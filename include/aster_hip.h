@@ -1,0 +1,243 @@
+/* aster_hip.h -- C ABI of the MI355X force and integration library.
+ *
+ * One opaque context owns all device memory and one HIP stream. Every entry
+ * point returns 0 on success or a negative aster_status; aster_last_error()
+ * gives the message. Nothing here exits the process (the reference's Check()
+ * macro, include/Aster/simulations/basic.h:25-29, calls exit(-1)).
+ *
+ * Host arrays use the reference layout: positions, velocities and
+ * accelerations are packed double[3] per body ("AoS-d3", the std::vector<vec3>
+ * of BodyArray, include/Aster/physics/body.h:9-24), masses double[n].
+ *
+ * Calls that take or fill host arrays are synchronous for those arrays; calls
+ * that only enqueue device work (aster_forces with a NULL output, aster_step,
+ * aster_batch_step) return once the work is queued. aster_sync() waits and
+ * reports any deferred device-side error (out-of-box Morton input, non-finite
+ * state).
+ *
+ * A context is single-thread-affine; distinct contexts are independent.
+ */
+#ifndef ASTER_HIP_H
+#define ASTER_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct aster_ctx aster_ctx;
+
+enum aster_status {
+  ASTER_OK = 0,
+  ASTER_ERR_INVALID = -1,    /* bad argument or unsupported combination */
+  ASTER_ERR_HIP = -2,        /* HIP runtime error */
+  ASTER_ERR_OUT_OF_BOX = -3, /* Morton input outside the box (reference UB) */
+  ASTER_ERR_NONFINITE = -4,  /* NaN/Inf in state */
+  ASTER_ERR_CAPACITY = -5,   /* size beyond a documented limit */
+  ASTER_ERR_STATE = -6,      /* call order: e.g. no state loaded yet */
+  ASTER_ERR_NOMEM = -7       /* device allocation failed */
+};
+
+/* Force-law flags: same values as the reference's NEWTON / PN1
+ * (include/Aster/simulations/basic.h:39-40). */
+#define ASTER_NEWTON 0x1
+#define ASTER_PN1 0x10
+
+/* Force methods. */
+enum aster_method {
+  /* FP32 pair forces, plain FP32 accumulation. */
+  ASTER_ALLPAIRS_FP32 = 0,
+  /* FP32 pair forces; FP32 sums over 64-source tiles, Kahan-compensated FP32
+   * accumulation across tiles. Replaces SimpleGPU::compute_forces
+   * (src/solvers/SimpleGPU.cpp:17-71). */
+  ASTER_ALLPAIRS_FP32_KAHAN = 1,
+  /* FP64 in the reference's operation order: bit-identical to
+   * Parallelized::compute_forces (src/utils/builder_endpoint.cpp:729-757). */
+  ASTER_ALLPAIRS_FP64 = 2,
+  /* LBVH Barnes-Hut with the reference opening test s^2 < theta^2 d^2 and
+   * sizes halved per level (src/solvers/barnes_hut.cpp:202-261); per-body
+   * stackless right-first traversal. Precision from aster_params.precision:
+   * FP64 is bit-identical to Barnes_Hut::compute_forces. Replaces
+   * BH_hyper::compute_forces (src/solvers/full_GPU.cpp:519-527). */
+  ASTER_BH_PARITY = 3,
+  /* LBVH Barnes-Hut, FP32, geometric opening test on node bounding boxes:
+   * accept when d > l/theta + delta (l = longest box side, delta = distance
+   * of the centre of mass from the box centre). Per-body stackless traversal. */
+  ASTER_BH_GEOMETRIC = 4,
+  /* As ASTER_BH_GEOMETRIC, but each wavefront of 64 Morton-consecutive bodies
+   * traverses once against the group's bounding box and evaluates the shared
+   * interaction list cooperatively (FP32). */
+  ASTER_BH_GROUP = 5
+};
+
+enum aster_precision { ASTER_FP32 = 0, ASTER_FP64 = 1 };
+
+/* Integrators: values follow the reference's update_type where one exists
+ * (LEAPFROG = 2, include/Aster/simulations/basic.h:56). */
+enum aster_integrator { ASTER_LEAPFROG = 2, ASTER_YOSHIDA4 = 5 };
+
+typedef struct aster_params {
+  double G;         /* gravitational constant (reference default 6e-11) */
+  double c;         /* speed of light for 1PN (reference default 299792458) */
+  double softening; /* eps^2, added to r^2 (tool_chain_impl.cpp:122) */
+  double theta;     /* opening parameter (reference default 0.2) */
+  double box[3];    /* Morton box half-extent (Barnes_Hut::bounding_box);
+                       all zero = symmetric max |coordinate| of the state */
+  int32_t pn_flags; /* ASTER_NEWTON | ASTER_PN1 */
+  int32_t precision;/* ASTER_FP32 / ASTER_FP64 for ASTER_BH_PARITY */
+  int32_t lower;    /* source range [lower, upper) for all-pairs, as */
+  int32_t upper;    /* Solver::set_bounds (sim_obj.h:45); upper < 0 = n */
+  int32_t split;    /* all-pairs source slices (0 = autotuned choice) */
+  int32_t reserved;
+} aster_params;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int aster_version(void);
+
+/* Create a context on HIP device `device`; fills *out. Parameters start at
+ * the reference defaults (G = 6e-11, c = 299792458, softening = 1e-14,
+ * theta = 0.2, sim_meta.h:23-24, sim_obj.h:180). */
+int aster_create(aster_ctx** out, int device);
+int aster_destroy(aster_ctx* ctx);
+/* Message for the last failing call on ctx (never NULL). */
+const char* aster_last_error(const aster_ctx* ctx);
+
+int aster_set_params(aster_ctx* ctx, const aster_params* p);
+int aster_get_params(const aster_ctx* ctx, aster_params* p);
+
+/* Upload n bodies (Simulation::load_gpu_buffers, builder_endpoint.cpp:102-119).
+ * vel may be NULL (zero velocities). Invalidates cached accelerations. */
+int aster_set_state(aster_ctx* ctx, int64_t n, const double* pos, const double* vel, const double* mass);
+/* Download state (Simulation::read_bodies_gpu, builder_endpoint.cpp:424-434);
+ * either pointer may be NULL. */
+int aster_get_state(aster_ctx* ctx, double* pos, double* vel);
+int64_t aster_num_bodies(const aster_ctx* ctx);
+
+/* Evaluate accelerations with `method` (Solver::compute_forces postcondition:
+ * accs[i] holds the acceleration of body i). acc may be NULL: the result then
+ * stays on the device (enqueue only). */
+int aster_forces(aster_ctx* ctx, int method, double* acc);
+/* Copy the last accelerations to the host. */
+int aster_get_acc(aster_ctx* ctx, double* acc);
+
+/* Advance nsteps with `integrator` and force `method` entirely on the device
+ * (update_leapfrog, tool_chain_impl.cpp:169-185; Yoshida-4 is the composition
+ * of three such steps with w1 dt, w0 dt, w1 dt). Accelerations are cached
+ * between steps, so a leapfrog step costs one force evaluation. Positions and
+ * velocities are synchronised at the end of the call. */
+int aster_step(aster_ctx* ctx, int method, int integrator, double dt, int64_t nsteps);
+
+/* Yoshida weights used by ASTER_YOSHIDA4: w[0] = w1, w[1] = w0, w[2] = w1. */
+int aster_yoshida_weights(double* w);
+
+/* Energies and angular momentum of the current state, FP64:
+ * e_ref     = sum 1/2 m v^2 - sum_{i<j} G m_i m_j / (|r_ij| + softening)
+ *             (Graphs::get_total_energy, src/utils/collectors.cpp:19-53);
+ * e_plummer = kinetic - sum_{i<j} G m_i m_j / sqrt(r_ij^2 + softening);
+ * L         = sum m x cross v (3 doubles). Any pointer may be NULL. */
+int aster_energy(aster_ctx* ctx, double* e_ref, double* e_plummer, double* L);
+
+/* Tree of the last Barnes-Hut evaluation, in the reference's NodesArray
+ * indexing (src/solvers/barnes_hut.cpp:309-400): leaf k = sorted position k,
+ * internal node of split i at n + i, root n, children -1 for leaves.
+ * codes: sorted Morton codes (n); perm: body at sorted position k (n);
+ * left/right: 2n-1; node_mass: 2n-1; node_com: 3(2n-1). NULL skips a field. */
+int aster_tree_dump(aster_ctx* ctx, uint32_t* codes, uint32_t* perm, int32_t* left, int32_t* right,
+                    double* node_mass, double* node_com);
+/* Morton codes of the current state, unsorted, with the box in use
+ * (get_morton, src/solvers/bh_specialization.cpp:38-46). box_out may be NULL. */
+int aster_morton(aster_ctx* ctx, uint32_t* codes, double* box_out);
+/* Barnes-Hut counters accumulated over all evaluations since the previous
+ * call (which resets them): accepted body-node interactions (self and
+ * coincident nodes excluded) and node visits, summed over bodies. */
+int aster_bh_stats(aster_ctx* ctx, double* interactions, double* node_visits);
+
+/* Wait for all queued work; reports deferred device errors. */
+int aster_sync(aster_ctx* ctx);
+/* HIP-event timers on the context's stream: mark records event `slot`
+ * (0..15); elapsed waits for `b` and returns ms between marks a and b. */
+int aster_timer_mark(aster_ctx* ctx, int slot);
+int aster_timer_elapsed(aster_ctx* ctx, int a, int b, float* ms);
+
+/* Per-phase device timing. When enabled, every launch phase below is
+ * bracketed by HIP events on the context's stream; aster_phase_times waits,
+ * writes the summed milliseconds and launch counts per phase (arrays of
+ * ASTER_NUM_PHASES) and clears the record. */
+enum aster_phase {
+  ASTER_PHASE_MORTON = 0,   /* box reduction + Morton codes */
+  ASTER_PHASE_SORT = 1,     /* 4-pass LDS radix sort */
+  ASTER_PHASE_BUILD = 2,    /* Karras split + leaf init */
+  ASTER_PHASE_MOMENTS = 3,  /* bottom-up moments */
+  ASTER_PHASE_LINKS = 4,    /* escape links + opening thresholds */
+  ASTER_PHASE_TRAVERSE = 5, /* Barnes-Hut traversal */
+  ASTER_PHASE_ALLPAIRS = 6, /* all-pairs pair kernel */
+  ASTER_PHASE_COMBINE = 7,  /* slice combine / 1PN add */
+  ASTER_PHASE_KICKDRIFT = 8,/* kick/drift update */
+  ASTER_PHASE_ADJOINT = 9,  /* fused force + adjoint kernel */
+  ASTER_PHASE_TILES = 10,   /* anchored FP32 source tiles for all-pairs */
+  ASTER_NUM_PHASES = 12
+};
+int aster_profile(aster_ctx* ctx, int enable);
+int aster_phase_times(aster_ctx* ctx, double* ms, int64_t* counts);
+
+/* ---- batched ensembles: many small independent systems ------------------ */
+/* Upload nsys systems; system s owns bodies [offsets[s], offsets[s+1]).
+ * Each system holds 1..64 bodies. Arrays are AoS-d3 over all bodies. */
+int aster_batch_set(aster_ctx* ctx, int64_t nsys, const int64_t* offsets, const double* pos, const double* vel,
+                    const double* mass);
+int aster_batch_get(aster_ctx* ctx, double* pos, double* vel);
+/* Integrate every system nsteps with all-pairs FP64 forces (Newton, plus
+ * pairwise 1PN if pn_flags has ASTER_PN1) in a persistent kernel, one
+ * wavefront per system; launches of steps_per_launch steps (0 = tuned) are
+ * captured once in a HIP graph and replayed when use_graph != 0. */
+int aster_batch_step(aster_ctx* ctx, int integrator, double dt, int64_t nsteps, int32_t steps_per_launch,
+                     int32_t use_graph);
+/* Per-system energy (kinetic + Plummer-softened potential) and angular
+ * momentum: e[nsys], L[3 nsys]; either may be NULL. */
+int aster_batch_energy(aster_ctx* ctx, double* e, double* L);
+/* Time candidate launch shapes on the loaded batch and keep the fastest.
+ * Writes the chosen systems-per-block and steps-per-launch. */
+int aster_batch_autotune(aster_ctx* ctx, int integrator, double dt, int32_t* systems_per_block,
+                         int32_t* steps_per_launch);
+
+/* ---- trajectory gradients (adjoint of leapfrog) ------------------------ */
+/* Forward: leapfrog with ASTER_ALLPAIRS_FP32_KAHAN forces (FP64 state) for
+ * nsteps from the current state, keeping a checkpoint every ckpt_every steps.
+ * snap_steps (ascending, in [1, nsteps]) select nsnap snapshots whose
+ * positions are written to snap_pos (nsnap * 3n, AoS-d3) if not NULL. The
+ * state is left at step nsteps. */
+int aster_traj_forward(aster_ctx* ctx, double dt, int64_t nsteps, int64_t ckpt_every, int32_t nsnap,
+                       const int64_t* snap_steps, double* snap_pos);
+/* Backward: given dLoss/d(snapshot positions) (nsnap * 3n), write
+ * dLoss/d(initial positions) (3n), d(initial velocities) (3n) and d(masses)
+ * (n) by the hand-written adjoint (transposed pair-force Jacobian-vector
+ * product fused with force recomputation, time-reversed recomputation,
+ * re-anchored at checkpoints). Restores the initial state on return. */
+int aster_traj_backward(aster_ctx* ctx, const double* grad_snap_pos, double* g_pos, double* g_vel,
+                        double* g_mass);
+
+/* ---- initial conditions (seeded, counter-based, reproducible) ---------- */
+/* Plummer sphere (Aarseth, Henon & Wielen 1974), G = 1, total mass `mtot`,
+ * scale radius b, truncated at r <= rmax b, recentred to zero centre of mass
+ * and momentum. mass_mode 0: equal masses; 1: log-uniform over two decades
+ * normalised to mtot. */
+int aster_ic_plummer(int64_t n, uint64_t seed, double mtot, double b, double rmax, int mass_mode, double* pos,
+                     double* vel, double* mass);
+/* Uniform cube [-h, h]^3 at rest, equal masses summing to mtot. */
+int aster_ic_cube(int64_t n, uint64_t seed, double h, double mtot, double* pos, double* vel, double* mass);
+/* Planetary ensembles: per system a central mass 1 and n-1 bodies on Kepler
+ * orbits (G = 1), barycentric. sizes[nsys] drawn uniform in [nmin, nmax]
+ * unless sizes_in is given; offsets[nsys+1]. Call once with pos == NULL to
+ * get sizes/offsets (and the total body count in *total), then again. */
+int aster_ic_ensemble(int64_t nsys, uint64_t seed, int32_t nmin, int32_t nmax, int64_t* offsets, int64_t* total,
+                      double* pos, double* vel, double* mass);
+/* Three-body systems: masses U[0.5,1.5], positions U[0,1]^3, velocities
+ * U[-0.5,0.5]^3 shifted to zero total momentum. Arrays of 3*nsys bodies. */
+int aster_ic_threebody(int64_t nsys, uint64_t seed, double* pos, double* vel, double* mass);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ASTER_HIP_H */

@@ -1,0 +1,698 @@
+// aster_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// A plain C++ FP64 restatement of Aster's CPU force/integration path. It is the
+// checker for the HIP library, nothing more: only tests/, the smoke() entry of
+// __graft_entry__.py and the cpu_baseline leg of bench.py may load it. The
+// product library never links or calls it.
+//
+// Every function names the reference file:line it follows (paths relative to
+// the Aster repository root). Expression order is kept exactly as in the
+// reference because the FP64 GPU paths are checked against it bit for bit;
+// build with -ffp-contract=off and without -ffast-math (the reference library
+// flags, CMakeLists.txt:58-65).
+//
+// Deviations from the reference, each deliberate and documented:
+//  * TBB parallel_for is replaced by std::thread over static chunks of bodies;
+//    each body's sum is sequential in j, as in the reference, so the result does
+//    not depend on the thread count.
+//  * tbb::parallel_sort (barnes_hut.cpp:324) compares codes only, leaving ties
+//    in an unspecified order; the oracle sorts by (code, body index), the one
+//    stable order, which equals the reference wherever codes are unique.
+//  * The traversal stack (barnes_hut.cpp:204, 64 entries, unchecked) is 512
+//    entries here and overflow is reported instead of corrupting memory.
+//  * get_total_energy (collectors.cpp:33-50) reduces with tbb::parallel_reduce
+//    in an unspecified order; the oracle sums serially in i order.
+//  * Yoshida-4 has no counterpart in the reference; it is the composition of
+//    three reference leapfrog steps (tool_chain_impl.cpp:169-185) with the
+//    Yoshida (1990) weights.
+//
+// Parity status of each function is stated in its comment: "pinned by" names
+// the independent checks in tests/ that would catch a plausible mistake.
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <thread>
+#include <vector>
+
+namespace {
+
+constexpr int kNewton = 0x1;   // basic.h:39
+constexpr int kPN1 = 0x10;     // basic.h:40
+
+struct V3 {
+  double x, y, z;
+};
+
+inline V3 load3(const double* a, int64_t i) { return {a[3 * i], a[3 * i + 1], a[3 * i + 2]}; }
+inline void store3(double* a, int64_t i, V3 v) {
+  a[3 * i] = v.x;
+  a[3 * i + 1] = v.y;
+  a[3 * i + 2] = v.z;
+}
+
+template <class F>
+void parallel_for(int64_t n, int nthreads, F&& f) {
+  if (nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
+  if (nthreads == 1 || n < 64) {
+    for (int64_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  int64_t chunk = (n + nthreads - 1) / nthreads;
+  for (int t = 0; t < nthreads; ++t) {
+    int64_t a = t * chunk, b = std::min(n, a + chunk);
+    if (a >= b) break;
+    th.emplace_back([a, b, &f] {
+      for (int64_t i = a; i < b; ++i) f(i);
+    });
+  }
+  for (auto& t : th) t.join();
+}
+
+struct Law {
+  double G, c, soft;
+  int pn;
+};
+
+// get_pn1, tool_chain_impl.cpp:110-118. `norm` is the caller's -norm (:133).
+// vec3*vec3 is a dot product (physics_endpoint.cpp:187-189); int*vec3 goes
+// through operator*(REAL, vec3) (physics_endpoint.cpp:216-218).
+inline V3 pn1_term(double m1, double m2, V3 v1, V3 v2, double inv_r, V3 n, double G) {
+  const double v2_sq = v2.x * v2.x + v2.y * v2.y + v2.z * v2.z;
+  const double v1_sq = v1.x * v1.x + v1.y * v1.y + v1.z * v1.z;
+  const double dot4 = (4.0 * v1.x) * v2.x + (4.0 * v1.y) * v2.y + (4.0 * v1.z) * v2.z;
+  const double nv2 = n.x * v2.x + n.y * v2.y + n.z * v2.z;
+  double s = -v1_sq - 2.0 * v2_sq;
+  s = s + dot4;
+  s = s + 1.5 * (nv2 * nv2);
+  s = s + 5.0 * G * m1 * inv_r;
+  s = s + 4.0 * G * m2 * inv_r;
+  V3 a{n.x * s, n.y * s, n.z * s};
+  const double n4v1 = (n.x * 4.0) * v1.x + (n.y * 4.0) * v1.y + (n.z * 4.0) * v1.z;
+  const double n3v2 = (n.x * 3.0) * v2.x + (n.y * 3.0) * v2.y + (n.z * 3.0) * v2.z;
+  const double t = n4v1 - n3v2;
+  a.x = a.x + (v1.x - v2.x) * t;
+  a.y = a.y + (v1.y - v2.y) * t;
+  a.z = a.z + (v1.z - v2.z) * t;
+  a.x = a.x * G * m2 * inv_r * inv_r;
+  a.y = a.y * G * m2 * inv_r * inv_r;
+  a.z = a.z * G * m2 * inv_r * inv_r;
+  return a;
+}
+
+// compute_gravitational, tool_chain_impl.cpp:120-137: force on body 1 from
+// body 2 (G m1 m2 / r^2 along d = p2 - p1, Plummer-softened with
+// softening = eps^2), plus the pairwise 1PN term scaled by m1.
+inline V3 gravitational(double m1, double m2, V3 v1, V3 v2, V3 p1, V3 p2, const Law& L) {
+  V3 d{p2.x - p1.x, p2.y - p1.y, p2.z - p1.z};
+  double r2 = d.x * d.x + d.y * d.y + d.z * d.z + L.soft;
+  double inv_r = 1.0 / std::sqrt(r2);
+  double inv_r2 = inv_r * inv_r;
+  V3 norm{d.x * inv_r, d.y * inv_r, d.z * inv_r};
+  double scal = L.G * m2 * m1 * inv_r2;
+  V3 f{norm.x * scal, norm.y * scal, norm.z * scal};
+  if (L.pn & kPN1) {
+    double inv_c = 1.0 / L.c;
+    double ic2 = inv_c * inv_c;
+    V3 p = pn1_term(m1, m2, v1, v2, inv_r, V3{-norm.x, -norm.y, -norm.z}, L.G);
+    V3 a1{p.x * ic2, p.y * ic2, p.z * ic2};
+    f.x = f.x + a1.x * m1;
+    f.y = f.y + a1.y * m1;
+    f.z = f.z + a1.z * m1;
+  }
+  return f;
+}
+
+// Parallelized::compute_forces, builder_endpoint.cpp:729-757: every target i in
+// [0,n) sums F(i,j)/m_i over sources j in [lower, upper), j != i, in j order,
+// then assigns accs[i].
+void allpairs(int64_t n, const double* pos, const double* vel, const double* mass, const Law& L,
+              int64_t lower, int64_t upper, double* acc, int nthreads) {
+  if (upper < 0) upper = n;
+  parallel_for(n, nthreads, [&](int64_t i) {
+    V3 a{0.0, 0.0, 0.0};
+    const double m1 = mass[i];
+    const V3 v1 = load3(vel, i), p1 = load3(pos, i);
+    for (int64_t j = lower; j < upper; ++j) {
+      if (i == j) continue;
+      V3 f = gravitational(m1, mass[j], v1, load3(vel, j), p1, load3(pos, j), L);
+      a.x += f.x / m1;
+      a.y += f.y / m1;
+      a.z += f.z / m1;
+    }
+    store3(acc, i, a);
+  });
+}
+
+// part1by2 + get_morton, bh_specialization.cpp:30-46 (PRECISION_BITS = 10,
+// barnes-hut.h:16). `box` is Barnes_Hut::bounding_box, the half-extent.
+inline uint32_t part1by2(uint32_t n) {
+  n = (n | (n << 16)) & 0x030000FFu;
+  n = (n | (n << 8)) & 0x0300F00Fu;
+  n = (n | (n << 4)) & 0x030C30C3u;
+  n = (n | (n << 2)) & 0x09249249u;
+  return n;
+}
+
+inline uint32_t morton(V3 p, const double* box) {
+  double px = (p.x + box[0]) / (2 * box[0]);
+  double py = (p.y + box[1]) / (2 * box[1]);
+  double pz = (p.z + box[2]) / (2 * box[2]);
+  uint32_t ix = std::min((uint32_t)(px * (1 << 10)), (1u << 10) - 1);
+  uint32_t iy = std::min((uint32_t)(py * (1 << 10)), (1u << 10) - 1);
+  uint32_t iz = std::min((uint32_t)(pz * (1 << 10)), (1u << 10) - 1);
+  return (part1by2(iz) << 2) | (part1by2(iy) << 1) | part1by2(ix);
+}
+
+// Karras (2012) split as written in Barnes_Hut::make_tree,
+// barnes_hut.cpp:338-396: internal node of split i is stored at n + i; leaf k
+// is sorted position k; equal codes tie-break on 32 + clz(i ^ j) of sorted
+// positions (:340-342).
+void karras(int64_t n, const uint32_t* codes, int32_t* left, int32_t* right, int nthreads) {
+  const size_t num_leaves = (size_t)n;
+  for (int64_t k = 0; k < 2 * n - 1; ++k) left[k] = right[k] = -1;
+  auto delta = [&](size_t i, size_t j) -> int {
+    if (j >= num_leaves) return -1;
+    if (codes[i] == codes[j]) return 32 + __builtin_clz((unsigned)(i ^ j));
+    return __builtin_clz(codes[i] ^ codes[j]);
+  };
+  parallel_for(n - 1, nthreads, [&](int64_t ii) {
+    size_t i = (size_t)ii;
+    int d_left = (i == 0) ? -1 : delta(i, i - 1);
+    int d_right = delta(i, i + 1);
+    int d = (d_right > d_left) ? 1 : -1;
+    int d_min = (d == 1) ? d_left : d_right;
+    size_t l_max = 2;
+    while ((int)(i + l_max * d) >= 0 && i + l_max * d < num_leaves && delta(i, i + l_max * d) > d_min)
+      l_max *= 2;
+    size_t l = 0;
+    for (size_t t = l_max / 2; t >= 1; t /= 2) {
+      size_t test_pos = i + (l + t) * d;
+      if ((int)test_pos >= 0 && test_pos < num_leaves && delta(i, test_pos) > d_min) l += t;
+    }
+    size_t j = i + l * d;
+    int d_node = delta(i, j);
+    size_t s = 0;
+    size_t range_size = (j > i) ? j - i : i - j;
+    for (size_t t = (range_size + 1) / 2; t >= 1; t = (t + 1) / 2) {
+      size_t test_pos = i + (s + t) * d;
+      if ((int)test_pos >= 0 && test_pos < num_leaves && delta(i, test_pos) > d_node) s += t;
+      if (t == 1) break;
+    }
+    size_t gamma = i + s * d + std::min(d, 0);
+    size_t node = num_leaves + i;
+    size_t lo = std::min(i, j), hi = std::max(i, j);
+    left[node] = (int32_t)(lo == gamma ? gamma : num_leaves + gamma);
+    right[node] = (int32_t)(hi == gamma + 1 ? gamma + 1 : num_leaves + gamma + 1);
+  });
+}
+
+struct Tree {
+  std::vector<int32_t> left, right;
+  std::vector<double> mass, com, vel;  // com/vel: 3 per node
+};
+
+// NodesArray::unite, barnes_hut.cpp:137-152, recursive from `start`; written as
+// an explicit post-order walk so deep trees cannot overflow the C stack. The
+// zero-mass guard (:146) leaves the node at its first-build value (zero).
+void unite(Tree& T, int64_t start) {
+  const int64_t size = (int64_t)T.mass.size();
+  std::vector<std::pair<int64_t, int>> st;
+  st.push_back({start, 0});
+  while (!st.empty()) {
+    auto& top = st.back();
+    int64_t k = top.first;
+    int32_t L = T.left[k], R = T.right[k];
+    if (L == -1 || R == -1 || L < 0 || L >= size || R < 0 || R >= size) {
+      st.pop_back();
+      continue;
+    }
+    if (top.second == 0) {
+      top.second = 1;
+      st.push_back({L, 0});
+      continue;
+    }
+    if (top.second == 1) {
+      top.second = 2;
+      st.push_back({R, 0});
+      continue;
+    }
+    st.pop_back();
+    double total = T.mass[R] + T.mass[L];
+    if (total <= 0) continue;
+    T.mass[k] = total;
+    for (int c = 0; c < 3; ++c) {
+      T.com[3 * k + c] = (T.com[3 * R + c] * T.mass[R] + T.com[3 * L + c] * T.mass[L]) / T.mass[k];
+      T.vel[3 * k + c] = (T.vel[3 * R + c] * T.mass[R] + T.vel[3 * L + c] * T.mass[L]) / T.mass[k];
+    }
+  }
+}
+
+struct BuiltTree {
+  std::vector<uint32_t> codes;   // per body
+  std::vector<uint32_t> perm;    // sorted position -> body
+  std::vector<uint32_t> sorted;  // codes in sorted order
+  Tree T;
+};
+
+// Barnes_Hut::make_tree, barnes_hut.cpp:309-400 (leaf init :326-336).
+void make_tree(int64_t n, const double* pos, const double* vel, const double* mass, const double* box,
+               BuiltTree& B, int nthreads) {
+  B.codes.resize(n);
+  parallel_for(n, nthreads, [&](int64_t i) { B.codes[i] = morton(load3(pos, i), box); });
+  B.perm.resize(n);
+  std::iota(B.perm.begin(), B.perm.end(), 0u);
+  std::stable_sort(B.perm.begin(), B.perm.end(),
+                   [&](uint32_t a, uint32_t b) { return B.codes[a] < B.codes[b]; });
+  B.sorted.resize(n);
+  for (int64_t k = 0; k < n; ++k) B.sorted[k] = B.codes[B.perm[k]];
+  const int64_t nn = 2 * n - 1;
+  B.T.left.assign(nn, -1);
+  B.T.right.assign(nn, -1);
+  B.T.mass.assign(nn, 0.0);
+  B.T.com.assign(3 * nn, 0.0);
+  B.T.vel.assign(3 * nn, 0.0);
+  for (int64_t k = 0; k < n; ++k) {
+    uint32_t b = B.perm[k];
+    B.T.mass[k] = mass[b];
+    for (int c = 0; c < 3; ++c) {
+      B.T.com[3 * k + c] = pos[3 * b + c];
+      B.T.vel[3 * k + c] = vel[3 * b + c];
+    }
+  }
+  if (n > 1) {
+    karras(n, B.sorted.data(), B.T.left.data(), B.T.right.data(), nthreads);
+    unite(B.T, n);
+  }
+}
+
+// Barnes_Hut::get_node_body, barnes_hut.cpp:202-261: right-first DFS from the
+// root with size bd_size halved per level; skip d2 == 0; accept on
+// s^2 < theta^2 d^2 or leaf; accumulate F/m_i in visit order.
+int traverse(const Tree& T, int64_t root, int64_t i, double bd_size, const double* pos, const double* vel,
+             const double* mass, double theta, const Law& L, double* acc, int64_t* count) {
+  struct Task {
+    int64_t node;
+    double size;
+  };
+  Task stack[512];
+  int sp = 0;
+  stack[sp++] = {root, bd_size};
+  const V3 pi = load3(pos, i), vi = load3(vel, i);
+  const double mi = mass[i];
+  const double inv_m = 1.0 / mi;
+  const double theta2 = theta * theta;
+  V3 a{0.0, 0.0, 0.0};
+  int64_t cnt = 0;
+  const size_t nn = T.mass.size();
+  int overflow = 0;
+  while (sp > 0) {
+    Task t = stack[--sp];
+    if ((size_t)t.node >= nn) continue;
+    V3 com = load3(T.com.data(), t.node);
+    V3 vn = load3(T.vel.data(), t.node);
+    double mn = T.mass[t.node];
+    V3 d{pi.x - com.x, pi.y - com.y, pi.z - com.z};
+    double d2 = d.x * d.x + d.y * d.y + d.z * d.z;
+    if (d2 == 0.0) continue;
+    bool leaf = T.right[t.node] == -1 && T.left[t.node] == -1;
+    if ((t.size * t.size < d2 * theta2) || leaf) {
+      V3 f = gravitational(mi, mn, vi, vn, pi, com, L);
+      a.x += f.x * inv_m;
+      a.y += f.y * inv_m;
+      a.z += f.z * inv_m;
+      ++cnt;
+    } else {
+      int64_t l = T.left[t.node], r = T.right[t.node];
+      if (sp + 2 > 512) {
+        overflow = 1;
+        break;
+      }
+      if ((size_t)l < nn) stack[sp++] = {l, t.size * 0.5};
+      if ((size_t)r < nn) stack[sp++] = {r, t.size * 0.5};
+    }
+  }
+  store3(acc, i, a);
+  if (count) count[i] = cnt;
+  return overflow;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_version() { return 1; }
+
+// Parallelized::compute_forces (builder_endpoint.cpp:729-757) with
+// compute_gravitational (tool_chain_impl.cpp:120-137). Pinned by: two-body
+// closed form, Newton's third law for equal masses, brute-force restatement in
+// tests, Plummer-field statistic.
+int orc_allpairs(int64_t n, const double* pos, const double* vel, const double* mass, double G, double c,
+                 double soft, int pn, int64_t lower, int64_t upper, double* acc, int nthreads) {
+  if (n < 0 || !pos || !vel || !mass || !acc) return -1;
+  allpairs(n, pos, vel, mass, Law{G, c, soft, pn}, lower, upper, acc, nthreads);
+  return 0;
+}
+
+// Single pair, for closed-form checks of compute_gravitational.
+int orc_pair_force(double m1, double m2, const double* v1, const double* v2, const double* p1, const double* p2,
+                   double G, double c, double soft, int pn, double* f_out) {
+  V3 f = gravitational(m1, m2, V3{v1[0], v1[1], v1[2]}, V3{v2[0], v2[1], v2[2]}, V3{p1[0], p1[1], p1[2]},
+                       V3{p2[0], p2[1], p2[2]}, Law{G, c, soft, pn});
+  f_out[0] = f.x;
+  f_out[1] = f.y;
+  f_out[2] = f.z;
+  return 0;
+}
+
+// get_morton, bh_specialization.cpp:38-46. Pinned by: per-bit interleave brute
+// force over all 2^10 cell indices per axis, corner table, clamp at +box.
+int orc_morton(int64_t n, const double* pos, const double* box, uint32_t* codes) {
+  for (int64_t i = 0; i < n; ++i) codes[i] = morton(load3(pos, i), box);
+  return 0;
+}
+
+// compare_bounding_vectors (bh_specialization.cpp:23-27) folded over all
+// bodies: symmetric box half-extent = max |coordinate| per axis.
+int orc_box_auto(int64_t n, const double* pos, double* box) {
+  box[0] = box[1] = box[2] = 0.0;
+  for (int64_t i = 0; i < n; ++i)
+    for (int c = 0; c < 3; ++c) box[c] = std::max(box[c], std::fabs(pos[3 * i + c]));
+  return 0;
+}
+
+// Sort step of make_tree (barnes_hut.cpp:324) in the canonical (code, index)
+// order. Pinned by: permutation + sortedness + tie order invariants.
+int orc_sort(int64_t n, const uint32_t* codes, uint32_t* perm) {
+  std::iota(perm, perm + n, 0u);
+  std::stable_sort(perm, perm + n, [&](uint32_t a, uint32_t b) { return codes[a] < codes[b]; });
+  return 0;
+}
+
+// Karras split, barnes_hut.cpp:338-396. left/right hold 2n-1 entries (leaves
+// -1). Pinned by: the 8-key example of Karras (2012) Fig. 3, structural
+// invariants and a top-down brute-force split on small inputs.
+int orc_karras(int64_t n, const uint32_t* sorted_codes, int32_t* left, int32_t* right) {
+  if (n < 1) return -1;
+  karras(n, sorted_codes, left, right, 1);
+  return 0;
+}
+
+// make_tree + unite: full tree for parity (barnes_hut.cpp:137-152, 309-400).
+// Pinned by: root mass = sum m, root COM = sum m x / sum m, per-node brute
+// force over each node's leaf range.
+int orc_build_tree(int64_t n, const double* pos, const double* vel, const double* mass, const double* box,
+                   uint32_t* codes_sorted, uint32_t* perm, int32_t* left, int32_t* right, double* nmass,
+                   double* ncom, double* nvel) {
+  if (n < 1) return -1;
+  BuiltTree B;
+  make_tree(n, pos, vel, mass, box, B, 0);
+  const int64_t nn = 2 * n - 1;
+  if (codes_sorted) std::memcpy(codes_sorted, B.sorted.data(), n * sizeof(uint32_t));
+  if (perm) std::memcpy(perm, B.perm.data(), n * sizeof(uint32_t));
+  if (left) std::memcpy(left, B.T.left.data(), nn * sizeof(int32_t));
+  if (right) std::memcpy(right, B.T.right.data(), nn * sizeof(int32_t));
+  if (nmass) std::memcpy(nmass, B.T.mass.data(), nn * sizeof(double));
+  if (ncom) std::memcpy(ncom, B.T.com.data(), 3 * nn * sizeof(double));
+  if (nvel) std::memcpy(nvel, B.T.vel.data(), 3 * nn * sizeof(double));
+  return 0;
+}
+
+// Barnes_Hut::compute_forces, barnes_hut.cpp:402-418: make_tree, then every
+// body traverses from root n with bd_size = |box|. `count` (optional) receives
+// the number of accepted nodes per body. Returns 1 if a traversal stack
+// overflowed. Pinned by: theta -> 0 equals all-pairs up to summation order;
+// brute-force restatement on tiny inputs.
+int orc_bh(int64_t n, const double* pos, const double* vel, const double* mass, const double* box,
+           double theta, double G, double c, double soft, int pn, double* acc, int64_t* count, int nthreads) {
+  if (n < 2) {
+    for (int64_t i = 0; i < 3 * n; ++i) acc[i] = 0.0;
+    if (count && n == 1) count[0] = 0;
+    return 0;
+  }
+  BuiltTree B;
+  make_tree(n, pos, vel, mass, box, B, nthreads);
+  const double bd_size = std::sqrt(box[0] * box[0] + box[1] * box[1] + box[2] * box[2]);
+  Law L{G, c, soft, pn};
+  std::vector<int> ovf(n, 0);
+  parallel_for(n, nthreads, [&](int64_t i) { ovf[i] = traverse(B.T, n, i, bd_size, pos, vel, mass, theta, L, acc, count); });
+  for (int64_t i = 0; i < n; ++i)
+    if (ovf[i]) return 1;
+  return 0;
+}
+
+// Graphs::get_total_energy, collectors.cpp:19-53 (serial order), plus the
+// Plummer-consistent potential -G m_i m_j / sqrt(r^2 + eps^2) and the total
+// angular momentum sum m x cross v (no reference counterpart).
+int orc_energy(int64_t n, const double* pos, const double* vel, const double* mass, double G, double soft,
+               double* e_ref, double* e_plummer, double* L3) {
+  double e = 0.0, ep = 0.0, kin = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    V3 vi = load3(vel, i);
+    double k = 0.5 * mass[i] * (vi.x * vi.x + vi.y * vi.y + vi.z * vi.z);
+    e += k;
+    kin += k;
+    V3 pi = load3(pos, i);
+    double pp = 0.0;
+    for (int64_t j = i + 1; j < n; ++j) {
+      V3 pj = load3(pos, j);
+      V3 dr{pj.x - pi.x, pj.y - pi.y, pj.z - pi.z};
+      double r2 = dr.x * dr.x + dr.y * dr.y + dr.z * dr.z;
+      double inv_r = 1.0 / (std::sqrt(r2) + soft);
+      e -= G * mass[i] * mass[j] * inv_r;
+      pp += mass[j] / std::sqrt(r2 + soft);
+    }
+    ep -= G * mass[i] * pp;
+  }
+  if (e_ref) *e_ref = e;
+  if (e_plummer) *e_plummer = kin + ep;
+  if (L3) {
+    double lx = 0, ly = 0, lz = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      V3 p = load3(pos, i), v = load3(vel, i);
+      lx += mass[i] * (p.y * v.z - p.z * v.y);
+      ly += mass[i] * (p.z * v.x - p.x * v.z);
+      lz += mass[i] * (p.x * v.y - p.y * v.x);
+    }
+    L3[0] = lx;
+    L3[1] = ly;
+    L3[2] = lz;
+  }
+  return 0;
+}
+
+// Yoshida (1990) fourth-order weights: w1 = 1/(2 - 2^(1/3)), w0 = 1 - 2 w1,
+// evaluated in FP64 from the correctly rounded cube root of 2 (the test pins
+// the closed form and the order conditions).
+int orc_yoshida_weights(double* w) {
+  const double cbrt2 = 0x1.428a2f98d728bp+0;
+  const double w1 = 1.0 / (2.0 - cbrt2);
+  w[0] = w1;
+  w[1] = 1.0 - 2.0 * w1;
+  w[2] = w1;
+  return 0;
+}
+
+// update_leapfrog, tool_chain_impl.cpp:169-185: F(x); x += v dt + (0.5 a) dt dt;
+// v += (0.5 a) dt; F(x); v += (0.5 a) dt. integrator 2 = leapfrog, 5 =
+// Yoshida-4 (three leapfrog steps of w_k dt). method 0 = all-pairs, 1 =
+// Barnes-Hut with `box`/`theta`. `acc` (3n) receives the last accelerations.
+// Pinned by: order-2/order-4 convergence on a Kepler orbit, time reversibility,
+// energy drift bounds.
+int orc_integrate(int64_t n, double* pos, double* vel, const double* mass, double G, double c, double soft,
+                  int pn, int method, const double* box, double theta, int integrator, double dt, int64_t nsteps,
+                  double* acc_out, int nthreads) {
+  if (integrator != 2 && integrator != 5) return -1;
+  std::vector<double> acc(3 * n, 0.0);
+  Law L{G, c, soft, pn};
+  auto forces = [&]() {
+    if (method == 0)
+      allpairs(n, pos, vel, mass, L, 0, n, acc.data(), nthreads);
+    else
+      orc_bh(n, pos, vel, mass, box, theta, G, c, soft, pn, acc.data(), nullptr, nthreads);
+  };
+  double w[3] = {1.0, 0.0, 0.0};
+  int nsub = 1;
+  if (integrator == 5) {
+    orc_yoshida_weights(w);
+    nsub = 3;
+  }
+  for (int64_t s = 0; s < nsteps; ++s) {
+    for (int k = 0; k < nsub; ++k) {
+      const double h = (integrator == 5) ? w[k] * dt : dt;
+      forces();
+      parallel_for(n, nthreads, [&](int64_t i) {
+        for (int c3 = 0; c3 < 3; ++c3) {
+          double a = acc[3 * i + c3];
+          pos[3 * i + c3] += vel[3 * i + c3] * h + 0.5 * a * h * h;
+          vel[3 * i + c3] += 0.5 * a * h;
+        }
+      });
+      forces();
+      parallel_for(n, nthreads, [&](int64_t i) {
+        for (int c3 = 0; c3 < 3; ++c3) vel[3 * i + c3] += 0.5 * acc[3 * i + c3] * h;
+      });
+    }
+  }
+  if (acc_out) std::memcpy(acc_out, acc.data(), 3 * n * sizeof(double));
+  return 0;
+}
+
+// Discrete adjoint of `nsteps` leapfrog steps with Newtonian all-pairs forces
+// (the FP64 CPU reference gradient; no reference counterpart). Stores the whole
+// trajectory, so it is meant for small n. loss = sum over snapshots s and
+// bodies i of |x_i(t_s) - target_s,i|^2 / (nsnap * n). Pinned by: central
+// finite differences of the same forward map.
+int orc_traj_grad(int64_t n, const double* pos0, const double* vel0, const double* mass, double G, double soft,
+                  double dt, int64_t nsteps, int nsnap, const int64_t* snap_steps, const double* target,
+                  double* loss, double* gx, double* gv, double* gm) {
+  const int64_t n3 = 3 * n;
+  std::vector<double> X((nsteps + 1) * n3), V((nsteps + 1) * n3), A((nsteps + 1) * n3);
+  std::memcpy(X.data(), pos0, n3 * sizeof(double));
+  std::memcpy(V.data(), vel0, n3 * sizeof(double));
+  std::vector<double> zero(n3, 0.0);
+  Law L{G, 0.0, soft, kNewton};
+  allpairs(n, X.data(), zero.data(), mass, L, 0, n, A.data(), 1);
+  const double h = 0.5 * dt;
+  for (int64_t s = 0; s < nsteps; ++s) {
+    const double *x = &X[s * n3], *v = &V[s * n3], *a = &A[s * n3];
+    double *xn = &X[(s + 1) * n3], *vn = &V[(s + 1) * n3], *an = &A[(s + 1) * n3];
+    std::vector<double> vh(n3);
+    for (int64_t k = 0; k < n3; ++k) {
+      xn[k] = x[k] + v[k] * dt + 0.5 * a[k] * dt * dt;
+      vh[k] = v[k] + 0.5 * a[k] * dt;
+    }
+    allpairs(n, xn, zero.data(), mass, L, 0, n, an, 1);
+    for (int64_t k = 0; k < n3; ++k) vn[k] = vh[k] + 0.5 * an[k] * dt;
+  }
+  double lsum = 0.0;
+  const double norm = 1.0 / ((double)nsnap * (double)n);
+  std::vector<double> lx(n3, 0.0), lv(n3, 0.0), lm(n, 0.0);
+  // dL/dx at snapshot steps
+  auto add_snap = [&](int64_t step) {
+    for (int q = 0; q < nsnap; ++q)
+      if (snap_steps[q] == step) {
+        const double* x = &X[step * n3];
+        const double* tg = &target[q * n3];
+        for (int64_t k = 0; k < n3; ++k) {
+          double dd = x[k] - tg[k];
+          lsum += dd * dd * norm;
+          lx[k] += 2.0 * dd * norm;
+        }
+      }
+  };
+  // Vector-Jacobian product of a(x) (Plummer-softened Newton) with w:
+  // gx_k += G sum_j K_kj (m_k w_j - m_j w_k), gm_k += -G sum_j (w_j . d_kj)/s^3,
+  // where d_kj = x_j - x_k, s^2 = |d|^2 + soft, K = I/s^3 - 3 d d^T / s^5.
+  auto vjp = [&](const double* x, const std::vector<double>& w, std::vector<double>& ox, std::vector<double>& om) {
+    for (int64_t k = 0; k < n; ++k) {
+      double gxx = 0, gxy = 0, gxz = 0, gmk = 0;
+      for (int64_t j = 0; j < n; ++j) {
+        if (j == k) continue;
+        double dx = x[3 * j] - x[3 * k], dy = x[3 * j + 1] - x[3 * k + 1], dz = x[3 * j + 2] - x[3 * k + 2];
+        double s2 = dx * dx + dy * dy + dz * dz + soft;
+        double is = 1.0 / std::sqrt(s2);
+        double is3 = is * is * is, is5 = is3 * is * is;
+        double ux = mass[k] * w[3 * j] - mass[j] * w[3 * k];
+        double uy = mass[k] * w[3 * j + 1] - mass[j] * w[3 * k + 1];
+        double uz = mass[k] * w[3 * j + 2] - mass[j] * w[3 * k + 2];
+        double du = dx * ux + dy * uy + dz * uz;
+        gxx += ux * is3 - 3.0 * dx * du * is5;
+        gxy += uy * is3 - 3.0 * dy * du * is5;
+        gxz += uz * is3 - 3.0 * dz * du * is5;
+        gmk -= (w[3 * j] * dx + w[3 * j + 1] * dy + w[3 * j + 2] * dz) * is3;
+      }
+      ox[3 * k] += G * gxx;
+      ox[3 * k + 1] += G * gxy;
+      ox[3 * k + 2] += G * gxz;
+      om[k] += G * gmk;
+    }
+  };
+  add_snap(nsteps);
+  std::vector<double> w(n3);
+  for (int64_t s = nsteps - 1; s >= 0; --s) {
+    // v_{s+1} = vh + h a(x_{s+1})
+    for (int64_t k = 0; k < n3; ++k) w[k] = h * lv[k];
+    vjp(&X[(s + 1) * n3], w, lx, lm);
+    // vh = lv (unchanged); x_{s+1} = x_s + dt vh  -> lvh += dt lx
+    std::vector<double> lvh(n3);
+    for (int64_t k = 0; k < n3; ++k) lvh[k] = lv[k] + dt * lx[k];
+    // vh = v_s + h a(x_s)
+    for (int64_t k = 0; k < n3; ++k) w[k] = h * lvh[k];
+    vjp(&X[s * n3], w, lx, lm);
+    lv = lvh;
+    add_snap(s);
+  }
+  if (loss) *loss = lsum;
+  if (gx) std::memcpy(gx, lx.data(), n3 * sizeof(double));
+  if (gv) std::memcpy(gv, lv.data(), n3 * sizeof(double));
+  if (gm) std::memcpy(gm, lm.data(), n * sizeof(double));
+  return 0;
+}
+
+// CPU baseline, bounded sample: Parallelized::compute_forces for targets
+// [t0, t1) over all n sources; wall seconds in *seconds.
+int orc_allpairs_sample(int64_t n, const double* pos, const double* vel, const double* mass, double G, double c,
+                        double soft, int pn, int64_t t0, int64_t t1, int nthreads, double* seconds, double* acc) {
+  if (t0 < 0 || t1 > n || t0 > t1) return -1;
+  Law L{G, c, soft, pn};
+  auto a = std::chrono::steady_clock::now();
+  parallel_for(t1 - t0, nthreads, [&](int64_t k) {
+    const int64_t i = t0 + k;
+    V3 s{0.0, 0.0, 0.0};
+    const double m1 = mass[i];
+    const V3 v1 = load3(vel, i), p1 = load3(pos, i);
+    for (int64_t j = 0; j < n; ++j) {
+      if (i == j) continue;
+      V3 f = gravitational(m1, mass[j], v1, load3(vel, j), p1, load3(pos, j), L);
+      s.x += f.x / m1;
+      s.y += f.y / m1;
+      s.z += f.z / m1;
+    }
+    store3(acc, k, s);
+  });
+  *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+  return 0;
+}
+
+// CPU baseline, bounded sample: Barnes_Hut::make_tree over all n bodies, then
+// get_node_body for bodies with i % stride == 0. Reports build and traversal
+// wall seconds and the accepted interactions of the sampled bodies.
+int orc_bh_sample(int64_t n, const double* pos, const double* vel, const double* mass, const double* box,
+                  double theta, double G, double c, double soft, int pn, int64_t stride, int nthreads,
+                  double* t_build, double* t_trav, int64_t* interactions) {
+  if (n < 2 || stride < 1) return -1;
+  auto a = std::chrono::steady_clock::now();
+  BuiltTree B;
+  make_tree(n, pos, vel, mass, box, B, nthreads);
+  auto b = std::chrono::steady_clock::now();
+  const double bd_size = std::sqrt(box[0] * box[0] + box[1] * box[1] + box[2] * box[2]);
+  Law L{G, c, soft, pn};
+  const int64_t ns = (n + stride - 1) / stride;
+  std::vector<double> acc(3 * n);
+  std::vector<int64_t> cnt(n, 0);
+  parallel_for(ns, nthreads, [&](int64_t k) {
+    traverse(B.T, n, k * stride, bd_size, pos, vel, mass, theta, L, acc.data(), cnt.data());
+  });
+  auto e = std::chrono::steady_clock::now();
+  *t_build = std::chrono::duration<double>(b - a).count();
+  *t_trav = std::chrono::duration<double>(e - b).count();
+  int64_t tot = 0;
+  for (int64_t i = 0; i < n; ++i) tot += cnt[i];
+  *interactions = tot;
+  return 0;
+}
+
+// Loss only (same forward map as orc_traj_grad), for finite differences.
+int orc_traj_loss(int64_t n, const double* pos0, const double* vel0, const double* mass, double G, double soft,
+                  double dt, int64_t nsteps, int nsnap, const int64_t* snap_steps, const double* target,
+                  double* loss) {
+  return orc_traj_grad(n, pos0, vel0, mass, G, soft, dt, nsteps, nsnap, snap_steps, target, loss, nullptr,
+                       nullptr, nullptr);
+}
+
+}  // extern "C"

@@ -1,0 +1,139 @@
+"""GPU parity for the LBVH Barnes-Hut pipeline, through the C ABI.
+
+Morton codes, sort permutation, tree topology, node masses/centres of mass and
+the FP64 reference-MAC forces must be bit-identical to the oracle (Aster's CPU
+path); FP32 traversals are checked against FP64 all-pairs with the stated
+tolerances."""
+import numpy as np
+import pytest
+
+import aster_amd as A
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed(n, seed=1):
+    return A.ic_mixed(n, seed)
+
+
+def rel_err(a, b):
+    return np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+
+
+def load(ctx, pos, vel, mass, **kw):
+    p = dict(G=1.0, softening=2.5e-5, theta=0.5)
+    p.update(kw)
+    ctx.set_params(**p)
+    ctx.set_state(pos, vel, mass)
+
+
+@pytest.mark.parametrize("n", [2, 3, 100, 4097, 1 << 16])
+def test_morton_bit_identical(ctx, orc, n):
+    pos, vel, mass = mixed(n, seed=n)
+    load(ctx, pos, vel, mass)
+    codes, box = ctx.morton()
+    assert np.array_equal(box, orc.box_auto(pos))
+    assert np.array_equal(codes, orc.morton(pos, box))
+
+
+def test_morton_user_box_and_out_of_box(ctx, orc):
+    pos, vel, mass = mixed(5000, seed=2)
+    load(ctx, pos, vel, mass, box=(12.0, 11.0, 10.5))
+    codes, box = ctx.morton()
+    assert np.array_equal(box, [12.0, 11.0, 10.5])
+    assert np.array_equal(codes, orc.morton(pos, box))
+    # the reference default box (1366, sim_meta.h:14-18) puts everything in a few cells
+    load(ctx, pos, vel, mass, box=(1366.0, 1366.0, 1366.0))
+    codes, box = ctx.morton()
+    assert np.array_equal(codes, orc.morton(pos, box))
+    assert len(np.unique(codes)) < 64
+    load(ctx, pos, vel, mass, box=(1.0, 1.0, 1.0))
+    with pytest.raises(A.AsterError) as ei:
+        ctx.morton()
+    assert ei.value.code == -3
+
+
+@pytest.mark.parametrize("n,dup", [(2, False), (3, False), (1000, False), (4096, True), (1 << 16, False),
+                                   (1 << 20, False)])
+def test_tree_bit_identical(ctx, orc, n, dup):
+    pos, vel, mass = mixed(n, seed=n + 1)
+    if dup:  # many equal Morton codes: cluster a third of the bodies in one cell
+        pos[: n // 3] = pos[0] + 1e-9 * np.arange(n // 3)[:, None]
+    load(ctx, pos, vel, mass)
+    ctx.forces(A.BH_PARITY, out=False)
+    T = ctx.tree()
+    box = orc.box_auto(pos)
+    R = orc.build_tree(pos, vel, mass, box)
+    assert np.array_equal(T["codes"], R["codes"])
+    assert np.array_equal(T["perm"], R["perm"])
+    assert np.array_equal(T["left"], R["left"])
+    assert np.array_equal(T["right"], R["right"])
+    assert np.array_equal(T["mass"], R["mass"])
+    assert np.array_equal(T["com"], R["com"])
+
+
+@pytest.mark.parametrize("theta", [0.3, 0.5, 0.7])
+def test_bh_fp64_parity_bit_identical(ctx, orc, theta):
+    pos, vel, mass = mixed(4096, seed=3)
+    load(ctx, pos, vel, mass, theta=theta, precision=A.FP64)
+    acc = ctx.forces(A.BH_PARITY)
+    ref, cnt = orc.bh(pos, vel, mass, orc.box_auto(pos), theta, soft=2.5e-5, counts=True)
+    assert np.array_equal(acc, ref)
+    inter, visits = ctx.bh_stats()
+    assert inter == cnt.sum()
+
+
+def test_bh_fp64_parity_pn1_bit_identical(ctx, orc):
+    pos, vel, mass = mixed(2048, seed=4)
+    load(ctx, pos, vel, mass, theta=0.4, precision=A.FP64, c=30.0, pn_flags=A.NEWTON | A.PN1)
+    acc = ctx.forces(A.BH_PARITY)
+    ref = orc.bh(pos, vel, mass, orc.box_auto(pos), 0.4, c=30.0, soft=2.5e-5, pn=0x11)
+    assert np.array_equal(acc, ref)
+
+
+def test_bh_fp64_user_box_bit_identical(ctx, orc):
+    pos, vel, mass = mixed(3000, seed=5)
+    box = np.array([10.5, 10.25, 11.0])
+    load(ctx, pos, vel, mass, theta=0.5, precision=A.FP64, box=tuple(box))
+    acc = ctx.forces(A.BH_PARITY)
+    assert np.array_equal(acc, orc.bh(pos, vel, mass, box, 0.5, soft=2.5e-5))
+
+
+def test_bh_fp32_parity_close_to_oracle(ctx, orc):
+    pos, vel, mass = mixed(1 << 14, seed=6)
+    load(ctx, pos, vel, mass, theta=0.5, precision=A.FP32)
+    acc = ctx.forces(A.BH_PARITY)
+    ref = orc.bh(pos, vel, mass, orc.box_auto(pos), 0.5, soft=2.5e-5)
+    e = rel_err(acc, ref)
+    print(f"\nBH parity FP32 vs FP64 oracle: median {np.median(e):.2e} p99 {np.quantile(e, .99):.2e}")
+    assert np.median(e) < 1e-5 and np.quantile(e, 0.99) < 1e-3
+
+
+@pytest.mark.parametrize("method", [A.BH_GEOMETRIC])
+def test_bh_accuracy_vs_allpairs(ctx, method):
+    # Force error against FP64 all-pairs at each theta (mixed Plummer + cube).
+    pos, vel, mass = mixed(1 << 16, seed=7)
+    load(ctx, pos, vel, mass)
+    ref = ctx.forces(A.ALLPAIRS_FP64)
+    out = {}
+    for theta in (0.3, 0.5, 0.7):
+        load(ctx, pos, vel, mass, theta=theta)
+        e = rel_err(ctx.forces(method), ref)
+        out[theta] = (np.median(e), np.quantile(e, 0.99), np.sqrt(np.mean(e ** 2)))
+        print(f"\nmethod {method} theta {theta}: median {out[theta][0]:.2e} p99 {out[theta][1]:.2e} rms {out[theta][2]:.2e}")
+    assert out[0.3][0] < out[0.5][0] < out[0.7][0]
+    assert out[0.5][2] <= 1e-3
+
+
+def test_bh_leapfrog_steps(ctx, orc):
+    # One BH evaluation per leapfrog step (accelerations cached across steps);
+    # with the reference MAC in FP64 and a fixed box the trajectory is
+    # bit-identical to update_leapfrog over the CPU Barnes_Hut solver.
+    pos, vel, mass = mixed(2048, seed=8)
+    box = (20.0, 20.0, 20.0)
+    load(ctx, pos, vel, mass, theta=0.5, precision=A.FP64, box=box)
+    ctx.step(A.BH_PARITY, A.LEAPFROG, 5e-4, 3)
+    p, v = ctx.get_state()
+    rp, rv, _ = orc.integrate(pos, vel, mass, 5e-4, 3, integrator=2, method=1, soft=2.5e-5, box=np.array(box),
+                              theta=0.5)
+    assert np.array_equal(p, rp) and np.array_equal(v, rv)

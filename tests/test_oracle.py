@@ -1,0 +1,407 @@
+"""Pins the CPU oracle against closed forms, published examples, invariants and
+independent brute-force restatements. None of these retypes the oracle's own
+formula: each check is chosen so that a dropped term, a wrong sign or index or
+a transposed operand in the oracle fails it."""
+import numpy as np
+import pytest
+
+G = 1.0
+
+
+def rng(seed):
+    return np.random.default_rng(seed)
+
+
+# ------------------------------------------------------------- pair force --
+
+def test_two_body_closed_form(orc):
+    # Plummer-softened Newton: a_1 = G m2 (p2 - p1) / (r^2 + eps^2)^{3/2}
+    p1 = np.array([0.3, -0.2, 0.1])
+    p2 = np.array([-0.4, 0.5, 0.9])
+    m1, m2, soft = 0.7, 1.9, 0.01
+    f = orc.pair_force(m1, m2, np.zeros(3), np.zeros(3), p1, p2, G=2.5, soft=soft, pn=1)
+    d = p2 - p1
+    expect = 2.5 * m1 * m2 * d / (d @ d + soft) ** 1.5
+    np.testing.assert_allclose(f, expect, rtol=1e-14)
+
+
+def blanchet_1pn(m1, m2, x1, x2, v1, v2, G, c):
+    """Two-body 1PN acceleration of body 1 in harmonic coordinates (Blanchet,
+    Living Rev. Relativ. 17 (2014) 2, eq. 203 truncated at 1PN)."""
+    r = np.linalg.norm(x1 - x2)
+    n = (x1 - x2) / r
+    v12 = v1 - v2
+    a = -G * m2 * n / r**2
+    a1 = (5 * G**2 * m1 * m2 / r**3 + 4 * G**2 * m2**2 / r**3
+          + G * m2 / r**2 * (1.5 * (n @ v2) ** 2 - v1 @ v1 + 4 * v1 @ v2 - 2 * v2 @ v2)) * n
+    a1 = a1 + G * m2 / r**2 * (4 * n @ v1 - 3 * n @ v2) * v12
+    return a, a1 / c**2
+
+
+def test_pn1_matches_blanchet(orc):
+    r = rng(1)
+    for _ in range(20):
+        x1, x2 = r.normal(size=3), r.normal(size=3)
+        v1, v2 = r.normal(size=3) * 0.3, r.normal(size=3) * 0.3
+        m1, m2 = r.uniform(0.1, 2.0, size=2)
+        c = 7.0
+        newton, a1 = blanchet_1pn(m1, m2, x1, x2, v1, v2, G, c)
+        f_n = orc.pair_force(m1, m2, v1, v2, x1, x2, G=G, c=c, soft=0.0, pn=0x1)
+        f_pn = orc.pair_force(m1, m2, v1, v2, x1, x2, G=G, c=c, soft=0.0, pn=0x11)
+        np.testing.assert_allclose(f_n / m1, newton, rtol=1e-13)
+        np.testing.assert_allclose((f_pn - f_n) / m1, a1, rtol=1e-11, atol=1e-15 * np.abs(a1).max())
+
+
+# --------------------------------------------------------------- all-pairs --
+
+def numpy_allpairs(pos, mass, soft):
+    d = pos[None, :, :] - pos[:, None, :]          # d[i, j] = x_j - x_i
+    r2 = (d ** 2).sum(-1) + soft
+    np.fill_diagonal(r2, 1.0)
+    w = mass[None, :] / r2 ** 1.5
+    np.fill_diagonal(w, 0.0)
+    return G * (w[:, :, None] * d).sum(1)
+
+
+def test_allpairs_brute_force(orc):
+    r = rng(2)
+    n = 300
+    pos = r.normal(size=(n, 3))
+    mass = r.uniform(0.5, 2.0, n) / n
+    acc = orc.allpairs(pos, np.zeros_like(pos), mass, soft=1e-3)
+    np.testing.assert_allclose(acc, numpy_allpairs(pos, mass, 1e-3), rtol=1e-11, atol=1e-13)
+
+
+def test_allpairs_newton_third_law(orc):
+    r = rng(3)
+    n = 500
+    pos = r.normal(size=(n, 3))
+    mass = r.uniform(0.5, 2.0, n) / n
+    acc = orc.allpairs(pos, np.zeros_like(pos), mass, soft=1e-4)
+    mom = (mass[:, None] * acc).sum(0)
+    assert np.abs(mom).max() < 1e-14 * np.abs(mass[:, None] * acc).sum()
+
+
+def test_allpairs_bounds_restrict_sources(orc):
+    # Parallelized::compute_forces: every target, sources in [lower, upper).
+    r = rng(4)
+    n = 64
+    pos = r.normal(size=(n, 3))
+    mass = r.uniform(0.5, 2.0, n)
+    full = orc.allpairs(pos, np.zeros_like(pos), mass, soft=1e-3, lower=10, upper=40)
+    sub = numpy_allpairs(pos, np.where((np.arange(n) >= 10) & (np.arange(n) < 40), mass, 0.0), 1e-3)
+    np.testing.assert_allclose(full, sub, rtol=1e-11, atol=1e-13)
+
+
+def test_plummer_field_statistic(orc):
+    import aster_amd
+    pos, vel, mass = aster_amd.ic_plummer(4096, 7)
+    acc = orc.allpairs(pos, vel, mass, soft=1e-4)
+    r = np.linalg.norm(pos, axis=1)
+    ar = -(acc * pos).sum(1) / r
+    sel = (r > 0.3) & (r < 3.0)
+    expect = r[sel] / (r[sel] ** 2 + 1.0) ** 1.5
+    ratio = np.median(ar[sel] / expect)
+    assert abs(ratio - 1.0) < 0.03
+
+
+# ------------------------------------------------------------------ Morton --
+
+def interleave(ix, iy, iz):
+    code = 0
+    for b in range(10):
+        code |= ((ix >> b) & 1) << (3 * b)
+        code |= ((iy >> b) & 1) << (3 * b + 1)
+        code |= ((iz >> b) & 1) << (3 * b + 2)
+    return code
+
+
+def test_morton_interleave_all_cells(orc):
+    cells = np.arange(1024)
+    box = np.array([1.0, 2.0, 4.0])
+    # Centre of each cell along each axis, the other axes at cell (k*7) % 1024.
+    p = np.stack([(cells + 0.5) / 512.0 * box[0] - box[0],
+                  ((cells * 7) % 1024 + 0.5) / 512.0 * box[1] - box[1],
+                  ((cells * 13) % 1024 + 0.5) / 512.0 * box[2] - box[2]], axis=1)
+    codes = orc.morton(p, box)
+    expect = [interleave(int(a), int((a * 7) % 1024), int((a * 13) % 1024)) for a in cells]
+    assert np.array_equal(codes, np.array(expect, np.uint32))
+
+
+def test_morton_corners_and_clamp(orc):
+    box = np.array([3.0, 3.0, 3.0])
+    p = np.array([[-3.0, -3.0, -3.0], [3.0, 3.0, 3.0], [3.0, -3.0, 0.0], [0.0, 0.0, 0.0]])
+    codes = orc.morton(p, box)
+    assert codes[0] == 0
+    assert codes[1] == interleave(1023, 1023, 1023) == (1 << 30) - 1
+    assert codes[2] == interleave(1023, 0, 512)
+    assert codes[3] == interleave(512, 512, 512)
+
+
+# -------------------------------------------------------------------- sort --
+
+def test_sort_is_stable_by_code(orc):
+    r = rng(5)
+    codes = r.integers(0, 50, size=5000).astype(np.uint32)
+    perm = orc.sort(codes)
+    assert np.array_equal(perm, np.argsort(codes, kind="stable").astype(np.uint32))
+    s = codes[perm]
+    assert np.all(s[1:] >= s[:-1])
+    ties = s[1:] == s[:-1]
+    assert np.all(perm[1:][ties] > perm[:-1][ties])
+
+
+# ------------------------------------------------------------------ Karras --
+
+def test_karras_fig3(orc):
+    # Karras (2012), "Maximizing parallelism in the construction of BVHs,
+    # octrees, and k-d trees", Fig. 3: 8 sorted 5-bit keys.
+    keys = np.array([0b00001, 0b00010, 0b00100, 0b00101, 0b10011, 0b11000, 0b11001, 0b11110], np.uint32)
+    left, right = orc.karras(keys)
+    n = 8
+    I = lambda i: n + i  # internal node i
+    expect = {0: (I(3), I(4)), 1: (0, 1), 2: (2, 3), 3: (I(1), I(2)), 4: (4, I(5)), 5: (I(6), 7), 6: (5, 6)}
+    for i, (l, r) in expect.items():
+        assert (left[I(i)], right[I(i)]) == (l, r), i
+    assert np.all(left[:n] == -1) and np.all(right[:n] == -1)
+
+
+def topdown_tree(keys):
+    """Brute-force radix tree: split each range at the highest differing bit of
+    the augmented keys (code << 32 | position). Node of range [lo, hi] with
+    split g has children [lo, g] (internal node g, or leaf) and [g+1, hi]
+    (internal node g+1, or leaf)."""
+    n = len(keys)
+    aug = [(int(k) << 32) | i for i, k in enumerate(keys)]
+    left = np.full(2 * n - 1, -1, np.int32)
+    right = np.full(2 * n - 1, -1, np.int32)
+    stack = [(0, n - 1, n + 0)]
+    while stack:
+        lo, hi, node = stack.pop()
+        bit = (aug[lo] ^ aug[hi]).bit_length() - 1
+        g = lo
+        while g + 1 <= hi and not (aug[g + 1] >> bit) & 1:
+            g += 1
+        if g == lo:
+            left[node] = lo
+        else:
+            left[node] = n + g
+            stack.append((lo, g, n + g))
+        if g + 1 == hi:
+            right[node] = hi
+        else:
+            right[node] = n + g + 1
+            stack.append((g + 1, hi, n + g + 1))
+    return left, right
+
+
+@pytest.mark.parametrize("n,span", [(2, 4), (3, 8), (17, 16), (64, 1 << 30), (200, 64), (513, 1 << 12)])
+def test_karras_matches_topdown(orc, n, span):
+    r = rng(n)
+    keys = np.sort(r.integers(0, span, size=n).astype(np.uint32))
+    left, right = orc.karras(keys)
+    el, er = topdown_tree(keys)
+    assert np.array_equal(left, el) and np.array_equal(right, er)
+
+
+def test_karras_invariants(orc):
+    r = rng(9)
+    n = 3000
+    keys = np.sort(r.integers(0, 1 << 12, size=n).astype(np.uint32))  # many duplicates
+    left, right = orc.karras(keys)
+    kids = np.concatenate([left[n:], right[n:]])
+    assert len(kids) == 2 * (n - 1)
+    assert sorted(kids.tolist()) == [k for k in range(2 * n - 1) if k != n]  # unique parents, root n
+
+
+# ----------------------------------------------------------------- moments --
+
+def leaf_ranges(left, right, n):
+    lo = np.zeros(2 * n - 1, np.int64)
+    hi = np.zeros(2 * n - 1, np.int64)
+    lo[:n] = hi[:n] = np.arange(n)
+
+    def rec(k):
+        if k < n:
+            return lo[k], hi[k]
+        a = rec(left[k])
+        b = rec(right[k])
+        lo[k], hi[k] = a[0], b[1]
+        return lo[k], hi[k]
+    import sys
+    sys.setrecursionlimit(100000)
+    rec(n)
+    return lo, hi
+
+
+def test_moments_root_and_nodes(orc):
+    import aster_amd
+    pos, vel, mass = aster_amd.ic_mixed(2048, 11)
+    box = orc.box_auto(pos)
+    T = orc.build_tree(pos, vel, mass, box)
+    n = len(mass)
+    np.testing.assert_allclose(T["mass"][n], mass.sum(), rtol=1e-14)
+    np.testing.assert_allclose(T["com"][n], (mass[:, None] * pos).sum(0) / mass.sum(), atol=1e-13)
+    lo, hi = leaf_ranges(T["left"], T["right"], n)
+    perm = T["perm"]
+    for k in range(n, 2 * n - 1, 37):
+        b = perm[lo[k]:hi[k] + 1]
+        np.testing.assert_allclose(T["mass"][k], mass[b].sum(), rtol=1e-13)
+        np.testing.assert_allclose(T["com"][k], (mass[b, None] * pos[b]).sum(0) / mass[b].sum(), rtol=1e-12,
+                                   atol=1e-13)
+    # sorted codes are the body codes in perm order, perm is the stable order
+    codes = orc.morton(pos, box)
+    assert np.array_equal(T["codes"], codes[perm])
+    assert np.array_equal(perm, np.argsort(codes, kind="stable"))
+
+
+# ------------------------------------------------------------- Barnes-Hut --
+
+def test_bh_theta_zero_is_allpairs(orc):
+    r = rng(12)
+    n = 400
+    pos = r.normal(size=(n, 3))
+    mass = r.uniform(0.5, 1.5, n) / n
+    box = orc.box_auto(pos)
+    a_bh = orc.bh(pos, np.zeros_like(pos), mass, box, theta=0.0, soft=1e-3)
+    a_ap = orc.allpairs(pos, np.zeros_like(pos), mass, soft=1e-3)
+    np.testing.assert_allclose(a_bh, a_ap, rtol=1e-11, atol=1e-13)
+
+
+def test_bh_huge_theta_uses_root_only(orc):
+    r = rng(13)
+    n = 300
+    pos = r.normal(size=(n, 3))
+    mass = r.uniform(0.5, 1.5, n) / n
+    box = orc.box_auto(pos)
+    a, cnt = orc.bh(pos, np.zeros_like(pos), mass, box, theta=1e9, soft=1e-3, counts=True)
+    com = (mass[:, None] * pos).sum(0) / mass.sum()
+    d = com - pos
+    expect = mass.sum() * d / ((d ** 2).sum(1, keepdims=True) + 1e-3) ** 1.5
+    np.testing.assert_allclose(a, expect, rtol=1e-12)
+    assert np.all(cnt == 1)
+
+
+def recursive_bh(T, n, i, pos, mass, size, theta, soft):
+    """Recursive restatement of get_node_body (barnes_hut.cpp:202-261):
+    right subtree first, skip d2 == 0, accept s^2 < theta^2 d^2 or leaf."""
+    acc = np.zeros(3)
+
+    def visit(k, s):
+        nonlocal acc
+        d = pos[i] - T["com"][k]
+        d2 = d @ d
+        if d2 == 0:
+            return
+        if s * s < d2 * theta * theta or k < n:
+            dd = T["com"][k] - pos[i]
+            acc = acc + T["mass"][k] * dd / (dd @ dd + soft) ** 1.5
+            return
+        visit(T["right"][k], s * 0.5)
+        visit(T["left"][k], s * 0.5)
+    visit(n, size)
+    return acc
+
+
+def test_bh_recursive_restatement(orc):
+    import aster_amd
+    pos, vel, mass = aster_amd.ic_mixed(96, 14)
+    box = orc.box_auto(pos)
+    T = orc.build_tree(pos, vel, mass, box)
+    a = orc.bh(pos, vel, mass, box, theta=0.6, soft=1e-4)
+    size = np.linalg.norm(box)
+    for i in range(0, 96, 5):
+        np.testing.assert_allclose(a[i], recursive_bh(T, 96, i, pos, mass, size, 0.6, 1e-4), rtol=1e-11)
+
+
+# ------------------------------------------------------------------ energy --
+
+def test_energy_closed_form(orc):
+    pos = np.array([[0.0, 0.0, 0.0], [1.0, 2.0, 2.0]])
+    vel = np.array([[0.1, 0.0, 0.0], [0.0, -0.2, 0.3]])
+    mass = np.array([2.0, 3.0])
+    soft = 0.5
+    e, ep, L = orc.energy(pos, vel, mass, G=1.5, soft=soft)
+    kin = 0.5 * 2.0 * 0.01 + 0.5 * 3.0 * 0.13
+    assert e == pytest.approx(kin - 1.5 * 6.0 / (3.0 + soft), rel=1e-14)
+    assert ep == pytest.approx(kin - 1.5 * 6.0 / np.sqrt(9.0 + soft), rel=1e-14)
+    np.testing.assert_allclose(L, 3.0 * np.cross(pos[1], vel[1]), rtol=1e-14)
+
+
+# -------------------------------------------------------------- integrators --
+
+def kepler_ic():
+    # Two bodies, circular-ish orbit, period ~ 2 pi.
+    pos = np.array([[-0.5, 0.0, 0.0], [0.5, 0.0, 0.0]])
+    vel = np.array([[0.0, -0.55, 0.0], [0.0, 0.55, 0.0]])
+    mass = np.array([1.0, 1.0])
+    return pos, vel, mass
+
+
+@pytest.mark.parametrize("integrator,order", [(2, 2), (5, 4)])
+def test_integrator_order(orc, integrator, order):
+    pos, vel, mass = kepler_ic()
+    T = 2.0
+    ref, _, _ = orc.integrate(pos, vel, mass, T / 4096, 4096, integrator=integrator)
+    errs = []
+    for k in (32, 64, 128):
+        p, _, _ = orc.integrate(pos, vel, mass, T / k, k, integrator=integrator)
+        errs.append(np.abs(p - ref).max())
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(np.abs(rates - order) < 0.3), rates
+
+
+def test_integrator_time_reversible(orc):
+    pos, vel, mass = kepler_ic()
+    for integ in (2, 5):
+        p, v, _ = orc.integrate(pos, vel, mass, 0.01, 200, integrator=integ)
+        p2, v2, _ = orc.integrate(p, -v, mass, 0.01, 200, integrator=integ)
+        np.testing.assert_allclose(p2, pos, atol=1e-11)
+        np.testing.assert_allclose(-v2, vel, atol=1e-11)
+
+
+def test_yoshida_weights(orc):
+    w = orc.yoshida_weights()
+    w1 = 1.0 / (2.0 - 2.0 ** (1.0 / 3.0))
+    assert w[0] == pytest.approx(w1, rel=1e-15) and w[2] == w[0]
+    assert 2 * w[0] + w[1] == pytest.approx(1.0, abs=1e-15)          # consistency
+    assert 2 * w[0] ** 3 + w[1] ** 3 == pytest.approx(0.0, abs=1e-14)  # third-order condition
+
+
+def test_leapfrog_energy_bounded(orc):
+    import aster_amd
+    pos, vel, mass = aster_amd.ic_plummer(128, 3)
+    e0 = orc.energy(pos, vel, mass, soft=1e-2)[1]
+    p, v, _ = orc.integrate(pos, vel, mass, 1e-3, 200, soft=1e-2)
+    e1 = orc.energy(p, v, mass, soft=1e-2)[1]
+    assert abs(e1 - e0) / abs(e0) < 1e-5
+
+
+# ----------------------------------------------------------------- adjoint --
+
+def test_traj_grad_finite_differences(orc):
+    r = rng(21)
+    n = 5
+    pos = r.uniform(-1, 1, (n, 3))
+    vel = r.uniform(-0.3, 0.3, (n, 3))
+    mass = r.uniform(0.5, 1.5, n)
+    snaps = np.array([5, 10, 20])
+    target = pos[None] + r.normal(scale=0.1, size=(3, n, 3))
+    dt, steps, soft = 0.01, 20, 0.05
+    loss, gx, gv, gm = orc.traj_grad(pos, vel, mass, dt, steps, snaps, target, soft=soft)
+    h = 1e-6
+    for (arr, g) in ((pos, gx), (vel, gv)):
+        for idx in [(0, 0), (2, 1), (4, 2)]:
+            a1 = arr.copy(); a1[idx] += h
+            a2 = arr.copy(); a2[idx] -= h
+            args1 = (a1, vel, mass) if arr is pos else (pos, a1, mass)
+            args2 = (a2, vel, mass) if arr is pos else (pos, a2, mass)
+            fd = (orc.traj_loss(*args1, dt, steps, snaps, target, soft=soft)
+                  - orc.traj_loss(*args2, dt, steps, snaps, target, soft=soft)) / (2 * h)
+            assert g[idx] == pytest.approx(fd, rel=1e-6, abs=1e-10)
+    for k in range(n):
+        m1 = mass.copy(); m1[k] += h
+        m2 = mass.copy(); m2[k] -= h
+        fd = (orc.traj_loss(pos, vel, m1, dt, steps, snaps, target, soft=soft)
+              - orc.traj_loss(pos, vel, m2, dt, steps, snaps, target, soft=soft)) / (2 * h)
+        assert gm[k] == pytest.approx(fd, rel=1e-6, abs=1e-10)
