@@ -17,6 +17,7 @@
 // sources, IEEE division and square root, no contraction: bit-identical to the
 // CPU path.
 #include "common.hpp"
+#include "device.hpp"
 
 namespace aster {
 
@@ -24,12 +25,7 @@ namespace aster {
 
 enum { ACC_PLAIN = 0, ACC_TILE_KAHAN = 1, ACC_FULL_KAHAN = 2 };
 
-__device__ __forceinline__ void kahan_add(float& s, float& c, float t) {
-  const float y = t - c;
-  const float u = s + y;
-  c = (u - s) - y;
-  s = u;
-}
+__device__ __forceinline__ void kahan_add(float& s, float& c, float t) { kahan_add32(s, c, t); }
 
 // Sources, padded to npad with zero-mass bodies at the origin.
 //   k_src32: FP32 {x, y, z, m} (positions rounded to FP32 once).
@@ -158,34 +154,6 @@ __global__ void k_combine(int n, int S, const double* __restrict__ part, const d
 
 // ---------------------------------------------------------------- FP64 ----
 
-// get_pn1 (tool_chain_impl.cpp:110-118) with the caller's -norm (:133).
-__device__ __forceinline__ void pn1_term(double m1, double m2, double v1x, double v1y, double v1z, double v2x,
-                                         double v2y, double v2z, double inv_r, double nx, double ny, double nz,
-                                         double G, double& ax, double& ay, double& az) {
-#pragma clang fp contract(off)
-  const double v2_sq = v2x * v2x + v2y * v2y + v2z * v2z;
-  const double v1_sq = v1x * v1x + v1y * v1y + v1z * v1z;
-  const double dot4 = (4.0 * v1x) * v2x + (4.0 * v1y) * v2y + (4.0 * v1z) * v2z;
-  const double nv2 = nx * v2x + ny * v2y + nz * v2z;
-  double s = -v1_sq - 2.0 * v2_sq;
-  s = s + dot4;
-  s = s + 1.5 * (nv2 * nv2);
-  s = s + 5.0 * G * m1 * inv_r;
-  s = s + 4.0 * G * m2 * inv_r;
-  ax = nx * s;
-  ay = ny * s;
-  az = nz * s;
-  const double n4v1 = (nx * 4.0) * v1x + (ny * 4.0) * v1y + (nz * 4.0) * v1z;
-  const double n3v2 = (nx * 3.0) * v2x + (ny * 3.0) * v2y + (nz * 3.0) * v2z;
-  const double t = n4v1 - n3v2;
-  ax = ax + (v1x - v2x) * t;
-  ay = ay + (v1y - v2y) * t;
-  az = az + (v1z - v2z) * t;
-  ax = ax * G * m2 * inv_r * inv_r;
-  ay = ay * G * m2 * inv_r * inv_r;
-  az = az * G * m2 * inv_r * inv_r;
-}
-
 // MODE 0: Newton (+1PN if PN); MODE 1: only the 1PN term (added to an FP32
 // Newtonian result). x, v: SoA planes [3][n].
 template <bool PN, int MODE>
@@ -223,7 +191,7 @@ __global__ __launch_bounds__(256) void k_allpairs_f64(int n, int lower, int uppe
     if (PN) {
       const double v2x = v[j], v2y = v[n + j], v2z = v[2 * n + j];
       double px, py, pz;
-      pn1_term(m1, m2, v1x, v1y, v1z, v2x, v2y, v2z, inv_r, -nx, -ny, -nz, G, px, py, pz);
+      pn1_term64(m1, m2, v1x, v1y, v1z, v2x, v2y, v2z, inv_r, -nx, -ny, -nz, G, px, py, pz);
       fx = fx + (px * ic2) * m1;
       fy = fy + (py * ic2) * m1;
       fz = fz + (pz * ic2) * m1;

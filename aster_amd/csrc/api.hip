@@ -145,9 +145,11 @@ int aster_destroy(aster_ctx* c) {
   c->codes.release(); c->idx.release(); c->codes2.release(); c->idx2.release(); c->hist.release();
   c->left.release(); c->right.release(); c->parent.release(); c->escape.release(); c->counter.release();
   c->depth.release(); c->nmass.release(); c->ncom.release(); c->nvel.release(); c->nbox.release();
+  c->levels.release(); c->bucket.release();
   c->nrec.release(); c->nlink.release(); c->bh_counters.release(); c->gsh.release();
-  c->batch.offsets.release(); c->batch.x.release(); c->batch.v.release(); c->batch.m.release();
-  c->batch.e.release(); c->batch.L.release(); c->batch.order.release();
+  auto& Bt = c->batch;
+  Bt.x.release(); Bt.v.release(); Bt.a.release(); Bt.m.release(); Bt.ebody.release(); Bt.esys.release();
+  Bt.doff.release(); Bt.lane_body.release(); Bt.lane_seg.release(); Bt.wave_max.release();
   auto& A = c->adj;
   A.ckpt_x.release(); A.ckpt_v.release(); A.x0.release(); A.v0.release(); A.lx.release(); A.lv.release();
   A.lm.release(); A.w.release(); A.snapg.release(); A.aprev.release(); A.part.release();
@@ -287,12 +289,22 @@ int aster_step(aster_ctx* c, int method, int integrator, double dt, int64_t nste
     nsub = 3;
   }
   int rc;
-  if (!c->acc_valid || c->acc_method != method)
+  // With velocity-dependent (1PN) forces the reference's first force
+  // evaluation of a step sees the fully kicked velocities, so it is not the
+  // previous step's second evaluation: re-evaluate, as update_leapfrog does.
+  // Newtonian forces depend on positions only and are reused.
+  const bool vdep = (c->P.pn_flags & ASTER_PN1) != 0;
+  if (!vdep && (!c->acc_valid || c->acc_method != method))
     if ((rc = compute_forces(c, method))) return rc;
   double pending = 0.0;
   for (int64_t s = 0; s < nsteps; ++s)
     for (int k = 0; k < nsub; ++k) {
       const double h = (integrator == ASTER_YOSHIDA4) ? w[k] * dt : dt;
+      if (vdep) {
+        if (pending != 0.0 && (rc = launch_kick(c, pending))) return rc;
+        if ((rc = compute_forces(c, method))) return rc;
+        pending = 0.0;
+      }
       if ((rc = launch_kick_drift(c, pending, h))) return rc;
       if ((rc = compute_forces(c, method))) return rc;
       pending = h;

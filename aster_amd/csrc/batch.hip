@@ -1,14 +1,544 @@
-// Batched ensembles (placeholder until the persistent kernel lands).
+// Batched ensembles: thousands of small independent systems (1..64 bodies)
+// integrated by one persistent kernel per launch.
+//
+// Layout: every system lives in the lanes of one wavefront; with packing,
+// several systems share a wavefront (systems are ordered by size, largest
+// first, and filled next-fit into 64 lanes), so a 3-body system does not idle
+// 61 lanes. Each lane keeps its body's FP64 position, velocity and
+// acceleration in registers for the whole launch; the wavefront exchanges
+// positions (and velocities for 1PN) through LDS once per force evaluation.
+// Force per pair: all-pairs within the system, Newton (+ pairwise 1PN),
+// optionally Plummer-softened.
+//
+// Two arithmetic modes:
+//   fast  (precision != ASTER_FP64): FP64 with an rsqrt estimate refined by
+//         two Newton steps; contraction allowed.
+//   exact (precision == ASTER_FP64): compute_gravitational in the reference's
+//         operation order and j order (src/physics/tool_chain_impl.cpp:120-137,
+//         Parallelized::compute_forces, src/utils/builder_endpoint.cpp:729-757),
+//         so each system's trajectory is bit-identical to the CPU path.
+// Integrators as in aster_step: leapfrog (update_leapfrog,
+// tool_chain_impl.cpp:169-185) or Yoshida-4, accelerations cached between
+// steps and across launches.
+//
+// A step sequence is split into launches of steps_per_launch steps; the
+// launches are captured once into a HIP graph and replayed.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
 #include "common.hpp"
+#include "device.hpp"
 
 namespace aster {
-int batch_step(aster_ctx* c, int, double, int64_t, int, int) { return fail(c, ASTER_ERR_INVALID, "batch not built"); }
-int batch_energy(aster_ctx* c, double*, double*) { return fail(c, ASTER_ERR_INVALID, "batch not built"); }
-int batch_autotune(aster_ctx* c, int, double, int32_t*, int32_t*) { return fail(c, ASTER_ERR_INVALID, "batch not built"); }
+
+constexpr int kBatchMaxWpb = 4;
+
+struct BatchArgs {
+  int nwaves;
+  const int32_t* lane_body;
+  const int32_t* lane_seg;  // (segment base lane << 8) | segment size
+  const int32_t* wave_max;  // largest segment in the wavefront
+  double* x;
+  double* v;
+  double* a;
+  const double* m;
+  int64_t nsteps;
+  int nsub;
+  double h[3];
+  int init;
+  double G, ic2, soft;
+};
+
+template <bool PN, bool EXACT>
+__device__ __forceinline__ void batch_accel(int lane, int base, int cnt, int maxcnt, bool live, const double* lx,
+                                            const double* ly, const double* lz, const double* lm, const double* lvx,
+                                            const double* lvy, const double* lvz, const double* X, const double* V,
+                                            double mi, double G, double ic2, double soft, double* A) {
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  for (int j = 0; j < maxcnt; ++j) {
+    const int src = min(base + j, 63);
+    const bool valid = live && j < cnt && src != lane;
+    const double x2 = lx[src], y2 = ly[src], z2 = lz[src], m2 = lm[src];
+    double v2x = 0.0, v2y = 0.0, v2z = 0.0;
+    if (PN) {
+      v2x = lvx[src];
+      v2y = lvy[src];
+      v2z = lvz[src];
+    }
+    if (EXACT) {
+      double fx, fy, fz;
+      grav64(mi, m2, V[0], V[1], V[2], v2x, v2y, v2z, X[0], X[1], X[2], x2, y2, z2, G, ic2, soft, PN, fx, fy, fz);
+      if (valid) {
+#pragma clang fp contract(off)
+        ax += fx / mi;
+        ay += fy / mi;
+        az += fz / mi;
+      }
+    } else {
+      const double dx = x2 - X[0], dy = y2 - X[1], dz = z2 - X[2];
+      const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, soft)));
+      const double inv = rsqrt64(valid ? r2 : 1.0);
+      const double f = valid ? G * m2 * (inv * inv * inv) : 0.0;
+      ax = fma(f, dx, ax);
+      ay = fma(f, dy, ay);
+      az = fma(f, dz, az);
+      if (PN) {
+        double px, py, pz;
+        pn1_term64(mi, m2, V[0], V[1], V[2], v2x, v2y, v2z, inv, -dx * inv, -dy * inv, -dz * inv, G, px, py, pz);
+        const double s = valid ? ic2 : 0.0;
+        ax = fma(px, s, ax);
+        ay = fma(py, s, ay);
+        az = fma(pz, s, az);
+      }
+    }
+  }
+  A[0] = ax;
+  A[1] = ay;
+  A[2] = az;
+}
+
+__device__ __forceinline__ void kick_drift_reg(double* X, double* V, const double* A, double dt_prev, double dt) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double vc = V[c];
+    if (dt_prev != 0.0) vc = vc + 0.5 * A[c] * dt_prev;
+    X[c] = X[c] + (vc * dt + 0.5 * A[c] * dt * dt);
+    V[c] = vc + 0.5 * A[c] * dt;
+  }
+}
+
+__device__ __forceinline__ void kick_reg(double* V, const double* A, double dt) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int c = 0; c < 3; ++c) V[c] = V[c] + 0.5 * A[c] * dt;
+}
+
+template <bool PN, bool EXACT>
+__global__ __launch_bounds__(256) void k_batch(BatchArgs P) {
+  __shared__ double sx[kBatchMaxWpb][64], sy[kBatchMaxWpb][64], sz[kBatchMaxWpb][64], sm[kBatchMaxWpb][64];
+  __shared__ double svx[PN ? kBatchMaxWpb : 1][64], svy[PN ? kBatchMaxWpb : 1][64], svz[PN ? kBatchMaxWpb : 1][64];
+  const int wl = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * (blockDim.x >> 6) + wl;
+  if (wave >= P.nwaves) return;
+  const int slot = wave * 64 + lane;
+  const int b = P.lane_body[slot];
+  const int seg = P.lane_seg[slot];
+  const int base = seg >> 8, cnt = seg & 255;
+  const int maxcnt = P.wave_max[wave];
+  const bool live = b >= 0;
+  double X[3] = {0.0, 0.0, 0.0}, V[3] = {0.0, 0.0, 0.0}, A[3] = {0.0, 0.0, 0.0};
+  double mi = 0.0;
+  if (live) {
+    for (int c = 0; c < 3; ++c) {
+      X[c] = P.x[3 * (size_t)b + c];
+      V[c] = P.v[3 * (size_t)b + c];
+      A[c] = P.a[3 * (size_t)b + c];
+    }
+    mi = P.m[b];
+  }
+  double* lx = sx[wl];
+  double* ly = sy[wl];
+  double* lz = sz[wl];
+  double* lvx = svx[PN ? wl : 0];
+  double* lvy = svy[PN ? wl : 0];
+  double* lvz = svz[PN ? wl : 0];
+  sm[wl][lane] = mi;
+  auto publish = [&]() {
+    lx[lane] = X[0];
+    ly[lane] = X[1];
+    lz[lane] = X[2];
+    if (PN) {
+      lvx[lane] = V[0];
+      lvy[lane] = V[1];
+      lvz[lane] = V[2];
+    }
+    wave_sync();
+  };
+  if (P.init) {
+    publish();
+    batch_accel<PN, EXACT>(lane, base, cnt, maxcnt, live, lx, ly, lz, sm[wl], lvx, lvy, lvz, X, V, mi, P.G, P.ic2,
+                           P.soft, A);
+    wave_sync();
+  }
+  double pending = 0.0;
+  for (int64_t s = 0; s < P.nsteps; ++s) {
+    for (int k = 0; k < P.nsub; ++k) {
+      const double h = P.h[k];
+      if (PN) {
+        // Velocity-dependent forces: re-evaluate after the final kick, as
+        // update_leapfrog's first compute_forces does.
+        if (pending != 0.0) kick_reg(V, A, pending);
+        pending = 0.0;
+        publish();
+        batch_accel<PN, EXACT>(lane, base, cnt, maxcnt, live, lx, ly, lz, sm[wl], lvx, lvy, lvz, X, V, mi, P.G,
+                               P.ic2, P.soft, A);
+        wave_sync();
+      }
+      kick_drift_reg(X, V, A, pending, h);
+      publish();
+      batch_accel<PN, EXACT>(lane, base, cnt, maxcnt, live, lx, ly, lz, sm[wl], lvx, lvy, lvz, X, V, mi, P.G,
+                             P.ic2, P.soft, A);
+      wave_sync();
+      pending = h;
+    }
+  }
+  if (P.nsteps > 0) kick_reg(V, A, pending);
+  if (live) {
+    for (int c = 0; c < 3; ++c) {
+      P.x[3 * (size_t)b + c] = X[c];
+      P.v[3 * (size_t)b + c] = V[c];
+      P.a[3 * (size_t)b + c] = A[c];
+    }
+  }
+}
+
+// Per-body energy terms: kinetic - 1/2 G m_i sum_j m_j / sqrt(r^2 + soft),
+// and m x cross v; out [4][nbodies].
+__global__ __launch_bounds__(256) void k_batch_energy(BatchArgs P, int nbodies, double* __restrict__ out) {
+  __shared__ double sx[kBatchMaxWpb][64], sy[kBatchMaxWpb][64], sz[kBatchMaxWpb][64], sm[kBatchMaxWpb][64];
+  const int wl = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * (blockDim.x >> 6) + wl;
+  if (wave >= P.nwaves) return;
+  const int slot = wave * 64 + lane;
+  const int b = P.lane_body[slot];
+  const int seg = P.lane_seg[slot];
+  const int base = seg >> 8, cnt = seg & 255;
+  const int maxcnt = P.wave_max[wave];
+  const bool live = b >= 0;
+  double X[3] = {0.0, 0.0, 0.0}, V[3] = {0.0, 0.0, 0.0}, mi = 0.0;
+  if (live) {
+    for (int c = 0; c < 3; ++c) {
+      X[c] = P.x[3 * (size_t)b + c];
+      V[c] = P.v[3 * (size_t)b + c];
+    }
+    mi = P.m[b];
+  }
+  sx[wl][lane] = X[0];
+  sy[wl][lane] = X[1];
+  sz[wl][lane] = X[2];
+  sm[wl][lane] = mi;
+  wave_sync();
+  double pot = 0.0;
+  for (int j = 0; j < maxcnt; ++j) {
+    const int src = min(base + j, 63);
+    const bool valid = live && j < cnt && src != lane;
+    const double dx = sx[wl][src] - X[0], dy = sy[wl][src] - X[1], dz = sz[wl][src] - X[2];
+    const double r2 = dx * dx + dy * dy + dz * dz + P.soft;
+    if (valid) pot += sm[wl][src] / sqrt(r2);
+  }
+  if (live) {
+    out[b] = 0.5 * mi * (V[0] * V[0] + V[1] * V[1] + V[2] * V[2]) - 0.5 * P.G * mi * pot;
+    out[(size_t)nbodies + b] = mi * (X[1] * V[2] - X[2] * V[1]);
+    out[(size_t)2 * nbodies + b] = mi * (X[2] * V[0] - X[0] * V[2]);
+    out[(size_t)3 * nbodies + b] = mi * (X[0] * V[1] - X[1] * V[0]);
+  }
+}
+
+__global__ void k_batch_sum(int64_t nsys, const int64_t* __restrict__ off, int64_t nbodies,
+                            const double* __restrict__ eb, double* __restrict__ es) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsys) return;
+  for (int q = 0; q < 4; ++q) {
+    double acc = 0.0;
+    for (int64_t k = off[s]; k < off[s + 1]; ++k) acc += eb[(size_t)q * nbodies + k];
+    es[(size_t)q * nsys + s] = acc;
+  }
+}
+
+// ----------------------------------------------------------------- host ----
+
+static int upload_plan(aster_ctx* c) {
+  BatchState& B = c->batch;
+  const int64_t nsys = B.nsys;
+  std::vector<int64_t> order(nsys);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int64_t p, int64_t q) {
+    return B.offsets[p + 1] - B.offsets[p] > B.offsets[q + 1] - B.offsets[q];
+  });
+  std::vector<int32_t> lb, ls, wm;
+  int used = 64;
+  for (int64_t s : order) {
+    const int sz = (int)(B.offsets[s + 1] - B.offsets[s]);
+    if (B.pack == 0 || used + sz > 64) {
+      lb.resize(lb.size() + 64, -1);
+      ls.resize(ls.size() + 64, 0);
+      wm.push_back(0);
+      used = 0;
+    }
+    const size_t w = wm.size() - 1;
+    for (int k = 0; k < sz; ++k) {
+      lb[w * 64 + used + k] = (int32_t)(B.offsets[s] + k);
+      ls[w * 64 + used + k] = (used << 8) | sz;
+    }
+    wm[w] = std::max(wm[w], sz);
+    used += sz;
+  }
+  B.nwaves = (int)wm.size();
+  ASTER_HIP(c, B.lane_body.reserve(lb.size()));
+  ASTER_HIP(c, B.lane_seg.reserve(ls.size()));
+  ASTER_HIP(c, B.wave_max.reserve(wm.size()));
+  ASTER_HIP(c, hipMemcpyAsync(B.lane_body.p, lb.data(), lb.size() * 4, hipMemcpyHostToDevice, c->stream));
+  ASTER_HIP(c, hipMemcpyAsync(B.lane_seg.p, ls.data(), ls.size() * 4, hipMemcpyHostToDevice, c->stream));
+  ASTER_HIP(c, hipMemcpyAsync(B.wave_max.p, wm.data(), wm.size() * 4, hipMemcpyHostToDevice, c->stream));
+  ASTER_HIP(c, hipStreamSynchronize(c->stream));
+  return ASTER_OK;
+}
+
+static void drop_graph(BatchState& B) {
+  if (B.exec) (void)hipGraphExecDestroy(B.exec);
+  if (B.graph) (void)hipGraphDestroy(B.graph);
+  B.exec = nullptr;
+  B.graph = nullptr;
+  B.g_steps = -1;
+}
+
+static BatchArgs make_args(aster_ctx* c, int64_t nsteps, int nsub, const double* h, int init) {
+  BatchState& B = c->batch;
+  BatchArgs P;
+  P.nwaves = B.nwaves;
+  P.lane_body = B.lane_body.p;
+  P.lane_seg = B.lane_seg.p;
+  P.wave_max = B.wave_max.p;
+  P.x = B.x.p;
+  P.v = B.v.p;
+  P.a = B.a.p;
+  P.m = B.m.p;
+  P.nsteps = nsteps;
+  P.nsub = nsub;
+  for (int k = 0; k < 3; ++k) P.h[k] = h[k];
+  P.init = init;
+  P.G = c->P.G;
+  P.ic2 = (1.0 / c->P.c) * (1.0 / c->P.c);
+  P.soft = c->P.softening;
+  return P;
+}
+
+static int launch_batch(aster_ctx* c, const BatchArgs& P) {
+  const int wpb = c->batch.waves_per_block;
+  const dim3 grid((P.nwaves + wpb - 1) / wpb), block(64 * wpb);
+  const bool pn = (c->P.pn_flags & ASTER_PN1) != 0;
+  const bool exact = c->P.precision == ASTER_FP64;
+  if (pn && exact) hipLaunchKernelGGL((k_batch<true, true>), grid, block, 0, c->stream, P);
+  else if (pn) hipLaunchKernelGGL((k_batch<true, false>), grid, block, 0, c->stream, P);
+  else if (exact) hipLaunchKernelGGL((k_batch<false, true>), grid, block, 0, c->stream, P);
+  else hipLaunchKernelGGL((k_batch<false, false>), grid, block, 0, c->stream, P);
+  ASTER_HIP(c, hipGetLastError());
+  return ASTER_OK;
+}
+
+static int substeps(int integrator, double dt, double* h) {
+  if (integrator == ASTER_LEAPFROG) {
+    h[0] = dt;
+    h[1] = h[2] = 0.0;
+    return 1;
+  }
+  double w[3];
+  aster_yoshida_weights(w);
+  for (int k = 0; k < 3; ++k) h[k] = w[k] * dt;
+  return 3;
+}
+
+int batch_step(aster_ctx* c, int integrator, double dt, int64_t nsteps, int spl, int use_graph) {
+  BatchState& B = c->batch;
+  if (B.nsys == 0) return fail(c, ASTER_ERR_STATE, "no batch loaded");
+  if (integrator != ASTER_LEAPFROG && integrator != ASTER_YOSHIDA4)
+    return fail(c, ASTER_ERR_INVALID, "integrator must be ASTER_LEAPFROG or ASTER_YOSHIDA4");
+  if (!std::isfinite(dt) || nsteps < 0 || spl < 0) return fail(c, ASTER_ERR_INVALID, "invalid dt, nsteps or spl");
+  double h[3];
+  const int nsub = substeps(integrator, dt, h);
+  const double h0[3] = {0.0, 0.0, 0.0};
+  int rc;
+  if (!B.a_valid) {
+    if ((rc = launch_batch(c, make_args(c, 0, 1, h0, 1)))) return rc;
+    B.a_valid = true;
+  }
+  if (nsteps == 0) return ASTER_OK;
+  if (spl == 0) spl = B.steps_per_launch > 0 ? B.steps_per_launch : 256;
+  const int64_t full = nsteps / spl, rem = nsteps % spl;
+  const bool pn = (c->P.pn_flags & ASTER_PN1) != 0;
+  const int exact = c->P.precision == ASTER_FP64;
+  if (!use_graph) {
+    for (int64_t l = 0; l < full; ++l)
+      if ((rc = launch_batch(c, make_args(c, spl, nsub, h, 0)))) return rc;
+    if (rem && (rc = launch_batch(c, make_args(c, rem, nsub, h, 0)))) return rc;
+    return ASTER_OK;
+  }
+  const bool same = B.exec && B.g_steps == nsteps && B.g_integrator == integrator && B.g_spl == spl &&
+                    B.g_dt == dt && B.g_pn == (int)pn && B.g_exact == exact && B.g_wpb == B.waves_per_block &&
+                    B.g_pack == B.pack && B.g_G == c->P.G && B.g_c == c->P.c && B.g_soft == c->P.softening;
+  if (!same) {
+    drop_graph(B);
+    ASTER_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int lrc = ASTER_OK;
+    for (int64_t l = 0; l < full && !lrc; ++l) lrc = launch_batch(c, make_args(c, spl, nsub, h, 0));
+    if (!lrc && rem) lrc = launch_batch(c, make_args(c, rem, nsub, h, 0));
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (lrc) {
+      if (g) (void)hipGraphDestroy(g);
+      return lrc;
+    }
+    if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+    B.graph = g;
+    ASTER_HIP(c, hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+    B.g_steps = nsteps;
+    B.g_integrator = integrator;
+    B.g_spl = spl;
+    B.g_dt = dt;
+    B.g_pn = pn;
+    B.g_exact = exact;
+    B.g_wpb = B.waves_per_block;
+    B.g_pack = B.pack;
+    B.g_G = c->P.G;
+    B.g_c = c->P.c;
+    B.g_soft = c->P.softening;
+  }
+  ASTER_HIP(c, hipGraphLaunch(B.exec, c->stream));
+  return ASTER_OK;
+}
+
+int batch_energy(aster_ctx* c, double* e, double* L) {
+  BatchState& B = c->batch;
+  if (B.nsys == 0) return fail(c, ASTER_ERR_STATE, "no batch loaded");
+  ASTER_HIP(c, B.ebody.reserve((size_t)4 * B.nbodies));
+  ASTER_HIP(c, B.esys.reserve((size_t)4 * B.nsys));
+  const double h0[3] = {0.0, 0.0, 0.0};
+  const BatchArgs P = make_args(c, 0, 1, h0, 0);
+  const int wpb = B.waves_per_block;
+  hipLaunchKernelGGL(k_batch_energy, dim3((P.nwaves + wpb - 1) / wpb), dim3(64 * wpb), 0, c->stream, P,
+                     (int)B.nbodies, B.ebody.p);
+  ASTER_HIP(c, hipGetLastError());
+  hipLaunchKernelGGL(k_batch_sum, dim3((unsigned)((B.nsys + 255) / 256)), dim3(256), 0, c->stream, B.nsys, B.doff.p,
+                     B.nbodies, B.ebody.p, B.esys.p);
+  ASTER_HIP(c, hipGetLastError());
+  std::vector<double> h((size_t)4 * B.nsys);
+  ASTER_HIP(c, hipMemcpyAsync(h.data(), B.esys.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  ASTER_HIP(c, hipStreamSynchronize(c->stream));
+  for (int64_t s = 0; s < B.nsys; ++s) {
+    if (e) e[s] = h[s];
+    if (L)
+      for (int q = 0; q < 3; ++q) L[3 * s + q] = h[(size_t)(1 + q) * B.nsys + s];
+  }
+  return ASTER_OK;
+}
+
+// Times 16 steps of every (packing, wavefronts-per-block) candidate on the
+// loaded batch, restoring the state after each trial, and keeps the fastest.
+// Steps per launch are then chosen so that one launch lasts about 20 ms.
+int batch_autotune(aster_ctx* c, int integrator, double dt, int32_t* spb, int32_t* spl) {
+  BatchState& B = c->batch;
+  if (B.nsys == 0) return fail(c, ASTER_ERR_STATE, "no batch loaded");
+  const size_t nb3 = (size_t)3 * B.nbodies;
+  DBuf<double> save;
+  ASTER_HIP(c, save.reserve(3 * nb3));
+  int rc;
+  const double h0[3] = {0.0, 0.0, 0.0};
+  if (!B.a_valid) {
+    if ((rc = launch_batch(c, make_args(c, 0, 1, h0, 1)))) return rc;
+    B.a_valid = true;
+  }
+  ASTER_HIP(c, hipMemcpyAsync(save.p, B.x.p, nb3 * 8, hipMemcpyDeviceToDevice, c->stream));
+  ASTER_HIP(c, hipMemcpyAsync(save.p + nb3, B.v.p, nb3 * 8, hipMemcpyDeviceToDevice, c->stream));
+  ASTER_HIP(c, hipMemcpyAsync(save.p + 2 * nb3, B.a.p, nb3 * 8, hipMemcpyDeviceToDevice, c->stream));
+  double h[3];
+  const int nsub = substeps(integrator, dt, h);
+  const int trial = 16;
+  float best = 1e30f;
+  int best_pack = B.pack, best_wpb = B.waves_per_block;
+  for (int pack = 0; pack <= 1; ++pack) {
+    B.pack = pack;
+    if ((rc = upload_plan(c))) return rc;
+    for (int wpb = 1; wpb <= kBatchMaxWpb; wpb *= 2) {
+      B.waves_per_block = wpb;
+      if ((rc = launch_batch(c, make_args(c, 2, nsub, h, 0)))) return rc;  // warm
+      ASTER_HIP(c, hipEventRecord(c->ev[14], c->stream));
+      if ((rc = launch_batch(c, make_args(c, trial, nsub, h, 0)))) return rc;
+      ASTER_HIP(c, hipEventRecord(c->ev[15], c->stream));
+      ASTER_HIP(c, hipEventSynchronize(c->ev[15]));
+      float ms = 0.f;
+      ASTER_HIP(c, hipEventElapsedTime(&ms, c->ev[14], c->ev[15]));
+      if (ms < best) {
+        best = ms;
+        best_pack = pack;
+        best_wpb = wpb;
+      }
+      ASTER_HIP(c, hipMemcpyAsync(B.x.p, save.p, nb3 * 8, hipMemcpyDeviceToDevice, c->stream));
+      ASTER_HIP(c, hipMemcpyAsync(B.v.p, save.p + nb3, nb3 * 8, hipMemcpyDeviceToDevice, c->stream));
+      ASTER_HIP(c, hipMemcpyAsync(B.a.p, save.p + 2 * nb3, nb3 * 8, hipMemcpyDeviceToDevice, c->stream));
+    }
+  }
+  ASTER_HIP(c, hipStreamSynchronize(c->stream));
+  save.release();
+  B.pack = best_pack;
+  B.waves_per_block = best_wpb;
+  if ((rc = upload_plan(c))) return rc;
+  const double ms_per_step = best / trial;
+  int s = (int)std::max(1.0, std::floor(20.0 / std::max(ms_per_step, 1e-6)));
+  B.steps_per_launch = std::min(s, 100000);
+  drop_graph(B);
+  if (spb) *spb = B.waves_per_block;
+  if (spl) *spl = B.steps_per_launch;
+  return ASTER_OK;
+}
+
 }  // namespace aster
+
+using namespace aster;
+
 extern "C" {
-int aster_batch_set(aster_ctx* c, int64_t, const int64_t*, const double*, const double*, const double*) {
-  return aster::fail(c, ASTER_ERR_INVALID, "batch not built");
+
+int aster_batch_set(aster_ctx* c, int64_t nsys, const int64_t* offsets, const double* pos, const double* vel,
+                    const double* mass) {
+  if (!c) return ASTER_ERR_INVALID;
+  if (nsys < 1 || !offsets || !pos || !mass) return fail(c, ASTER_ERR_INVALID, "need nsys >= 1, offsets, pos, mass");
+  if (offsets[0] != 0) return fail(c, ASTER_ERR_INVALID, "offsets[0] must be 0");
+  for (int64_t s = 0; s < nsys; ++s) {
+    const int64_t k = offsets[s + 1] - offsets[s];
+    if (k < 1 || k > 64) return fail(c, ASTER_ERR_CAPACITY, "each system needs 1..64 bodies");
+  }
+  const int64_t nb = offsets[nsys];
+  if (nb > (int64_t(1) << 28)) return fail(c, ASTER_ERR_CAPACITY, "at most 2^28 bodies per batch");
+  for (int64_t i = 0; i < nb; ++i) {
+    if (!(mass[i] > 0.0) || !std::isfinite(mass[i])) return fail(c, ASTER_ERR_INVALID, "masses must be > 0");
+    for (int q = 0; q < 3; ++q)
+      if (!std::isfinite(pos[3 * i + q]) || (vel && !std::isfinite(vel[3 * i + q])))
+        return fail(c, ASTER_ERR_NONFINITE, "non-finite position or velocity in input");
+  }
+  ASTER_HIP(c, hipSetDevice(c->device));
+  BatchState& B = c->batch;
+  drop_graph(B);
+  B.nsys = nsys;
+  B.nbodies = nb;
+  B.offsets.assign(offsets, offsets + nsys + 1);
+  ASTER_HIP(c, B.x.reserve(3 * nb));
+  ASTER_HIP(c, B.v.reserve(3 * nb));
+  ASTER_HIP(c, B.a.reserve(3 * nb));
+  ASTER_HIP(c, B.m.reserve(nb));
+  ASTER_HIP(c, B.doff.reserve(nsys + 1));
+  ASTER_HIP(c, hipMemcpyAsync(B.x.p, pos, 3 * nb * 8, hipMemcpyHostToDevice, c->stream));
+  if (vel)
+    ASTER_HIP(c, hipMemcpyAsync(B.v.p, vel, 3 * nb * 8, hipMemcpyHostToDevice, c->stream));
+  else
+    ASTER_HIP(c, hipMemsetAsync(B.v.p, 0, 3 * nb * 8, c->stream));
+  ASTER_HIP(c, hipMemsetAsync(B.a.p, 0, 3 * nb * 8, c->stream));
+  ASTER_HIP(c, hipMemcpyAsync(B.m.p, mass, nb * 8, hipMemcpyHostToDevice, c->stream));
+  ASTER_HIP(c, hipMemcpyAsync(B.doff.p, offsets, (nsys + 1) * 8, hipMemcpyHostToDevice, c->stream));
+  B.a_valid = false;
+  return upload_plan(c);
 }
-int aster_batch_get(aster_ctx* c, double*, double*) { return aster::fail(c, ASTER_ERR_INVALID, "batch not built"); }
+
+int aster_batch_get(aster_ctx* c, double* pos, double* vel) {
+  if (!c) return ASTER_ERR_INVALID;
+  BatchState& B = c->batch;
+  if (B.nsys == 0) return fail(c, ASTER_ERR_STATE, "no batch loaded");
+  ASTER_HIP(c, hipSetDevice(c->device));
+  if (pos) ASTER_HIP(c, hipMemcpyAsync(pos, B.x.p, 3 * B.nbodies * 8, hipMemcpyDeviceToHost, c->stream));
+  if (vel) ASTER_HIP(c, hipMemcpyAsync(vel, B.v.p, 3 * B.nbodies * 8, hipMemcpyDeviceToHost, c->stream));
+  ASTER_HIP(c, hipStreamSynchronize(c->stream));
+  return ASTER_OK;
 }
+
+}  // extern "C"

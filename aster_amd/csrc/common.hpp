@@ -45,19 +45,24 @@ enum { BHC_INTERACTIONS = 0, BHC_VISITS = 1, BHC_COUNT = 4 };
 
 struct BatchState {
   int64_t nsys = 0, nbodies = 0;
-  DBuf<int64_t> offsets;   // nsys+1
-  DBuf<double> x, v, m;    // AoS-d3 over bodies (x, v), m per body
-  DBuf<double> e, L;       // per-system energy / angular momentum
-  DBuf<int32_t> order;     // systems sorted by size (descending) for packing
-  int systems_per_block = 4;
-  int steps_per_launch = 0;
-  // Captured launch sequence.
+  std::vector<int64_t> offsets;      // host copy, nsys+1
+  DBuf<double> x, v, a, m;           // AoS-d3 over bodies (x, v, a), m per body
+  DBuf<double> ebody;                // per-body energy terms [4][nbodies]
+  DBuf<double> esys;                 // per-system energy + L [4][nsys]
+  DBuf<int64_t> doff;                // device offsets
+  // Launch plan: wavefront w owns lanes [64 w, 64 w + 64).
+  int pack = 1;                      // 0: one system per wavefront; 1: systems packed into wavefronts
+  int waves_per_block = 4;
+  int steps_per_launch = 0;          // 0: not tuned yet
+  int nwaves = 0;
+  DBuf<int32_t> lane_body, lane_seg, wave_max;
+  bool a_valid = false;
+  // Captured launch sequence and the arguments it was captured with.
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  int graph_steps = 0, graph_integrator = -1, graph_spl = 0, graph_spb = 0;
-  double graph_dt = 0.0;
-  int graph_pn = -1;
-  double graph_G = 0.0, graph_c = 0.0, graph_soft = -1.0;
+  int64_t g_steps = -1;
+  int g_integrator = -1, g_spl = 0, g_pn = -1, g_exact = -1, g_wpb = 0, g_pack = -1;
+  double g_dt = 0.0, g_G = 0.0, g_c = 0.0, g_soft = -1.0;
 };
 
 struct AdjointState {
@@ -106,6 +111,9 @@ struct aster_ctx {
   aster::DBuf<int32_t> left, right, parent, escape;// 2n-1
   aster::DBuf<int32_t> counter;                    // n-1 arrival counters
   aster::DBuf<int32_t> depth;                      // 2n-1
+  aster::DBuf<int32_t> levels;                     // per-depth count, offset, fill (3 x 128)
+  aster::DBuf<int32_t> bucket;                     // internal nodes grouped by depth (n-1)
+  std::vector<int32_t> level_host;                 // per-depth counts on the host
   aster::DBuf<double> nmass, ncom, nvel;           // FP64 moments: 2n-1, [3][2n-1], [3][2n-1]
   aster::DBuf<float> nbox;                         // FP32 AABB [6][2n-1]
   aster::DBuf<float4> nrec;                        // FP32 node record: com, mass (2n-1)

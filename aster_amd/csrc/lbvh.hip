@@ -21,11 +21,15 @@
 //    next node of that same order, so the FP64 traversal accumulates in the
 //    reference's order and is bit-identical to get_node_body (:202-261).
 #include "common.hpp"
+#include "device.hpp"
 
 namespace aster {
 
 constexpr int kRsThreads = 256;
 constexpr int kRsTile = 2048;  // keys per block (8 per thread)
+// Depth bound of the radix tree: an internal node's split bit strictly
+// increases with depth over 30 code bits + 32 tie-break bits, so depth <= 63.
+constexpr int kMaxLevels = 128;
 
 // ------------------------------------------------------------ Morton -------
 
@@ -235,26 +239,76 @@ __global__ __launch_bounds__(256) void k_build(int n, const uint32_t* __restrict
   counter[i] = 0;
 }
 
-// Bottom-up moments: each leaf climbs; the second arrival at a node combines
-// both children (acquire/release on the arrival counter orders the data).
-__global__ __launch_bounds__(256) void k_moments(int n, const int32_t* __restrict__ left,
-                                                 const int32_t* __restrict__ right, const int32_t* __restrict__ parent,
-                                                 int32_t* __restrict__ counter, double* __restrict__ nmass,
-                                                 double* __restrict__ ncom, double* __restrict__ nvel,
-                                                 float* __restrict__ nbox, float4* __restrict__ nrec,
-                                                 uint32_t* __restrict__ flags) {
-#pragma clang fp contract(off)
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n || n < 2) return;
+// Depth of every node and its escape link in the right-first preorder (the
+// next node after its subtree: the left sibling if it is a right child, else
+// its parent's escape). Internal nodes are counted per depth for the
+// level-synchronous moment pass.
+__global__ __launch_bounds__(256) void k_depth(int n, const int32_t* __restrict__ left,
+                                               const int32_t* __restrict__ right, const int32_t* __restrict__ parent,
+                                               int32_t* __restrict__ depth, int32_t* __restrict__ escape,
+                                               int32_t* __restrict__ level_count, uint32_t* __restrict__ flags) {
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
   const int nn = 2 * n - 1;
-  int node = parent[k];
-  for (int guard = 0; node >= 0; ++guard) {
-    if (guard > nn) {
+  if (node >= nn) return;
+  int esc = -1, dep = 0;
+  int cur = node;
+  for (;;) {
+    const int p = parent[cur];
+    if (p < 0) break;
+    if (dep >= kMaxLevels - 1) {
       atomicOr(flags, (uint32_t)FLAG_TREE);
-      return;
+      break;
     }
-    const int old = __hip_atomic_fetch_add(&counter[node - n], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == 0) return;
+    if (esc < 0 && right[p] == cur) esc = left[p];
+    ++dep;
+    cur = p;
+  }
+  depth[node] = dep;
+  escape[node] = esc;
+  if (node >= n) atomicAdd(&level_count[dep], 1);
+}
+
+// Exclusive scan of the per-depth counts (kMaxLevels entries, one thread).
+__global__ void k_level_scan(const int32_t* __restrict__ level_count, int32_t* __restrict__ level_off,
+                             int32_t* __restrict__ level_fill) {
+  if (threadIdx.x != 0) return;
+  int32_t run = 0;
+  for (int d = 0; d < kMaxLevels; ++d) {
+    level_off[d] = run;
+    level_fill[d] = 0;
+    run += level_count[d];
+  }
+}
+
+__global__ void k_zero_i32(int32_t* p, int k) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < k) p[i] = 0;
+}
+
+// Internal nodes into per-depth buckets (order within a depth is irrelevant).
+__global__ __launch_bounds__(256) void k_bucket(int n, const int32_t* __restrict__ depth,
+                                                const int32_t* __restrict__ level_off, int32_t* __restrict__ level_fill,
+                                                int32_t* __restrict__ bucket) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  const int node = n + i;
+  const int d = depth[node];
+  bucket[level_off[d] + atomicAdd(&level_fill[d], 1)] = node;
+}
+
+// Monopole moments of the internal nodes of one depth; their children are one
+// level deeper and complete (kernel boundaries order the levels).
+__global__ __launch_bounds__(256) void k_moments_level(int n, const int32_t* __restrict__ bucket, int count,
+                                                       const int32_t* __restrict__ left,
+                                                       const int32_t* __restrict__ right, double* __restrict__ nmass,
+                                                       double* __restrict__ ncom, double* __restrict__ nvel,
+                                                       float* __restrict__ nbox, float4* __restrict__ nrec) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const int nn = 2 * n - 1;
+  {
+    const int node = bucket[t];
     const int L = left[node], R = right[node];
     const double mR = nmass[R], mL = nmass[L];
     const double total = mR + mL;
@@ -281,38 +335,24 @@ __global__ __launch_bounds__(256) void k_moments(int n, const int32_t* __restric
       nbox[(size_t)(3 + c) * nn + node] = fmaxf(nbox[(size_t)(3 + c) * nn + L], nbox[(size_t)(3 + c) * nn + R]);
     }
     nrec[node] = make_float4((float)com[0], (float)com[1], (float)com[2], (float)mass);
-    node = parent[node];
   }
 }
 
-// Per node: depth, escape link of the right-first preorder and the opening
-// threshold crit2 (accept when d^2 > crit2).
+// Per node: traversal record = right child (or -1), escape link, opening
+// threshold crit2 (accept when d^2 > crit2) and, for leaves, the body.
 // mac 0 (parity): crit2 = (s / theta)^2 with s = |box| 2^-depth.
 // mac 1 (geometric): crit2 = (l / theta + delta)^2, l = longest AABB side,
 //                    delta = |COM - AABB centre|.
-__global__ __launch_bounds__(256) void k_links(int n, const int32_t* __restrict__ left,
-                                               const int32_t* __restrict__ right, const int32_t* __restrict__ parent,
+__global__ __launch_bounds__(256) void k_links(int n, const int32_t* __restrict__ right,
+                                               const int32_t* __restrict__ escape, const int32_t* __restrict__ depth,
                                                const float* __restrict__ nbox, const float4* __restrict__ nrec,
                                                const double* __restrict__ boxd, double theta, int mac,
-                                               int32_t* __restrict__ depth, int4* __restrict__ nlink,
-                                               const uint32_t* __restrict__ perm, uint32_t* __restrict__ flags) {
+                                               int4* __restrict__ nlink, const uint32_t* __restrict__ perm) {
   const int node = blockIdx.x * blockDim.x + threadIdx.x;
   const int nn = 2 * n - 1;
   if (node >= nn) return;
-  int esc = -1, dep = 0;
-  int cur = node;
-  for (int guard = 0;; ++guard) {
-    const int p = parent[cur];
-    if (p < 0) break;
-    if (guard > nn) {
-      atomicOr(flags, (uint32_t)FLAG_TREE);
-      break;
-    }
-    if (esc < 0 && right[p] == cur) esc = left[p];
-    ++dep;
-    cur = p;
-  }
-  depth[node] = dep;
+  const int dep = depth[node];
+  const int esc = escape[node];
   float crit2;
   if (mac == 0) {
     const double s = ldexp(boxd[3], -dep);
@@ -396,62 +436,6 @@ __global__ __launch_bounds__(256) void k_bh_f32(int n, const float4* __restrict_
   }
 }
 
-// compute_gravitational (tool_chain_impl.cpp:120-137) in FP64, reference order.
-__device__ __forceinline__ void grav64(double m1, double m2, double v1x, double v1y, double v1z, double v2x,
-                                       double v2y, double v2z, double p1x, double p1y, double p1z, double p2x,
-                                       double p2y, double p2z, double G, double ic2, double soft, bool pn,
-                                       double& fx, double& fy, double& fz);
-
-__device__ __forceinline__ void pn1_term64(double m1, double m2, double v1x, double v1y, double v1z, double v2x,
-                                           double v2y, double v2z, double inv_r, double nx, double ny, double nz,
-                                           double G, double& ax, double& ay, double& az) {
-#pragma clang fp contract(off)
-  const double v2_sq = v2x * v2x + v2y * v2y + v2z * v2z;
-  const double v1_sq = v1x * v1x + v1y * v1y + v1z * v1z;
-  const double dot4 = (4.0 * v1x) * v2x + (4.0 * v1y) * v2y + (4.0 * v1z) * v2z;
-  const double nv2 = nx * v2x + ny * v2y + nz * v2z;
-  double s = -v1_sq - 2.0 * v2_sq;
-  s = s + dot4;
-  s = s + 1.5 * (nv2 * nv2);
-  s = s + 5.0 * G * m1 * inv_r;
-  s = s + 4.0 * G * m2 * inv_r;
-  ax = nx * s;
-  ay = ny * s;
-  az = nz * s;
-  const double n4v1 = (nx * 4.0) * v1x + (ny * 4.0) * v1y + (nz * 4.0) * v1z;
-  const double n3v2 = (nx * 3.0) * v2x + (ny * 3.0) * v2y + (nz * 3.0) * v2z;
-  const double t = n4v1 - n3v2;
-  ax = ax + (v1x - v2x) * t;
-  ay = ay + (v1y - v2y) * t;
-  az = az + (v1z - v2z) * t;
-  ax = ax * G * m2 * inv_r * inv_r;
-  ay = ay * G * m2 * inv_r * inv_r;
-  az = az * G * m2 * inv_r * inv_r;
-}
-
-__device__ __forceinline__ void grav64(double m1, double m2, double v1x, double v1y, double v1z, double v2x,
-                                       double v2y, double v2z, double p1x, double p1y, double p1z, double p2x,
-                                       double p2y, double p2z, double G, double ic2, double soft, bool pn,
-                                       double& fx, double& fy, double& fz) {
-#pragma clang fp contract(off)
-  const double dx = p2x - p1x, dy = p2y - p1y, dz = p2z - p1z;
-  const double r2 = dx * dx + dy * dy + dz * dz + soft;
-  const double inv_r = 1.0 / sqrt(r2);
-  const double inv_r2 = inv_r * inv_r;
-  const double nx = dx * inv_r, ny = dy * inv_r, nz = dz * inv_r;
-  const double scal = G * m2 * m1 * inv_r2;
-  fx = nx * scal;
-  fy = ny * scal;
-  fz = nz * scal;
-  if (pn) {
-    double px, py, pz;
-    pn1_term64(m1, m2, v1x, v1y, v1z, v2x, v2y, v2z, inv_r, -nx, -ny, -nz, G, px, py, pz);
-    fx = fx + (px * ic2) * m1;
-    fy = fy + (py * ic2) * m1;
-    fz = fz + (pz * ic2) * m1;
-  }
-}
-
 // FP64 per-body traversal in the reference's order and arithmetic
 // (get_node_body, barnes_hut.cpp:202-261).
 template <bool PN>
@@ -532,6 +516,132 @@ __global__ __launch_bounds__(256) void k_bh_f64(int n, const double* __restrict_
   }
 }
 
+// Wavefront-cooperative traversal. The 64 lanes hold 64 consecutive bodies in
+// Morton order (a spatially compact group); the wavefront walks the tree once
+// for the whole group with a shared LDS stack of opened nodes, testing up to
+// 64 candidate nodes per iteration (one per lane, so 64 independent node
+// loads are in flight). A node is accepted when the distance from its centre
+// of mass to the group's bounding box exceeds its opening radius (the
+// geometric test, so it holds for every body of the group); leaves are always
+// accepted. Accepted nodes are appended to an LDS interaction list that all
+// lanes then evaluate coherently (LDS broadcast reads, 12 VALU + 1 rsq per
+// interaction), FP32 sums per list flush, Kahan-compensated across flushes.
+// Stack bound: each iteration pops at most 32 opened nodes (64 children) and
+// pushes at most 64, and internal-node depth is at most 63, so fewer than
+// 32 * 63 + 64 entries are ever live.
+constexpr int kGrpStack = 2304;
+constexpr int kGrpList = 256;
+
+template <bool MASK>
+__global__ __launch_bounds__(64) void k_bh_group(int n, const float4* __restrict__ gsh,
+                                                 const float4* __restrict__ nrec, const int4* __restrict__ nlink,
+                                                 const int32_t* __restrict__ left, float eps2, double G,
+                                                 double* __restrict__ a, unsigned long long* __restrict__ counters,
+                                                 uint32_t* __restrict__ flags) {
+  __shared__ int stk[kGrpStack];
+  __shared__ float4 lst[kGrpList];
+  const int lane = threadIdx.x;
+  const int k = blockIdx.x * 64 + lane;
+  const bool live = k < n;
+  const float4 me = gsh[live ? k : n - 1];
+  float lo[3] = {me.x, me.y, me.z}, hi[3] = {me.x, me.y, me.z};
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    for (int off = 32; off > 0; off >>= 1) {
+      lo[c] = fminf(lo[c], __shfl_xor(lo[c], off, 64));
+      hi[c] = fmaxf(hi[c], __shfl_xor(hi[c], off, 64));
+    }
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  float sx = 0.f, sy = 0.f, sz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+  unsigned long long listed = 0, visits = 1;
+  int sp = 0, cnt = 0;
+  auto dmin2 = [&](const float4& r) {
+    const float ex = fmaxf(fmaxf(lo[0] - r.x, r.x - hi[0]), 0.f);
+    const float ey = fmaxf(fmaxf(lo[1] - r.y, r.y - hi[1]), 0.f);
+    const float ez = fmaxf(fmaxf(lo[2] - r.z, r.z - hi[2]), 0.f);
+    return fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+  };
+  auto flush = [&]() {
+    float tx = 0.f, ty = 0.f, tz = 0.f;
+    for (int e = 0; e < cnt; ++e) {
+      const float4 q = lst[e];
+      const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
+      float r2 = fmaf(dx, dx, eps2);
+      r2 = fmaf(dy, dy, r2);
+      r2 = fmaf(dz, dz, r2);
+      float inv = __builtin_amdgcn_rsqf(r2);
+      if (MASK) inv = (r2 > 0.f) ? inv : 0.f;
+      const float f = (q.w * inv) * (inv * inv);
+      tx = fmaf(f, dx, tx);
+      ty = fmaf(f, dy, ty);
+      tz = fmaf(f, dz, tz);
+    }
+    kahan_add32(sx, cx, tx);
+    kahan_add32(sy, cy, ty);
+    kahan_add32(sz, cz, tz);
+    listed += (unsigned long long)cnt;
+    cnt = 0;
+  };
+  {
+    const int root = n;
+    const float4 r = nrec[root];
+    if (dmin2(r) > __int_as_float(nlink[root].z)) {
+      if (lane == 0) lst[0] = r;
+      cnt = 1;
+    } else {
+      if (lane == 0) stk[0] = root;
+      sp = 1;
+    }
+  }
+  wave_sync();
+  while (sp > 0) {
+    const int take = min(sp, 32);
+    int node = -1;
+    if (lane < 2 * take) {
+      const int opened = stk[sp - take + (lane >> 1)];
+      node = (lane & 1) ? nlink[opened].x : left[opened];
+    }
+    sp -= take;
+    visits += 2 * take;
+    wave_sync();
+    bool acc = false, opn = false;
+    float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (node >= 0) {
+      r = nrec[node];
+      acc = node < n || dmin2(r) > __int_as_float(nlink[node].z);
+      opn = !acc;
+    }
+    const uint64_t am = __ballot(acc), om = __ballot(opn);
+    const int nopen = (int)__popcll(om);
+    if (sp + nopen > kGrpStack) {  // cannot happen for a valid tree (see bound above)
+      if (lane == 0) atomicOr(flags, (uint32_t)FLAG_TREE);
+      break;
+    }
+    if (acc) lst[cnt + __popcll(am & lt)] = r;
+    if (opn) stk[sp + __popcll(om & lt)] = node;
+    cnt += (int)__popcll(am);
+    sp += nopen;
+    wave_sync();
+    if (cnt > kGrpList - 64) {
+      flush();
+      wave_sync();
+    }
+  }
+  if (cnt > 0) flush();
+  if (live) {
+    const int b = nlink[k].w;
+    a[b] = G * ((double)sx - (double)cx);
+    a[n + b] = G * ((double)sy - (double)cy);
+    a[2 * n + b] = G * ((double)sz - (double)cz);
+  }
+  if (lane == 0) {
+    const unsigned long long nlive = (unsigned long long)min(64, n - blockIdx.x * 64);
+    // every body's own leaf is in its group's list (distance 0, no force)
+    atomicAdd(&counters[BHC_INTERACTIONS], listed * nlive - nlive);
+    atomicAdd(&counters[BHC_VISITS], visits);
+  }
+}
+
 __global__ void k_zero_counters(unsigned long long* p) {
   if (threadIdx.x < BHC_COUNT) p[threadIdx.x] = 0ull;
 }
@@ -558,6 +668,8 @@ static int reserve_tree(aster_ctx* c) {
   ASTER_HIP(c, c->parent.reserve(nn));
   ASTER_HIP(c, c->escape.reserve(nn));
   ASTER_HIP(c, c->depth.reserve(nn));
+  ASTER_HIP(c, c->levels.reserve(3 * kMaxLevels));
+  ASTER_HIP(c, c->bucket.reserve(n));
   ASTER_HIP(c, c->counter.reserve(n));
   ASTER_HIP(c, c->nmass.reserve(nn));
   ASTER_HIP(c, c->ncom.reserve(3 * nn));
@@ -630,16 +742,40 @@ static int build_tree(aster_ctx* c, int mac) {
                      c->nvel.p, c->nbox.p, c->nrec.p, c->gsh.p, c->flags.p);
   ASTER_HIP(c, hipGetLastError());
   phase_end(c, ASTER_PHASE_BUILD);
+  const int nn = 2 * n - 1;
   phase_begin(c, ASTER_PHASE_MOMENTS);
-  hipLaunchKernelGGL(k_moments, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->left.p, c->right.p,
-                     c->parent.p, c->counter.p, c->nmass.p, c->ncom.p, c->nvel.p, c->nbox.p, c->nrec.p, c->flags.p);
+  int32_t* lcount = c->levels.p;
+  int32_t* loff = c->levels.p + kMaxLevels;
+  int32_t* lfill = c->levels.p + 2 * kMaxLevels;
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(kMaxLevels), 0, c->stream, lcount, kMaxLevels);
+  hipLaunchKernelGGL(k_depth, dim3((nn + 255) / 256), dim3(256), 0, c->stream, n, c->left.p, c->right.p,
+                     c->parent.p, c->depth.p, c->escape.p, lcount, c->flags.p);
+  hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(64), 0, c->stream, lcount, loff, lfill);
+  hipLaunchKernelGGL(k_bucket, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->depth.p, loff, lfill,
+                     c->bucket.p);
+  ASTER_HIP(c, hipGetLastError());
+  c->level_host.resize(kMaxLevels);
+  ASTER_HIP(c, hipMemcpyAsync(c->level_host.data(), lcount, kMaxLevels * sizeof(int32_t), hipMemcpyDeviceToHost,
+                              c->stream));
+  ASTER_HIP(c, hipStreamSynchronize(c->stream));
+  int32_t off = 0;
+  std::vector<int32_t> offs(kMaxLevels);
+  for (int d = 0; d < kMaxLevels; ++d) {
+    offs[d] = off;
+    off += c->level_host[d];
+  }
+  if (off != n - 1) return fail(c, ASTER_ERR_STATE, "radix tree depth census does not cover every internal node");
+  for (int d = kMaxLevels - 1; d >= 0; --d) {
+    const int cnt = c->level_host[d];
+    if (cnt == 0) continue;
+    hipLaunchKernelGGL(k_moments_level, dim3((cnt + 255) / 256), dim3(256), 0, c->stream, n, c->bucket.p + offs[d],
+                       cnt, c->left.p, c->right.p, c->nmass.p, c->ncom.p, c->nvel.p, c->nbox.p, c->nrec.p);
+  }
   ASTER_HIP(c, hipGetLastError());
   phase_end(c, ASTER_PHASE_MOMENTS);
-  const int nn = 2 * n - 1;
   phase_begin(c, ASTER_PHASE_LINKS);
-  hipLaunchKernelGGL(k_links, dim3((nn + 255) / 256), dim3(256), 0, c->stream, n, c->left.p, c->right.p,
-                     c->parent.p, c->nbox.p, c->nrec.p, c->boxd.p, c->P.theta, mac, c->depth.p, c->nlink.p, c->idx.p,
-                     c->flags.p);
+  hipLaunchKernelGGL(k_links, dim3((nn + 255) / 256), dim3(256), 0, c->stream, n, c->right.p, c->escape.p,
+                     c->depth.p, c->nbox.p, c->nrec.p, c->boxd.p, c->P.theta, mac, c->nlink.p, c->idx.p);
   ASTER_HIP(c, hipGetLastError());
   phase_end(c, ASTER_PHASE_LINKS);
   c->tree_valid = true;
@@ -654,7 +790,6 @@ int bh_forces(aster_ctx* c, int method) {
   if (pn && !f64)
     return fail(c, ASTER_ERR_INVALID, "Barnes-Hut with 1PN needs ASTER_BH_PARITY with precision ASTER_FP64");
   if (!(c->P.theta >= 0.0)) return fail(c, ASTER_ERR_INVALID, "theta must be >= 0");
-  if (method == ASTER_BH_GROUP) return fail(c, ASTER_ERR_INVALID, "ASTER_BH_GROUP is not available in this build");
   if (c->P.lower > 0 || (c->P.upper >= 0 && c->P.upper < n))
     return fail(c, ASTER_ERR_INVALID, "Barnes-Hut evaluates the full body range only");
   if (n == 1) {
@@ -677,12 +812,22 @@ int bh_forces(aster_ctx* c, int method) {
                          c->P.softening, c->a.p, c->bh_counters.p, c->flags.p);
   } else {
     const float eps2 = (float)c->P.softening;
-    if (c->P.softening > 0.0)
-      hipLaunchKernelGGL((k_bh_f32<false>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->gsh.p, c->nrec.p,
-                         c->nlink.p, eps2, c->P.G, c->a.p, c->bh_counters.p, c->flags.p);
-    else
+    const bool mask = !(c->P.softening > 0.0);
+    if (method == ASTER_BH_GROUP) {
+      const dim3 grid((n + 63) / 64), block(64);
+      if (mask)
+        hipLaunchKernelGGL((k_bh_group<true>), grid, block, 0, c->stream, n, c->gsh.p, c->nrec.p, c->nlink.p,
+                           c->left.p, eps2, c->P.G, c->a.p, c->bh_counters.p, c->flags.p);
+      else
+        hipLaunchKernelGGL((k_bh_group<false>), grid, block, 0, c->stream, n, c->gsh.p, c->nrec.p, c->nlink.p,
+                           c->left.p, eps2, c->P.G, c->a.p, c->bh_counters.p, c->flags.p);
+    } else if (mask) {
       hipLaunchKernelGGL((k_bh_f32<true>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->gsh.p, c->nrec.p,
                          c->nlink.p, eps2, c->P.G, c->a.p, c->bh_counters.p, c->flags.p);
+    } else {
+      hipLaunchKernelGGL((k_bh_f32<false>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->gsh.p, c->nrec.p,
+                         c->nlink.p, eps2, c->P.G, c->a.p, c->bh_counters.p, c->flags.p);
+    }
   }
   ASTER_HIP(c, hipGetLastError());
   phase_end(c, ASTER_PHASE_TRAVERSE);

@@ -123,9 +123,11 @@ int aster_get_acc(aster_ctx* ctx, double* acc);
 
 /* Advance nsteps with `integrator` and force `method` entirely on the device
  * (update_leapfrog, tool_chain_impl.cpp:169-185; Yoshida-4 is the composition
- * of three such steps with w1 dt, w0 dt, w1 dt). Accelerations are cached
- * between steps, so a leapfrog step costs one force evaluation. Positions and
- * velocities are synchronised at the end of the call. */
+ * of three such steps with w1 dt, w0 dt, w1 dt). Newtonian accelerations
+ * depend on positions only and are cached between steps, so a leapfrog step
+ * costs one force evaluation; with ASTER_PN1 (velocity-dependent) each step
+ * keeps the reference's two evaluations. Positions and velocities are
+ * synchronised at the end of the call. */
 int aster_step(aster_ctx* ctx, int method, int integrator, double dt, int64_t nsteps);
 
 /* Yoshida weights used by ASTER_YOSHIDA4: w[0] = w1, w[1] = w0, w[2] = w1. */
@@ -188,16 +190,21 @@ int aster_batch_set(aster_ctx* ctx, int64_t nsys, const int64_t* offsets, const 
                     const double* mass);
 int aster_batch_get(aster_ctx* ctx, double* pos, double* vel);
 /* Integrate every system nsteps with all-pairs FP64 forces (Newton, plus
- * pairwise 1PN if pn_flags has ASTER_PN1) in a persistent kernel, one
- * wavefront per system; launches of steps_per_launch steps (0 = tuned) are
+ * pairwise 1PN if pn_flags has ASTER_PN1, Plummer-softened by softening) in a
+ * persistent kernel: each system lives in the lanes of one wavefront (several
+ * small systems may share one). precision ASTER_FP64 evaluates
+ * compute_gravitational in the reference's operation order (bit-identical to
+ * the CPU path per system); ASTER_FP32 selects the fast FP64 path (rsqrt
+ * estimate + Newton steps). Launches of steps_per_launch steps (0 = tuned) are
  * captured once in a HIP graph and replayed when use_graph != 0. */
 int aster_batch_step(aster_ctx* ctx, int integrator, double dt, int64_t nsteps, int32_t steps_per_launch,
                      int32_t use_graph);
 /* Per-system energy (kinetic + Plummer-softened potential) and angular
  * momentum: e[nsys], L[3 nsys]; either may be NULL. */
 int aster_batch_energy(aster_ctx* ctx, double* e, double* L);
-/* Time candidate launch shapes on the loaded batch and keep the fastest.
- * Writes the chosen systems-per-block and steps-per-launch. */
+/* Time candidate launch shapes (packed or one system per wavefront; 1, 2 or
+ * 4 wavefronts per block) on the loaded batch, restoring its state, and keep
+ * the fastest. Writes the chosen wavefronts per block and steps per launch. */
 int aster_batch_autotune(aster_ctx* ctx, int integrator, double dt, int32_t* systems_per_block,
                          int32_t* steps_per_launch);
 

@@ -119,6 +119,20 @@ def test_yoshida_trajectory_bit_identical(ctx, orc):
     assert np.array_equal(A.yoshida_weights(), orc.yoshida_weights())
 
 
+@pytest.mark.parametrize("integrator,oi", [(A.LEAPFROG, 2), (A.YOSHIDA4, 5)])
+def test_pn1_trajectory_bit_identical(ctx, orc, integrator, oi):
+    # Velocity-dependent forces: two force evaluations per (sub)step, as the
+    # reference's update_leapfrog.
+    pos, vel, mass = plummer(256, seed=12)
+    vel = vel * 20.0
+    ctx.set_params(G=1.0, c=40.0, softening=1e-4, pn_flags=A.NEWTON | A.PN1)
+    ctx.set_state(pos, vel, mass)
+    ctx.step(A.ALLPAIRS_FP64, integrator, 1e-3, 4)
+    p, v = ctx.get_state()
+    rp, rv, _ = orc.integrate(pos, vel, mass, 1e-3, 4, integrator=oi, c=40.0, soft=1e-4, pn=0x11)
+    assert np.array_equal(p, rp) and np.array_equal(v, rv)
+
+
 def test_mixed_precision_trajectory(ctx, orc):
     # FP32 forces + FP64 state vs the all-FP64 path over 50 Yoshida steps.
     pos, vel, mass = plummer(4096, seed=10, mass_mode=1)

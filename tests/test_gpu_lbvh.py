@@ -46,7 +46,7 @@ def test_morton_user_box_and_out_of_box(ctx, orc):
     load(ctx, pos, vel, mass, box=(1366.0, 1366.0, 1366.0))
     codes, box = ctx.morton()
     assert np.array_equal(codes, orc.morton(pos, box))
-    assert len(np.unique(codes)) < 64
+    assert len(np.unique(codes)) < len(codes) // 10
     load(ctx, pos, vel, mass, box=(1.0, 1.0, 1.0))
     with pytest.raises(A.AsterError) as ei:
         ctx.morton()
@@ -109,20 +109,52 @@ def test_bh_fp32_parity_close_to_oracle(ctx, orc):
     assert np.median(e) < 1e-5 and np.quantile(e, 0.99) < 1e-3
 
 
-@pytest.mark.parametrize("method", [A.BH_GEOMETRIC])
-def test_bh_accuracy_vs_allpairs(ctx, method):
-    # Force error against FP64 all-pairs at each theta (mixed Plummer + cube).
-    pos, vel, mass = mixed(1 << 16, seed=7)
+def test_bh_group_theta_zero_is_allpairs(ctx, orc):
+    pos, vel, mass = mixed(3000, seed=9)
+    load(ctx, pos, vel, mass, theta=0.0)
+    acc = ctx.forces(A.BH_GROUP)
+    inter, _ = ctx.bh_stats()
+    ref = orc.allpairs(pos, vel, mass, soft=2.5e-5)
+    e = rel_err(acc, ref)
+    assert np.median(e) < 1e-6 and e.max() < 1e-4
+    assert inter == 3000 * 2999
+
+
+def test_bh_group_handles_partial_wavefronts_and_tiny_n(ctx, orc):
+    for n in (2, 3, 65, 130):
+        pos, vel, mass = mixed(n, seed=n)
+        load(ctx, pos, vel, mass, theta=0.5)
+        e = rel_err(ctx.forces(A.BH_GROUP), orc.allpairs(pos, vel, mass, soft=2.5e-5))
+        assert np.median(e) < 1e-2, n
+
+
+def force_errors(ctx, pos, vel, mass, method, thetas):
     load(ctx, pos, vel, mass)
     ref = ctx.forces(A.ALLPAIRS_FP64)
     out = {}
-    for theta in (0.3, 0.5, 0.7):
+    for theta in thetas:
         load(ctx, pos, vel, mass, theta=theta)
         e = rel_err(ctx.forces(method), ref)
-        out[theta] = (np.median(e), np.quantile(e, 0.99), np.sqrt(np.mean(e ** 2)))
-        print(f"\nmethod {method} theta {theta}: median {out[theta][0]:.2e} p99 {out[theta][1]:.2e} rms {out[theta][2]:.2e}")
-    assert out[0.3][0] < out[0.5][0] < out[0.7][0]
-    assert out[0.5][2] <= 1e-3
+        inter, _ = ctx.bh_stats()
+        out[theta] = dict(median=np.median(e), p99=np.quantile(e, 0.99), rms=np.sqrt(np.mean(e ** 2)),
+                          inter=inter / len(mass))
+        print(f"\nmethod {method} theta {theta}: " + " ".join(f"{k} {v:.3e}" for k, v in out[theta].items()))
+    return out
+
+
+def test_bh_accuracy_vs_allpairs(ctx):
+    # Force error of the FP32 traversals against FP64 all-pairs at each theta
+    # (mixed Plummer + cube, 2^16 bodies). Tolerances: errors grow with theta;
+    # the group traversal (box-to-node test) is at least as accurate as the
+    # per-body geometric test; group median error at theta = 0.5 <= 1e-3.
+    pos, vel, mass = mixed(1 << 16, seed=7)
+    geo = force_errors(ctx, pos, vel, mass, A.BH_GEOMETRIC, (0.3, 0.5, 0.7))
+    grp = force_errors(ctx, pos, vel, mass, A.BH_GROUP, (0.3, 0.5, 0.7))
+    for out in (geo, grp):
+        assert out[0.3]["median"] < out[0.5]["median"] < out[0.7]["median"]
+    for t in (0.3, 0.5, 0.7):
+        assert grp[t]["median"] <= geo[t]["median"]
+    assert grp[0.5]["median"] <= 1e-3
 
 
 def test_bh_leapfrog_steps(ctx, orc):

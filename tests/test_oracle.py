@@ -368,6 +368,48 @@ def test_yoshida_weights(orc):
     assert 2 * w[0] ** 3 + w[1] ** 3 == pytest.approx(0.0, abs=1e-14)  # third-order condition
 
 
+def test_pn1_periastron_advance(orc):
+    # Integration-level known answer for the pairwise 1PN term: a test mass
+    # around M = 1 precesses by 6 pi G M / (c^2 a (1 - e^2)) per orbit.
+    a, e, c = 1.0, 0.3, 40.0
+    rp = a * (1 - e)
+    vp = np.sqrt((1 + e) / (a * (1 - e)))
+    pos = np.array([[0.0, 0.0, 0.0], [rp, 0.0, 0.0]])
+    vel = np.array([[0.0, 0.0, 0.0], [0.0, vp, 0.0]])
+    mass = np.array([1.0, 1e-9])
+    period = 2 * np.pi * a ** 1.5
+    dt = period / 2000
+
+    def step(p, v, pn):
+        return orc.integrate(p, v, mass, dt, 1, integrator=5, c=c, soft=0.0, pn=pn, threads=1)[:2]
+
+    expect = 6 * np.pi / (c ** 2 * a * (1 - e * e))
+    for pn, norbit_expect in ((0x11, expect), (0x1, 0.0)):
+        angles = periastron_angles(step, pos, vel, pn, 3 * 2000 + 200)
+        assert len(angles) == 3
+        adv = np.diff(np.concatenate([[0.0], angles]))
+        assert adv == pytest.approx(np.full(3, norbit_expect), abs=0.01 * expect)
+
+
+def periastron_angles(step, pos, vel, pn, nsteps):
+    """Directions of the relative position at each periastron passage (radial
+    velocity crossing zero from below), linearly interpolated between steps."""
+    out = []
+    p, v = pos, vel
+    rv_prev, ang_prev = None, None
+    for _ in range(nsteps):
+        p, v = step(p, v, pn)
+        r, w = p[1] - p[0], v[1] - v[0]
+        rv = r @ w
+        ang = np.arctan2(r[1], r[0])
+        if rv_prev is not None and rv_prev < 0 <= rv:
+            f = -rv_prev / (rv - rv_prev)
+            da = np.angle(np.exp(1j * (ang - ang_prev)))
+            out.append(ang_prev + f * da)
+        rv_prev, ang_prev = rv, ang
+    return np.array(out)
+
+
 def test_leapfrog_energy_bounded(orc):
     import aster_amd
     pos, vel, mass = aster_amd.ic_plummer(128, 3)
