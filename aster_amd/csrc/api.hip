@@ -151,8 +151,8 @@ int aster_destroy(aster_ctx* c) {
   Bt.x.release(); Bt.v.release(); Bt.a.release(); Bt.m.release(); Bt.ebody.release(); Bt.esys.release();
   Bt.doff.release(); Bt.lane_body.release(); Bt.lane_seg.release(); Bt.wave_max.release();
   auto& A = c->adj;
-  A.ckpt_x.release(); A.ckpt_v.release(); A.x0.release(); A.v0.release(); A.lx.release(); A.lv.release();
-  A.lm.release(); A.w.release(); A.snapg.release(); A.aprev.release(); A.part.release();
+  A.ckpt_x.release(); A.ckpt_v.release(); A.lx.release(); A.lv.release(); A.lm.release(); A.w.release();
+  A.snapg.release(); A.g.release(); A.wsrc.release(); A.part.release();
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return ASTER_OK;

@@ -71,12 +71,12 @@ struct AdjointState {
   double dt = 0.0;
   std::vector<int64_t> snap_steps;
   DBuf<double> ckpt_x, ckpt_v;  // (nsteps/ckpt_every + 1) * 3n each
-  DBuf<double> x0, v0;          // initial state (3n each)
   DBuf<double> lx, lv, lm;      // adjoint accumulators (3n, 3n, n)
   DBuf<double> w;               // weights for the fused VJP (3n)
-  DBuf<double> snapg;           // snapshot gradients (nsnap * 3n)
-  DBuf<double> aprev;           // a(x_{s+1}) during the reverse sweep (3n)
-  DBuf<double> part;            // partial VJP outputs: slices * 7 * n
+  DBuf<double> snapg;           // snapshot positions, then their gradients (nsnap * 3n)
+  DBuf<double> g;               // J^T w (3n) and M^T w (n) of the current step
+  DBuf<float4> wsrc;            // FP32 {w, m} per source, npad
+  DBuf<double> part;            // slice partials: slices * 7 * n
   bool ready = false;
 };
 
@@ -171,6 +171,7 @@ int traj_forward(aster_ctx* c, double dt, int64_t nsteps, int64_t ckpt, int32_t 
                  double* snap_pos);
 int traj_backward(aster_ctx* c, const double* gsnap, double* gx, double* gv, double* gm);
 int choose_split(aster_ctx* c);
+int compute_forces(aster_ctx* c, int method);             // all-pairs or Barnes-Hut into c->a
 void phase_begin(aster_ctx* c, int phase);
 void phase_end(aster_ctx* c, int phase);
 
