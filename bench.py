@@ -95,46 +95,112 @@ def bench_bh(A, ctx, steps, warmup, method, n=1 << 20, theta=0.5, dt=5e-4):
 
 
 def bench_allpairs(A, ctx, evals, n=65536):
+    """All-pairs at the mixed-precision configuration's shape (Plummer N=65536,
+    log-uniform masses, eps=0.005): both FP32 methods, timed per kernel."""
     pos, vel, mass = A.ic_plummer(n, 3, mass_mode=1)
     ctx.set_params(G=1.0, softening=0.005 ** 2, theta=0.5)
     ctx.set_state(pos, vel, mass)
     ref = ctx.forces(A.ALLPAIRS_FP64)
-    acc = ctx.forces(A.ALLPAIRS_FP32_KAHAN)
-    errs = err_stats(rel_err(acc, ref))
-    for _ in range(3):
-        ctx.forces(A.ALLPAIRS_FP32_KAHAN, out=False)
-    ctx.profile(True)
+    out = {}
+    for name, method, vop in (("fp32_mixed_kahan", A.ALLPAIRS_FP32_KAHAN, 15), ("fp32", A.ALLPAIRS_FP32, 12)):
+        errs = err_stats(rel_err(ctx.forces(method), ref))
+        for _ in range(3):
+            ctx.forces(method, out=False)
+        ctx.profile(True)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(evals):
+            ctx.forces(method, out=False)
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        phases = ctx.phase_times()
+        ctx.profile(False)
+        eval_ms = sum(v[0] for v in phases.values()) / evals
+        kernel_ms = phases["allpairs"][0] / phases["allpairs"][1]
+        inter = float(n) * (n - 1)
+        achieved = 20.0 * inter / (kernel_ms * 1e-3) / 1e12
+        log(f"[bench] all-pairs {name} n={n}: kernel {kernel_ms:.3f} ms, eval {eval_ms:.3f} ms, "
+            f"{achieved:.1f} TFLOP/s (20-flop convention), err {errs}")
+        out[name] = {
+            "value": inter / (eval_ms * 1e-3),
+            "unit": "interactions/s",
+            "ms_per_eval": eval_ms,
+            "wall_ms_per_eval": wall / evals * 1e3,
+            "kernel_ms": kernel_ms,
+            "force_error_vs_fp64": errs,
+            "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": "k_allpairs_f32",
+                         "flop_per_interaction": 20, "valu_instructions_per_interaction": vop + 1},
+        }
+    res = dict(out["fp32_mixed_kahan"])
+    res["n"] = n
+    res["method"] = "ALLPAIRS_FP32_KAHAN (FP64 separations, FP32 pair force, Kahan FP32 sums)"
+    res["fp32_plain"] = out["fp32"]
+    return res, pos, vel, mass
+
+
+def bench_adjoint(A, ctx, n=65536, T=512, ckpt=64, dt=1e-3):
+    """Differentiable-trajectory configuration: Plummer N=65536, FP32 forces /
+    FP64 state, leapfrog dt=1e-3 over T=512 steps, 8 snapshots, checkpoints
+    every 64 steps. Ratio = one backward step / one forward step."""
+    pos, vel, mass = A.ic_plummer(n, 5)
+    rng = np.random.default_rng(5)
+    snaps = np.arange(1, 9) * (T // 8)
+    ctx.set_params(G=1.0, softening=0.005 ** 2)
+    ctx.set_state(pos + rng.normal(scale=1e-3, size=pos.shape), vel + rng.normal(scale=1e-3, size=vel.shape), mass)
+    target = ctx.traj_forward(dt, T, ckpt, snaps)
+    ctx.set_state(pos, vel, mass)
     ctx.sync()
     t0 = time.perf_counter()
-    for _ in range(evals):
-        ctx.forces(A.ALLPAIRS_FP32_KAHAN, out=False)
+    snap_pos = ctx.traj_forward(dt, T, ckpt, snaps)
+    ctx.sync()
+    t_fwd = time.perf_counter() - t0
+    d = snap_pos - target
+    loss = float((d ** 2).sum() / (len(snaps) * n))
+    gsnap = 2.0 * d / (len(snaps) * n)
+    t0 = time.perf_counter()
+    gx, gv, gm = ctx.traj_backward(gsnap)
+    t_bwd = time.perf_counter() - t0
+    ratio = t_bwd / t_fwd
+    log(f"[bench] adjoint n={n} T={T}: forward {t_fwd:.3f}s backward {t_bwd:.3f}s ratio {ratio:.3f}, loss {loss:.3e}")
+    return {"n": n, "steps": T, "checkpoint_every": ckpt, "forward_s": t_fwd, "backward_s": t_bwd,
+            "forward_ms_per_step": t_fwd / T * 1e3, "backward_ms_per_step": t_bwd / T * 1e3,
+            "ratio": ratio, "loss": loss, "grad_norms": [float(np.linalg.norm(g)) for g in (gx, gv, gm)]}
+
+
+def bench_batch(A, ctx, nsys=8192, steps=10000, dt=1e-3):
+    """Planetary ensembles: 8192 systems of N in [3, 64], 1PN (c=1e4) in FP64,
+    Yoshida-4 dt=1e-3 for 10^4 steps in captured launches."""
+    off, pos, vel, mass = A.ic_ensemble(nsys, 11, 3, 64)
+    ctx.set_params(G=1.0, softening=0.0, c=1e4, pn_flags=A.NEWTON | A.PN1, precision=A.FP32)
+    ctx.batch_set(off, pos, vel, mass)
+    e0, L0 = ctx.batch_energy()
+    wpb, spl = ctx.batch_autotune(A.YOSHIDA4, dt)
+    ctx.batch_step(A.YOSHIDA4, dt, 20, steps_per_launch=spl)  # warm + capture of a short graph
+    ctx.batch_set(off, pos, vel, mass)
+    ctx.batch_step(A.YOSHIDA4, dt, 0)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.batch_step(A.YOSHIDA4, dt, steps, steps_per_launch=spl, use_graph=True)
     ctx.sync()
     wall = time.perf_counter() - t0
-    phases = ctx.phase_times()
-    ctx.profile(False)
-    ms_k, cnt = phases["allpairs"]
-    ms_c, _ = phases.get("combine", (0.0, 1))
-    kernel_ms = ms_k / cnt
-    inter = float(n) * (n - 1)
-    flops = 20.0 * inter
-    achieved = flops / (kernel_ms * 1e-3) / 1e12
-    log(f"[bench] all-pairs n={n}: kernel {kernel_ms:.3f} ms, combine {ms_c / cnt:.4f} ms, "
-        f"{achieved:.1f} TFLOP/s, err {errs}")
-    return {
-        "n": n,
-        "value": inter / ((ms_k + ms_c) / cnt * 1e-3),
-        "unit": "interactions/s",
-        "ms_per_eval": (ms_k + ms_c) / cnt,
-        "wall_ms_per_eval": wall / evals * 1e3,
-        "kernel_ms": kernel_ms,
-        "force_error_vs_fp64": errs,
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_allpairs_f32<TILE_KAHAN>", "flop_per_interaction": 20},
-    }, pos, vel, mass
+    e1, L1 = ctx.batch_energy()
+    de = np.abs(e1 - e0) / np.abs(e0)
+    sizes = np.diff(off)
+    pairs = float((sizes * (sizes - 1)).sum())
+    # two force evaluations per Yoshida substep with velocity-dependent forces
+    inter = pairs * steps * 3 * 2
+    log(f"[bench] batch {nsys} systems x {steps} steps: {wall:.3f}s, {inter / wall:.3e} pair interactions/s, "
+        f"wpb {wpb} spl {spl}, |dE/E| median {np.median(de):.2e}")
+    return {"systems": nsys, "bodies": int(off[-1]), "steps": steps, "wall_s": wall,
+            "system_steps_per_s": nsys * steps / wall, "pair_interactions_per_s": inter / wall,
+            "waves_per_block": wpb, "steps_per_launch": spl,
+            "energy_drift": {"median": float(np.median(de)), "p90": float(np.quantile(de, 0.9)),
+                             "max": float(de.max())},
+            "L_drift_max": float((np.abs(L1 - L0).max(1) / np.abs(L0).max(1)).max())}
 
 
-def cpu_baseline_bh(orc, A, threads, n=1 << 20, theta=0.5, stride=16):
+def cpu_baseline_bh(orc, A, threads, n=1 << 20, theta=0.5, stride=1):
     pos, vel, mass = A.ic_mixed(n, 20240)
     box = orc.box_auto(pos)
     tb, tt, inter = orc.bh_sample(pos, vel, mass, box, theta, stride, soft=0.005 ** 2, threads=threads)
@@ -164,6 +230,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--method", default="BH_GROUP")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward ratio")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched ensemble")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, "
@@ -174,6 +242,8 @@ def main():
     method = getattr(A, args.method)
     bh = bench_bh(A, ctx, args.steps, args.warmup, method)
     ap_res, pos64k, vel64k, mass64k = bench_allpairs(A, ctx, max(args.steps, 10))
+    adj = None if args.no_adjoint else bench_adjoint(A, ctx)
+    batch = None if args.no_batch else bench_batch(A, ctx)
 
     # roofline of the dominant kernel of the headline step: the traversal
     ms_t, cnt_t = bh["phases"]["traverse"]
@@ -220,7 +290,9 @@ def main():
         "phases_ms": phases,
         "event_ms_per_step": bh["event_ms"] / args.steps,
         "allpairs": ap_res,
-        "adjoint_forward_ratio": None,
+        "adjoint_forward_ratio": None if adj is None else adj["ratio"],
+        "adjoint": adj,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
 
