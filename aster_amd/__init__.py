@@ -102,6 +102,8 @@ SIGNATURES = {
     "aster_batch_step": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, _I64, _I32, _I32]),
     "aster_batch_energy": (ctypes.c_int, [_P, _D, _D]),
     "aster_batch_autotune": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, _PI32, _PI32]),
+    "aster_batch_traj_forward": (ctypes.c_int, [_P, ctypes.c_double, _I64, _I32, _PI64, _D]),
+    "aster_batch_traj_backward": (ctypes.c_int, [_P, _D, _D, _D, _D]),
     "aster_traj_forward": (ctypes.c_int, [_P, ctypes.c_double, _I64, _I64, _I32, _PI64, _D]),
     "aster_traj_backward": (ctypes.c_int, [_P, _D, _D, _D, _D]),
     "aster_ic_plummer": (ctypes.c_int, [_I64, _U64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -308,6 +310,21 @@ class Context:
         spl = ctypes.c_int32()
         self._ck(lib().aster_batch_autotune(self.h, integrator, float(dt), ctypes.byref(spb), ctypes.byref(spl)))
         return spb.value, spl.value
+
+    def batch_traj_forward(self, dt, nsteps, snap_steps):
+        snap = np.ascontiguousarray(snap_steps, np.int64)
+        out = np.empty((len(snap), self.batch_n, 3))
+        self._ck(lib().aster_batch_traj_forward(self.h, float(dt), int(nsteps), len(snap),
+                                                snap.ctypes.data_as(_PI64), _dp(out)))
+        return out
+
+    def batch_traj_backward(self, grad_snap):
+        g = _f64(grad_snap)
+        gx = np.empty((self.batch_n, 3))
+        gv = np.empty((self.batch_n, 3))
+        gm = np.empty(self.batch_n)
+        self._ck(lib().aster_batch_traj_backward(self.h, _dp(g), _dp(gx), _dp(gv), _dp(gm)))
+        return gx, gv, gm
 
     # ---- trajectory adjoint
     def traj_forward(self, dt, nsteps, ckpt_every, snap_steps, want_pos=True):

@@ -150,6 +150,7 @@ int aster_destroy(aster_ctx* c) {
   auto& Bt = c->batch;
   Bt.x.release(); Bt.v.release(); Bt.a.release(); Bt.m.release(); Bt.ebody.release(); Bt.esys.release();
   Bt.doff.release(); Bt.lane_body.release(); Bt.lane_seg.release(); Bt.wave_max.release();
+  Bt.traj.release(); Bt.x0.release(); Bt.v0.release(); Bt.tgrad.release(); Bt.tout.release(); Bt.tsnap.release();
   auto& A = c->adj;
   A.ckpt_x.release(); A.ckpt_v.release(); A.lx.release(); A.lv.release(); A.lm.release(); A.w.release();
   A.snapg.release(); A.g.release(); A.wsrc.release(); A.part.release();

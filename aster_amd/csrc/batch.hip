@@ -484,6 +484,219 @@ int batch_autotune(aster_ctx* c, int integrator, double dt, int32_t* spb, int32_
   return ASTER_OK;
 }
 
+// ------------------------------------------------- ensemble trajectory adjoint
+// Leapfrog with Newtonian FP64 forces (fast mode arithmetic); the positions of
+// every step are stored (small systems: 4096 three-body systems x 513 steps is
+// 150 MB), so the backward pass evaluates each vector-Jacobian product at the
+// exact forward positions. Recursion as in adjoint.hip:
+//   lvh = lv + dt lx ; w = c_s lvh (c_s = dt, c_0 = dt/2)
+//   lx += J(x_s)^T w [+ dL/dx_s] ; lm += M(x_s)^T w ; lv = lvh
+// with, per body k and partner j in its system (d = x_j - x_k, s^2 = |d|^2 + eps^2):
+//   J^T w : G sum_j [u/s^3 - 3 d (d.u)/s^5],  u = m_k w_j - m_j w_k
+//   M^T w : -G sum_j (w_j . d)/s^3
+
+__global__ __launch_bounds__(256) void k_batch_traj_fwd(BatchArgs P, double* __restrict__ traj, int64_t nbodies) {
+  __shared__ double sx[kBatchMaxWpb][64], sy[kBatchMaxWpb][64], sz[kBatchMaxWpb][64], sm[kBatchMaxWpb][64];
+  const int wl = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * (blockDim.x >> 6) + wl;
+  if (wave >= P.nwaves) return;
+  const int slot = wave * 64 + lane;
+  const int b = P.lane_body[slot];
+  const int seg = P.lane_seg[slot];
+  const int base = seg >> 8, cnt = seg & 255;
+  const int maxcnt = P.wave_max[wave];
+  const bool live = b >= 0;
+  double X[3] = {0.0, 0.0, 0.0}, V[3] = {0.0, 0.0, 0.0}, A[3] = {0.0, 0.0, 0.0};
+  double mi = 0.0;
+  if (live) {
+    for (int c = 0; c < 3; ++c) {
+      X[c] = P.x[3 * (size_t)b + c];
+      V[c] = P.v[3 * (size_t)b + c];
+    }
+    mi = P.m[b];
+  }
+  sm[wl][lane] = mi;
+  auto publish = [&]() {
+    sx[wl][lane] = X[0];
+    sy[wl][lane] = X[1];
+    sz[wl][lane] = X[2];
+    wave_sync();
+  };
+  auto store = [&](int64_t s) {
+    if (live)
+      for (int c = 0; c < 3; ++c) traj[((size_t)s * nbodies + b) * 3 + c] = X[c];
+  };
+  publish();
+  batch_accel<false, false>(lane, base, cnt, maxcnt, live, sx[wl], sy[wl], sz[wl], sm[wl], nullptr, nullptr,
+                            nullptr, X, V, mi, P.G, P.ic2, P.soft, A);
+  wave_sync();
+  store(0);
+  const double dt = P.h[0];
+  double pending = 0.0;
+  for (int64_t s = 0; s < P.nsteps; ++s) {
+    kick_drift_reg(X, V, A, pending, dt);
+    publish();
+    batch_accel<false, false>(lane, base, cnt, maxcnt, live, sx[wl], sy[wl], sz[wl], sm[wl], nullptr, nullptr,
+                              nullptr, X, V, mi, P.G, P.ic2, P.soft, A);
+    wave_sync();
+    pending = dt;
+    store(s + 1);
+  }
+  if (P.nsteps > 0) kick_reg(V, A, pending);
+  if (live)
+    for (int c = 0; c < 3; ++c) {
+      P.x[3 * (size_t)b + c] = X[c];
+      P.v[3 * (size_t)b + c] = V[c];
+      P.a[3 * (size_t)b + c] = A[c];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_batch_traj_bwd(BatchArgs P, const double* __restrict__ traj,
+                                                        int64_t nbodies, int64_t nsteps, double dt,
+                                                        const double* __restrict__ gsnap,
+                                                        const int64_t* __restrict__ snaps, int nsnap,
+                                                        double* __restrict__ out) {
+  __shared__ double sx[kBatchMaxWpb][64], sy[kBatchMaxWpb][64], sz[kBatchMaxWpb][64], sm[kBatchMaxWpb][64];
+  __shared__ double swx[kBatchMaxWpb][64], swy[kBatchMaxWpb][64], swz[kBatchMaxWpb][64];
+  const int wl = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * (blockDim.x >> 6) + wl;
+  if (wave >= P.nwaves) return;
+  const int slot = wave * 64 + lane;
+  const int b = P.lane_body[slot];
+  const int seg = P.lane_seg[slot];
+  const int base = seg >> 8, cnt = seg & 255;
+  const int maxcnt = P.wave_max[wave];
+  const bool live = b >= 0;
+  const double mk = live ? P.m[b] : 0.0;
+  sm[wl][lane] = mk;
+  double lx[3] = {0.0, 0.0, 0.0}, lv[3] = {0.0, 0.0, 0.0}, lm = 0.0;
+  int q = nsnap - 1;
+  if (q >= 0 && snaps[q] == nsteps) {
+    if (live)
+      for (int c = 0; c < 3; ++c) lx[c] = gsnap[((size_t)q * nbodies + b) * 3 + c];
+    --q;
+  }
+  for (int64_t s = nsteps - 1; s >= 0; --s) {
+    const double cw = (s > 0) ? dt : 0.5 * dt;
+    double w[3], X[3];
+    for (int c = 0; c < 3; ++c) {
+      lv[c] = lv[c] + dt * lx[c];
+      w[c] = cw * lv[c];
+      X[c] = live ? traj[((size_t)s * nbodies + b) * 3 + c] : 0.0;
+    }
+    sx[wl][lane] = X[0];
+    sy[wl][lane] = X[1];
+    sz[wl][lane] = X[2];
+    swx[wl][lane] = w[0];
+    swy[wl][lane] = w[1];
+    swz[wl][lane] = w[2];
+    wave_sync();
+    double gx = 0.0, gy = 0.0, gz = 0.0, gm = 0.0;
+    for (int j = 0; j < maxcnt; ++j) {
+      const int src = min(base + j, 63);
+      const bool valid = live && j < cnt && src != lane;
+      const double dx = sx[wl][src] - X[0], dy = sy[wl][src] - X[1], dz = sz[wl][src] - X[2];
+      const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, P.soft)));
+      const double inv = rsqrt64(valid ? r2 : 1.0);
+      const double inv3 = valid ? inv * inv * inv : 0.0;
+      const double inv5 = inv3 * inv * inv;
+      const double mj = sm[wl][src];
+      const double wjx = swx[wl][src], wjy = swy[wl][src], wjz = swz[wl][src];
+      const double ux = mk * wjx - mj * w[0], uy = mk * wjy - mj * w[1], uz = mk * wjz - mj * w[2];
+      const double du = dx * ux + dy * uy + dz * uz;
+      gx += ux * inv3 - 3.0 * dx * du * inv5;
+      gy += uy * inv3 - 3.0 * dy * du * inv5;
+      gz += uz * inv3 - 3.0 * dz * du * inv5;
+      gm -= (wjx * dx + wjy * dy + wjz * dz) * inv3;
+    }
+    wave_sync();
+    lx[0] += P.G * gx;
+    lx[1] += P.G * gy;
+    lx[2] += P.G * gz;
+    lm += P.G * gm;
+    if (q >= 0 && snaps[q] == s) {
+      if (live)
+        for (int c = 0; c < 3; ++c) lx[c] += gsnap[((size_t)q * nbodies + b) * 3 + c];
+      --q;
+    }
+  }
+  if (live) {
+    for (int c = 0; c < 3; ++c) {
+      out[3 * (size_t)b + c] = lx[c];
+      out[3 * (size_t)nbodies + 3 * (size_t)b + c] = lv[c];
+    }
+    out[6 * (size_t)nbodies + b] = lm;
+  }
+}
+
+int batch_traj_forward(aster_ctx* c, double dt, int64_t nsteps, int32_t nsnap, const int64_t* snaps,
+                       double* snap_pos) {
+  BatchState& B = c->batch;
+  B.traj_ready = false;
+  if (B.nsys == 0) return fail(c, ASTER_ERR_STATE, "no batch loaded");
+  if (!std::isfinite(dt) || nsteps < 1 || nsnap < 0 || (nsnap > 0 && !snaps))
+    return fail(c, ASTER_ERR_INVALID, "need dt finite, nsteps >= 1 and snapshot steps");
+  for (int q = 0; q < nsnap; ++q)
+    if (snaps[q] < 1 || snaps[q] > nsteps || (q > 0 && snaps[q] <= snaps[q - 1]))
+      return fail(c, ASTER_ERR_INVALID, "snapshot steps must be ascending within [1, nsteps]");
+  if (c->P.pn_flags & ASTER_PN1) return fail(c, ASTER_ERR_INVALID, "the adjoint covers Newtonian forces");
+  const size_t nb3 = (size_t)3 * B.nbodies;
+  if ((double)(nsteps + 1) * nb3 * 8.0 > 64e9) return fail(c, ASTER_ERR_CAPACITY, "trajectory would exceed 64 GB");
+  ASTER_HIP(c, B.traj.reserve((size_t)(nsteps + 1) * nb3));
+  ASTER_HIP(c, B.x0.reserve(nb3));
+  ASTER_HIP(c, B.v0.reserve(nb3));
+  ASTER_HIP(c, hipMemcpyAsync(B.x0.p, B.x.p, nb3 * 8, hipMemcpyDeviceToDevice, c->stream));
+  ASTER_HIP(c, hipMemcpyAsync(B.v0.p, B.v.p, nb3 * 8, hipMemcpyDeviceToDevice, c->stream));
+  const double h[3] = {dt, 0.0, 0.0};
+  const BatchArgs P = make_args(c, nsteps, 1, h, 0);
+  const int wpb = B.waves_per_block;
+  hipLaunchKernelGGL(k_batch_traj_fwd, dim3((P.nwaves + wpb - 1) / wpb), dim3(64 * wpb), 0, c->stream, P, B.traj.p,
+                     B.nbodies);
+  ASTER_HIP(c, hipGetLastError());
+  B.a_valid = true;
+  if (snap_pos)
+    for (int q = 0; q < nsnap; ++q)
+      ASTER_HIP(c, hipMemcpyAsync(snap_pos + (size_t)q * nb3, B.traj.p + (size_t)snaps[q] * nb3, nb3 * 8,
+                                  hipMemcpyDeviceToHost, c->stream));
+  ASTER_HIP(c, hipStreamSynchronize(c->stream));
+  B.traj_snaps.assign(snaps, snaps + nsnap);
+  B.traj_steps = nsteps;
+  B.traj_dt = dt;
+  B.traj_ready = true;
+  return ASTER_OK;
+}
+
+int batch_traj_backward(aster_ctx* c, const double* gsnap, double* gx, double* gv, double* gm) {
+  BatchState& B = c->batch;
+  if (!B.traj_ready) return fail(c, ASTER_ERR_STATE, "call aster_batch_traj_forward first");
+  const int nsnap = (int)B.traj_snaps.size();
+  if (nsnap > 0 && !gsnap) return fail(c, ASTER_ERR_INVALID, "snapshot gradients required");
+  const size_t nb3 = (size_t)3 * B.nbodies;
+  ASTER_HIP(c, B.tgrad.reserve(std::max<size_t>(1, nsnap * nb3)));
+  ASTER_HIP(c, B.tsnap.reserve(std::max(1, nsnap)));
+  ASTER_HIP(c, B.tout.reserve(7 * (size_t)B.nbodies));
+  if (nsnap > 0) {
+    ASTER_HIP(c, hipMemcpyAsync(B.tgrad.p, gsnap, nsnap * nb3 * 8, hipMemcpyHostToDevice, c->stream));
+    ASTER_HIP(c, hipMemcpyAsync(B.tsnap.p, B.traj_snaps.data(), nsnap * 8, hipMemcpyHostToDevice, c->stream));
+  }
+  const double h0[3] = {0.0, 0.0, 0.0};
+  const BatchArgs P = make_args(c, 0, 1, h0, 0);
+  const int wpb = B.waves_per_block;
+  hipLaunchKernelGGL(k_batch_traj_bwd, dim3((P.nwaves + wpb - 1) / wpb), dim3(64 * wpb), 0, c->stream, P, B.traj.p,
+                     B.nbodies, B.traj_steps, B.traj_dt, B.tgrad.p, B.tsnap.p, nsnap, B.tout.p);
+  ASTER_HIP(c, hipGetLastError());
+  if (gx) ASTER_HIP(c, hipMemcpyAsync(gx, B.tout.p, nb3 * 8, hipMemcpyDeviceToHost, c->stream));
+  if (gv) ASTER_HIP(c, hipMemcpyAsync(gv, B.tout.p + nb3, nb3 * 8, hipMemcpyDeviceToHost, c->stream));
+  if (gm) ASTER_HIP(c, hipMemcpyAsync(gm, B.tout.p + 2 * nb3, B.nbodies * 8, hipMemcpyDeviceToHost, c->stream));
+  // restore the initial state
+  ASTER_HIP(c, hipMemcpyAsync(B.x.p, B.x0.p, nb3 * 8, hipMemcpyDeviceToDevice, c->stream));
+  ASTER_HIP(c, hipMemcpyAsync(B.v.p, B.v0.p, nb3 * 8, hipMemcpyDeviceToDevice, c->stream));
+  ASTER_HIP(c, hipStreamSynchronize(c->stream));
+  B.a_valid = false;
+  B.traj_ready = false;
+  return ASTER_OK;
+}
+
 }  // namespace aster
 
 using namespace aster;
@@ -528,6 +741,20 @@ int aster_batch_set(aster_ctx* c, int64_t nsys, const int64_t* offsets, const do
   ASTER_HIP(c, hipMemcpyAsync(B.doff.p, offsets, (nsys + 1) * 8, hipMemcpyHostToDevice, c->stream));
   B.a_valid = false;
   return upload_plan(c);
+}
+
+int aster_batch_traj_forward(aster_ctx* c, double dt, int64_t nsteps, int32_t nsnap, const int64_t* snap_steps,
+                             double* snap_pos) {
+  if (!c) return ASTER_ERR_INVALID;
+  ASTER_HIP(c, hipSetDevice(c->device));
+  return batch_traj_forward(c, dt, nsteps, nsnap, snap_steps, snap_pos);
+}
+
+int aster_batch_traj_backward(aster_ctx* c, const double* grad_snap_pos, double* g_pos, double* g_vel,
+                              double* g_mass) {
+  if (!c) return ASTER_ERR_INVALID;
+  ASTER_HIP(c, hipSetDevice(c->device));
+  return batch_traj_backward(c, grad_snap_pos, g_pos, g_vel, g_mass);
 }
 
 int aster_batch_get(aster_ctx* c, double* pos, double* vel) {

@@ -63,6 +63,16 @@ struct BatchState {
   int64_t g_steps = -1;
   int g_integrator = -1, g_spl = 0, g_pn = -1, g_exact = -1, g_wpb = 0, g_pack = -1;
   double g_dt = 0.0, g_G = 0.0, g_c = 0.0, g_soft = -1.0;
+  // Trajectory adjoint of the ensemble: positions at every step are kept.
+  DBuf<double> traj;                 // (nsteps + 1) * 3 * nbodies, AoS-d3 per step
+  DBuf<double> x0, v0;               // initial state, restored after the backward pass
+  DBuf<double> tgrad;                // snapshot gradients, nsnap * 3 * nbodies
+  DBuf<double> tout;                 // gx, gv (AoS-d3) and gm per body
+  DBuf<int64_t> tsnap;               // snapshot steps on the device
+  std::vector<int64_t> traj_snaps;
+  int64_t traj_steps = 0;
+  double traj_dt = 0.0;
+  bool traj_ready = false;
 };
 
 struct AdjointState {
@@ -170,6 +180,9 @@ int batch_autotune(aster_ctx* c, int integrator, double dt, int32_t* spb, int32_
 int traj_forward(aster_ctx* c, double dt, int64_t nsteps, int64_t ckpt, int32_t nsnap, const int64_t* snaps,
                  double* snap_pos);
 int traj_backward(aster_ctx* c, const double* gsnap, double* gx, double* gv, double* gm);
+int batch_traj_forward(aster_ctx* c, double dt, int64_t nsteps, int32_t nsnap, const int64_t* snaps,
+                       double* snap_pos);
+int batch_traj_backward(aster_ctx* c, const double* gsnap, double* gx, double* gv, double* gm);
 int choose_split(aster_ctx* c);
 int compute_forces(aster_ctx* c, int method);             // all-pairs or Barnes-Hut into c->a
 void phase_begin(aster_ctx* c, int phase);

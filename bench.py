@@ -61,18 +61,20 @@ def cpu_model():
     return "unknown"
 
 
-def bench_bh(A, ctx, steps, warmup, method, n=1 << 20, theta=0.5, dt=5e-4):
+def bench_bh(A, ctx, steps, warmup, method, n=1 << 20, theta=0.5, dt=5e-4, reference=True):
     pos, vel, mass = A.ic_mixed(n, 20240)
     ctx.set_params(G=1.0, softening=0.005 ** 2, theta=theta, precision=A.FP32)
     ctx.set_state(pos, vel, mass)
-    # accuracy of one evaluation against FP64 all-pairs (device path that is
-    # bit-identical to the reference CPU path)
-    log(f"[bench] BH n={n}: FP64 all-pairs reference")
-    ref = ctx.forces(A.ALLPAIRS_FP64)
-    acc = ctx.forces(method)
-    ctx.bh_stats()
-    errs = err_stats(rel_err(acc, ref))
-    log(f"[bench] BH force error vs FP64 all-pairs: {errs}")
+    errs = None
+    if reference:
+        # accuracy of one evaluation against FP64 all-pairs (device path that
+        # is bit-identical to the reference CPU path)
+        log(f"[bench] BH n={n}: FP64 all-pairs reference")
+        ref = ctx.forces(A.ALLPAIRS_FP64)
+        acc = ctx.forces(method)
+        ctx.bh_stats()
+        errs = err_stats(rel_err(acc, ref))
+        log(f"[bench] BH force error vs FP64 all-pairs: {errs}")
     # warmup, then K timed leapfrog steps
     ctx.step(method, A.LEAPFROG, dt, warmup)
     ctx.sync()
@@ -232,6 +234,8 @@ def main():
     ap.add_argument("--method", default="BH_GROUP")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward ratio")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched ensemble")
+    ap.add_argument("--only", choices=["bh", "allpairs", "adjoint", "batch"],
+                    help="run one section only (profiling); no contract line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, "
@@ -240,6 +244,18 @@ def main():
     import aster_amd as A
     ctx = A.Context(0)
     method = getattr(A, args.method)
+    if args.only:
+        # One section, for profiling runs; prints that section's result only.
+        if args.only == "bh":
+            r = bench_bh(A, ctx, args.steps, args.warmup, method, reference=False)
+        elif args.only == "allpairs":
+            r = bench_allpairs(A, ctx, max(args.steps, 10))[0]
+        elif args.only == "adjoint":
+            r = bench_adjoint(A, ctx)
+        else:
+            r = bench_batch(A, ctx)
+        print(json.dumps({"section": args.only, "result": r}, default=float), flush=True)
+        return
     bh = bench_bh(A, ctx, args.steps, args.warmup, method)
     ap_res, pos64k, vel64k, mass64k = bench_allpairs(A, ctx, max(args.steps, 10))
     adj = None if args.no_adjoint else bench_adjoint(A, ctx)

@@ -208,6 +208,18 @@ int aster_batch_energy(aster_ctx* ctx, double* e, double* L);
 int aster_batch_autotune(aster_ctx* ctx, int integrator, double dt, int32_t* systems_per_block,
                          int32_t* steps_per_launch);
 
+/* Trajectory adjoint of the loaded ensemble (no reference counterpart):
+ * leapfrog with Newtonian FP64 forces (fast mode) for nsteps, keeping the
+ * positions of every step on the device; snapshot positions (nsnap * 3 *
+ * nbodies, AoS-d3) go to snap_pos if not NULL. The backward pass takes
+ * dLoss/d(snapshot positions) and writes dLoss/d(initial positions,
+ * velocities) (3 * nbodies each) and d(masses) (nbodies) for every system at
+ * once, then restores the initial state. */
+int aster_batch_traj_forward(aster_ctx* ctx, double dt, int64_t nsteps, int32_t nsnap, const int64_t* snap_steps,
+                             double* snap_pos);
+int aster_batch_traj_backward(aster_ctx* ctx, const double* grad_snap_pos, double* g_pos, double* g_vel,
+                              double* g_mass);
+
 /* ---- trajectory gradients (adjoint of leapfrog) ------------------------ */
 /* Forward: leapfrog with ASTER_ALLPAIRS_FP32_KAHAN forces (FP64 state) for
  * nsteps from the current state, keeping a checkpoint every ckpt_every steps.

@@ -132,3 +132,53 @@ def test_adjoint_errors(ctx):
     ctx.set_params(G=1.0, softening=1e-4, pn_flags=A.NEWTON | A.PN1)
     with pytest.raises(A.AsterError):
         ctx.traj_forward(1e-3, 10, 4, [5])
+
+
+def test_batched_threebody_fits(ctx, orc):
+    # 4096 three-body systems (masses U[0.5,1.5], unit box, zero momentum); the
+    # targets come from initial conditions perturbed by N(0, 1e-3^2). One
+    # backward pass gives every system's gradient; checked against the FP64
+    # oracle adjoint (relative 2-norm <= 1e-8) and against central differences
+    # of the device forward map (relative 1e-6).
+    nsys, steps, dt, soft = 4096, 200, 1e-3, 1e-2
+    pos, vel, mass = A.ic_threebody(nsys, 1)
+    off = np.arange(nsys + 1) * 3
+    snaps = np.array([50, 100, 200])
+    rng = np.random.default_rng(7)
+    ctx.set_params(G=1.0, softening=soft, precision=A.FP32)
+    ctx.batch_set(off, pos + rng.normal(scale=1e-3, size=pos.shape), vel + rng.normal(scale=1e-3, size=vel.shape),
+                  mass)
+    target = ctx.batch_traj_forward(dt, steps, snaps)
+
+    def losses(p, v, m):
+        ctx.batch_set(off, p, v, m)
+        s = ctx.batch_traj_forward(dt, steps, snaps)
+        d = (s - target) ** 2
+        return d.reshape(len(snaps), nsys, 3, 3).sum(axis=(0, 2, 3)) / (len(snaps) * 3)
+
+    ctx.batch_set(off, pos, vel, mass)
+    s = ctx.batch_traj_forward(dt, steps, snaps)
+    gsnap = 2.0 * (s - target) / (len(snaps) * 3)
+    gx, gv, gm = ctx.batch_traj_backward(gsnap)
+    for sidx in range(0, nsys, 512):
+        b = slice(3 * sidx, 3 * sidx + 3)
+        _, rgx, rgv, rgm = orc.traj_grad(pos[b], vel[b], mass[b], dt, steps, snaps, target[:, b], soft=soft)
+        assert relnorm(gx[b], rgx) < 1e-8 and relnorm(gv[b], rgv) < 1e-8 and relnorm(gm[b], rgm) < 1e-8, sidx
+    eps = 1e-6
+    for sidx, comp in ((3, (0, 1)), (1000, (2, 2)), (4095, (1, 0))):
+        i = 3 * sidx + comp[0]
+        for which, g in (("x", gx), ("v", gv)):
+            p, v = pos.copy(), vel.copy()
+            arr = p if which == "x" else v
+            arr[i, comp[1]] += eps
+            lp = losses(p, v, mass)[sidx]
+            arr[i, comp[1]] -= 2 * eps
+            lm = losses(p, v, mass)[sidx]
+            fd = (lp - lm) / (2 * eps)
+            assert g[i, comp[1]] == pytest.approx(fd, rel=1e-6, abs=1e-12), (sidx, which)
+        m = mass.copy()
+        m[i] += eps
+        lp = losses(pos, vel, m)[sidx]
+        m[i] -= 2 * eps
+        lm = losses(pos, vel, m)[sidx]
+        assert gm[i] == pytest.approx((lp - lm) / (2 * eps), rel=1e-6, abs=1e-12), (sidx, "m")
