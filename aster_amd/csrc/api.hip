@@ -141,11 +141,12 @@ int aster_destroy(aster_ctx* c) {
     if (e) (void)hipEventDestroy(e);
   c->x.release(); c->v.release(); c->m.release(); c->a.release(); c->sh.release(); c->src64.release();
   c->apart.release(); c->apn.release(); c->ework.release(); c->eout.release(); c->flags.release();
-  c->boxbits.release(); c->boxd.release();
+  c->boxpart.release(); c->boxd.release();
   c->codes.release(); c->idx.release(); c->codes2.release(); c->idx2.release(); c->hist.release();
-  c->left.release(); c->right.release(); c->parent.release(); c->escape.release(); c->counter.release();
+  c->left.release(); c->right.release(); c->parent.release(); c->escape.release(); c->scanpart.release();
   c->depth.release(); c->nmass.release(); c->ncom.release(); c->nvel.release(); c->nbox.release();
-  c->levels.release(); c->bucket.release();
+  c->levels.release(); c->bucket.release(); c->dsorted.release(); c->grange.release(); c->gscan.release();
+  c->gstart.release(); c->gcount.release();
   c->nrec.release(); c->nlink.release(); c->bh_counters.release(); c->gsh.release();
   auto& Bt = c->batch;
   Bt.x.release(); Bt.v.release(); Bt.a.release(); Bt.m.release(); Bt.ebody.release(); Bt.esys.release();

@@ -112,17 +112,22 @@ struct aster_ctx {
   aster::DBuf<double> ework;    // energy work [6][n] + reduction scratch
   aster::DBuf<double> eout;     // 8 doubles
   aster::DBuf<uint32_t> flags;  // device error word(s), 4 words
-  aster::DBuf<unsigned long long> boxbits;  // auto box as ordered bit patterns (4)
+  aster::DBuf<double> boxpart;  // per-block |x| maxima for the auto box [blocks][3]
   aster::DBuf<double> boxd;     // box half-extent used (3) + |box| (1)
 
   // Barnes-Hut / LBVH.
   aster::DBuf<uint32_t> codes, idx, codes2, idx2;  // sort ping-pong, n
   aster::DBuf<uint32_t> hist;                      // per-block digit counts
   aster::DBuf<int32_t> left, right, parent, escape;// 2n-1
-  aster::DBuf<int32_t> counter;                    // n-1 arrival counters
+  aster::DBuf<uint32_t> scanpart;                  // per-chunk totals of the device scan
   aster::DBuf<int32_t> depth;                      // 2n-1
-  aster::DBuf<int32_t> levels;                     // per-depth count, offset, fill (3 x 128)
+  aster::DBuf<uint32_t> dsorted;                   // internal-node depths in bucket order (n-1)
+  aster::DBuf<int32_t> levels;                     // per-depth bucket offsets + level count
   aster::DBuf<int32_t> bucket;                     // internal nodes grouped by depth (n-1)
+  aster::DBuf<int2> grange;                        // sorted-position range of internal node n+i
+  aster::DBuf<uint32_t> gscan;                     // group-start flags, then their exclusive scan (n)
+  aster::DBuf<int32_t> gstart;                     // first sorted position of each group (+ n)
+  aster::DBuf<uint32_t> gcount;                    // number of groups (1)
   std::vector<int32_t> level_host;                 // per-depth counts on the host
   aster::DBuf<double> nmass, ncom, nvel;           // FP64 moments: 2n-1, [3][2n-1], [3][2n-1]
   aster::DBuf<float> nbox;                         // FP32 AABB [6][2n-1]

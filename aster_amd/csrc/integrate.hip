@@ -106,29 +106,45 @@ __global__ __launch_bounds__(1024) void k_reduce_rows(int n, int rows, const dou
   }
 }
 
-// Symmetric box: max |coordinate| per axis. Non-negative doubles order like
-// their bit patterns, so an integer atomicMax on the bits is exact and
-// order-independent.
-__global__ __launch_bounds__(256) void k_box(int n, const double* __restrict__ x,
-                                             unsigned long long* __restrict__ bits) {
+// Symmetric box: max |coordinate| per axis, in two stages (per-block maxima,
+// then one block over those), so no two blocks contend for an address. fmax
+// is exact and order-independent.
+constexpr int kBoxBlocks = 1024;
+
+__global__ __launch_bounds__(256) void k_box(int n, const double* __restrict__ x, double* __restrict__ part) {
+  __shared__ double wm[3][4];
   double mx[3] = {0.0, 0.0, 0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
 #pragma unroll
     for (int c = 0; c < 3; ++c) mx[c] = fmax(mx[c], fabs(x[(size_t)c * n + i]));
+  const int w = threadIdx.x >> 6;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     double r = mx[c];
     for (int off = 32; off > 0; off >>= 1) r = fmax(r, __shfl_xor(r, off, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(&bits[c], (unsigned long long)__double_as_longlong(r));
+    if ((threadIdx.x & 63) == 0) wm[c][w] = r;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int c = threadIdx.x;
+    part[3 * blockIdx.x + c] = fmax(fmax(wm[c][0], wm[c][1]), fmax(wm[c][2], wm[c][3]));
   }
 }
 
-__global__ void k_box_finish(const unsigned long long* __restrict__ bits, const double* __restrict__ user,
-                             int use_user, double* __restrict__ box) {
+__global__ __launch_bounds__(64) void k_box_finish(const double* __restrict__ part, int nparts,
+                                                   const double* __restrict__ user, int use_user,
+                                                   double* __restrict__ box) {
 #pragma clang fp contract(off)
-  if (threadIdx.x != 0) return;
   double b[3];
-  for (int c = 0; c < 3; ++c) b[c] = use_user ? user[c] : __longlong_as_double((long long)bits[c]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double r = 0.0;
+    if (!use_user)
+      for (int i = threadIdx.x; i < nparts; i += 64) r = fmax(r, part[3 * i + c]);
+    for (int off = 32; off > 0; off >>= 1) r = fmax(r, __shfl_xor(r, off, 64));
+    b[c] = use_user ? user[c] : r;
+  }
+  if (threadIdx.x != 0) return;
   box[0] = b[0];
   box[1] = b[1];
   box[2] = b[2];
@@ -138,9 +154,6 @@ __global__ void k_box_finish(const unsigned long long* __restrict__ bits, const 
 
 __global__ void k_zero_u32(uint32_t* p, int k) {
   if ((int)threadIdx.x < k) p[threadIdx.x] = 0u;
-}
-__global__ void k_zero_u64(unsigned long long* p, int k) {
-  if ((int)threadIdx.x < k) p[threadIdx.x] = 0ull;
 }
 
 // --------------------------------------------------------------- launch ----
@@ -192,19 +205,19 @@ int launch_energy(aster_ctx* c, double* out6) {
 
 int launch_box(aster_ctx* c, bool user_box) {
   const int n = (int)c->n;
-  ASTER_HIP(c, c->boxbits.reserve(4));
+  ASTER_HIP(c, c->boxpart.reserve(3 * kBoxBlocks));
   ASTER_HIP(c, c->boxd.reserve(8));
   // A user box was uploaded to boxd[4..6] by aster_set_params.
   const bool user = user_box && (c->P.box[0] != 0.0 || c->P.box[1] != 0.0 || c->P.box[2] != 0.0);
+  int blocks = 0;
   if (!user) {
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, c->boxbits.p, 4);
-    int blocks = (n + 255) / 256;
-    if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(k_box, dim3(blocks), dim3(256), 0, c->stream, n, c->x.p, c->boxbits.p);
+    blocks = (n + 255) / 256;
+    if (blocks > kBoxBlocks) blocks = kBoxBlocks;
+    hipLaunchKernelGGL(k_box, dim3(blocks), dim3(256), 0, c->stream, n, c->x.p, c->boxpart.p);
     ASTER_HIP(c, hipGetLastError());
   }
-  hipLaunchKernelGGL(k_box_finish, dim3(1), dim3(64), 0, c->stream, c->boxbits.p, c->boxd.p + 4, user ? 1 : 0,
-                     c->boxd.p);
+  hipLaunchKernelGGL(k_box_finish, dim3(1), dim3(64), 0, c->stream, c->boxpart.p, blocks, c->boxd.p + 4,
+                     user ? 1 : 0, c->boxd.p);
   ASTER_HIP(c, hipGetLastError());
   return ASTER_OK;
 }

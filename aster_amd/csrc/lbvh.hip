@@ -1,5 +1,6 @@
 // LBVH Barnes-Hut on CDNA4: Morton codes -> LDS radix sort -> Karras radix
-// tree -> atomic bottom-up moments -> escape links -> stackless traversal.
+// tree -> level-synchronous bottom-up moments -> escape links -> stackless or
+// wavefront-cooperative traversal.
 //
 // Tree layout follows the reference's NodesArray indexing
 // (src/solvers/barnes_hut.cpp:309-400): leaf k is the body at sorted position
@@ -13,8 +14,8 @@
 //    (barnes_hut.cpp:340-342) and its search sequence: identical topology.
 //  * Moments: M = m_R + m_L, COM = (c_R m_R + c_L m_L) / M in FP64 without
 //    contraction (barnes_hut.cpp:145-150). Each node depends only on its two
-//    children, so the bottom-up atomic order cannot change the result: node
-//    masses and centres of mass are bit-identical to the recursive unite().
+//    children, so the bottom-up order cannot change the result: node masses
+//    and centres of mass are bit-identical to the recursive unite().
 //  * Traversal: the reference visits right children first (it pushes left,
 //    then right, barnes_hut.cpp:241-256). Opening goes to the right child and
 //    every accepted/skipped node continues at its escape link, which is the
@@ -84,28 +85,70 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint32_t* __restri
   hist[(size_t)t * nblocks + blockIdx.x] = cnt[t];
 }
 
-// Exclusive prefix sum over `total` counts, one block.
-__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ hist, int total) {
-  __shared__ uint32_t part[1024];
-  const int t = threadIdx.x;
-  const int per = (total + 1023) / 1024;
-  const int b = t * per, e = min(b + per, total);
-  uint32_t s = 0;
-  for (int k = b; k < e; ++k) s += hist[k];
-  part[t] = s;
+// Exclusive prefix sum of a u32 array, in place, in two passes over chunks of
+// kScanChunk entries: chunk totals, then each chunk scanned and offset by the
+// sum of the totals before it (every block re-reduces that short prefix, so
+// no block waits on another).
+constexpr int kScanThreads = 256;
+constexpr int kScanPer = 16;
+constexpr int kScanChunk = kScanThreads * kScanPer;
+
+__device__ __forceinline__ uint32_t block_sum_256(uint32_t v, uint32_t* ws) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    uint32_t v = (t >= off) ? part[t - off] : 0u;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ in, int count,
+                                                              uint32_t* __restrict__ part) {
+  __shared__ uint32_t ws[4];
+  const int base = blockIdx.x * kScanChunk;
+  uint32_t s = 0;
+#pragma unroll
+  for (int r = 0; r < kScanPer; ++r) {
+    const int p = base + r * kScanThreads + (int)threadIdx.x;
+    if (p < count) s += in[p];
   }
-  uint32_t run = part[t] - s;  // exclusive
-  for (int k = b; k < e; ++k) {
-    const uint32_t v = hist[k];
-    hist[k] = run;
-    run += v;
+  s = block_sum_256(s, ws);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// Thread t owns entries [base + 16 t, base + 16 t + 16) of its chunk.
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(uint32_t* __restrict__ data, int count,
+                                                             const uint32_t* __restrict__ part,
+                                                             uint32_t* __restrict__ total) {
+  __shared__ uint32_t ws[4];
+  __shared__ uint32_t wsum[4];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  uint32_t pre = 0;
+  for (int b = t; b < (int)blockIdx.x; b += kScanThreads) pre += part[b];
+  pre = block_sum_256(pre, ws);
+  const int base = blockIdx.x * kScanChunk + kScanPer * t;
+  uint32_t v[kScanPer];
+  uint32_t s = 0;
+#pragma unroll
+  for (int r = 0; r < kScanPer; ++r) {
+    v[r] = (base + r < count) ? data[base + r] : 0u;
+    s += v[r];
   }
+  // inclusive scan of the per-thread sums across the wave, then the waves
+  uint32_t inc = s;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += y;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint32_t run = pre + inc - s;
+  for (int q = 0; q < w; ++q) run += wsum[q];
+#pragma unroll
+  for (int r = 0; r < kScanPer; ++r) {
+    if (base + r < count) data[base + r] = run;
+    run += v[r];
+  }
+  if (total && base < count && base + kScanPer >= count) *total = run;
 }
 
 // Stable scatter: rank of a key among equal digits = earlier rounds (run[]),
@@ -169,12 +212,12 @@ __device__ __forceinline__ int kdelta(const uint32_t* __restrict__ codes, int n,
 }
 
 // Thread k initialises leaf k (body perm[k]) and, for k < n-1, the internal
-// node n+k. Also resets arrival counters and writes parent links.
+// node n+k with its sorted-position range. Also writes parent links.
 __global__ __launch_bounds__(256) void k_build(int n, const uint32_t* __restrict__ codes,
                                                const uint32_t* __restrict__ perm, const double* __restrict__ x,
                                                const double* __restrict__ v, const double* __restrict__ m,
                                                int32_t* __restrict__ left, int32_t* __restrict__ right,
-                                               int32_t* __restrict__ parent, int32_t* __restrict__ counter,
+                                               int32_t* __restrict__ parent, int2* __restrict__ grange,
                                                double* __restrict__ nmass, double* __restrict__ ncom,
                                                double* __restrict__ nvel, float* __restrict__ nbox,
                                                float4* __restrict__ nrec, float4* __restrict__ gsh,
@@ -236,17 +279,18 @@ __global__ __launch_bounds__(256) void k_build(int n, const uint32_t* __restrict
   right[n + i] = R;
   parent[L] = n + i;
   parent[R] = n + i;
-  counter[i] = 0;
+  grange[i] = make_int2(lo, hi);
 }
 
 // Depth of every node and its escape link in the right-first preorder (the
 // next node after its subtree: the left sibling if it is a right child, else
-// its parent's escape). Internal nodes are counted per depth for the
-// level-synchronous moment pass.
+// its parent's escape). Internal node n+i also writes (depth, n+i) as a
+// key/value pair for the counting sort that buckets nodes by depth.
 __global__ __launch_bounds__(256) void k_depth(int n, const int32_t* __restrict__ left,
                                                const int32_t* __restrict__ right, const int32_t* __restrict__ parent,
                                                int32_t* __restrict__ depth, int32_t* __restrict__ escape,
-                                               int32_t* __restrict__ level_count, uint32_t* __restrict__ flags) {
+                                               uint32_t* __restrict__ dkey, uint32_t* __restrict__ dval,
+                                               uint32_t* __restrict__ flags) {
   const int node = blockIdx.x * blockDim.x + threadIdx.x;
   const int nn = 2 * n - 1;
   if (node >= nn) return;
@@ -265,35 +309,26 @@ __global__ __launch_bounds__(256) void k_depth(int n, const int32_t* __restrict_
   }
   depth[node] = dep;
   escape[node] = esc;
-  if (node >= n) atomicAdd(&level_count[dep], 1);
-}
-
-// Exclusive scan of the per-depth counts (kMaxLevels entries, one thread).
-__global__ void k_level_scan(const int32_t* __restrict__ level_count, int32_t* __restrict__ level_off,
-                             int32_t* __restrict__ level_fill) {
-  if (threadIdx.x != 0) return;
-  int32_t run = 0;
-  for (int d = 0; d < kMaxLevels; ++d) {
-    level_off[d] = run;
-    level_fill[d] = 0;
-    run += level_count[d];
+  if (node >= n) {
+    dkey[node - n] = (uint32_t)dep;
+    dval[node - n] = (uint32_t)node;
   }
 }
 
-__global__ void k_zero_i32(int32_t* p, int k) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < k) p[i] = 0;
-}
-
-// Internal nodes into per-depth buckets (order within a depth is irrelevant).
-__global__ __launch_bounds__(256) void k_bucket(int n, const int32_t* __restrict__ depth,
-                                                const int32_t* __restrict__ level_off, int32_t* __restrict__ level_fill,
-                                                int32_t* __restrict__ bucket) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n - 1) return;
-  const int node = n + i;
-  const int d = depth[node];
-  bucket[level_off[d] + atomicAdd(&level_fill[d], 1)] = node;
+// Bucket bounds from the depth-sorted keys: off[d] = first bucket slot of
+// depth d for d <= max depth, off[max depth + 1] = count, and
+// off[kMaxLevels + 1] = number of depths. Internal-node depths have no gaps
+// (every internal node below the root has an internal parent one level up).
+__global__ __launch_bounds__(256) void k_level_bounds(int count, const uint32_t* __restrict__ key,
+                                                      int32_t* __restrict__ off) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= count) return;
+  const uint32_t d = key[p];
+  if (p == 0 || key[p - 1] != d) off[d] = p;
+  if (p == count - 1) {
+    off[d + 1] = count;
+    off[kMaxLevels + 1] = (int32_t)d + 1;
+  }
 }
 
 // Monopole moments of the internal nodes of one depth; their children are one
@@ -516,24 +551,67 @@ __global__ __launch_bounds__(256) void k_bh_f64(int n, const double* __restrict_
   }
 }
 
-// Wavefront-cooperative traversal. The 64 lanes hold 64 consecutive bodies in
-// Morton order (a spatially compact group); the wavefront walks the tree once
-// for the whole group with a shared LDS stack of opened nodes, testing up to
-// 64 candidate nodes per iteration (one per lane, so 64 independent node
-// loads are in flight). A node is accepted when the distance from its centre
-// of mass to the group's bounding box exceeds its opening radius (the
-// geometric test, so it holds for every body of the group); leaves are always
-// accepted. Accepted nodes are appended to an LDS interaction list that all
-// lanes then evaluate coherently (LDS broadcast reads, 12 VALU + 1 rsq per
-// interaction), FP32 sums per list flush, Kahan-compensated across flushes.
+// ------------------------------------------------------------- groups ------
+//
+// The wavefront traversal works on groups of bodies that share one walk. A
+// group is a subtree of the radix tree with at most kGroup leaves whose
+// parent has more: its bodies are contiguous in Morton order and lie in one
+// Morton cell (the subtree's common code prefix), so the group box is compact
+// wherever the curve jumps. Groups partition the sorted bodies.
+constexpr int kGroup = 64;
+
+// Leaf k climbs while the parent still has at most kGroup leaves; k starts a
+// group when it is the first position of the subtree it stops at.
+__global__ __launch_bounds__(256) void k_group_flags(int n, const int32_t* __restrict__ parent,
+                                                     const int2* __restrict__ grange, uint32_t* __restrict__ flag) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int v = k, lo = k;
+  for (;;) {
+    const int p = parent[v];
+    if (p < 0) break;
+    const int2 r = grange[p - n];
+    if (r.y - r.x + 1 > kGroup) break;
+    v = p;
+    lo = r.x;
+  }
+  flag[k] = (lo == k) ? 1u : 0u;
+}
+
+// From the exclusive scan of the flags: gstart[g] = first position of group g,
+// gstart[ngroups] = n.
+__global__ __launch_bounds__(256) void k_group_emit(int n, const uint32_t* __restrict__ ex,
+                                                    const uint32_t* __restrict__ total, int32_t* __restrict__ gstart) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t e = ex[k];
+  const uint32_t next = (k + 1 < n) ? ex[k + 1] : *total;
+  if (next != e) gstart[e] = k;
+  if (k == n - 1) gstart[*total] = n;
+}
+
+// Wavefront-cooperative traversal, one group per wavefront (lanes = the
+// group's bodies, at most 64). The wavefront walks the tree once for the
+// whole group with a shared LDS stack of opened nodes, testing up to 64
+// candidate nodes per iteration (one per lane, so 64 independent node loads
+// are in flight). A node is accepted when the distance from its centre of
+// mass to the group's bounding box exceeds its opening radius (the geometric
+// test, so it holds for every body of the group); leaves are always accepted.
+// Accepted nodes are appended to an LDS interaction list that all lanes then
+// evaluate coherently (LDS broadcast reads, 12 VALU + 1 rsq per interaction),
+// FP32 sums per list flush, Kahan-compensated across flushes.
 // Stack bound: each iteration pops at most 32 opened nodes (64 children) and
 // pushes at most 64, and internal-node depth is at most 63, so fewer than
 // 32 * 63 + 64 entries are ever live.
+// Scheduling: the 8 XCDs take workgroups round-robin, so workgroup b works on
+// the groups of chunk (b mod 8), consecutive in Morton order, and neighbouring
+// groups (which open the same nodes) share an L2.
 constexpr int kGrpStack = 2304;
 constexpr int kGrpList = 256;
 
 template <bool MASK>
-__global__ __launch_bounds__(64) void k_bh_group(int n, const float4* __restrict__ gsh,
+__global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restrict__ gstart,
+                                                 const uint32_t* __restrict__ gcount, const float4* __restrict__ gsh,
                                                  const float4* __restrict__ nrec, const int4* __restrict__ nlink,
                                                  const int32_t* __restrict__ left, float eps2, double G,
                                                  double* __restrict__ a, unsigned long long* __restrict__ counters,
@@ -541,104 +619,120 @@ __global__ __launch_bounds__(64) void k_bh_group(int n, const float4* __restrict
   __shared__ int stk[kGrpStack];
   __shared__ float4 lst[kGrpList];
   const int lane = threadIdx.x;
-  const int k = blockIdx.x * 64 + lane;
-  const bool live = k < n;
-  const float4 me = gsh[live ? k : n - 1];
-  float lo[3] = {me.x, me.y, me.z}, hi[3] = {me.x, me.y, me.z};
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-    for (int off = 32; off > 0; off >>= 1) {
-      lo[c] = fminf(lo[c], __shfl_xor(lo[c], off, 64));
-      hi[c] = fmaxf(hi[c], __shfl_xor(hi[c], off, 64));
-    }
+  const int ngroups = (int)*gcount;
+  const int per_xcd = (ngroups + 7) >> 3;
+  const int xcd = blockIdx.x & 7;
+  const int stride = (int)(gridDim.x >> 3);
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  float sx = 0.f, sy = 0.f, sz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
-  unsigned long long listed = 0, visits = 1;
-  int sp = 0, cnt = 0;
-  auto dmin2 = [&](const float4& r) {
-    const float ex = fmaxf(fmaxf(lo[0] - r.x, r.x - hi[0]), 0.f);
-    const float ey = fmaxf(fmaxf(lo[1] - r.y, r.y - hi[1]), 0.f);
-    const float ez = fmaxf(fmaxf(lo[2] - r.z, r.z - hi[2]), 0.f);
-    return fmaf(ex, ex, fmaf(ey, ey, ez * ez));
-  };
-  auto flush = [&]() {
-    float tx = 0.f, ty = 0.f, tz = 0.f;
-    for (int e = 0; e < cnt; ++e) {
-      const float4 q = lst[e];
-      const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
-      float r2 = fmaf(dx, dx, eps2);
-      r2 = fmaf(dy, dy, r2);
-      r2 = fmaf(dz, dz, r2);
-      float inv = __builtin_amdgcn_rsqf(r2);
-      if (MASK) inv = (r2 > 0.f) ? inv : 0.f;
-      const float f = (q.w * inv) * (inv * inv);
-      tx = fmaf(f, dx, tx);
-      ty = fmaf(f, dy, ty);
-      tz = fmaf(f, dz, tz);
+  unsigned long long tot_inter = 0, tot_visits = 0;
+  for (int i = (int)(blockIdx.x >> 3); i < per_xcd; i += stride) {
+    const int g = xcd * per_xcd + i;
+    if (g >= ngroups) break;
+    const int lo = gstart[g], hi = gstart[g + 1] - 1;
+    const int k = lo + lane;
+    const bool live = k <= hi;
+    const float4 me = gsh[live ? k : hi];
+    float blo[3] = {me.x, me.y, me.z}, bhi[3] = {me.x, me.y, me.z};
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      for (int off = 32; off > 0; off >>= 1) {
+        blo[c] = fminf(blo[c], __shfl_xor(blo[c], off, 64));
+        bhi[c] = fmaxf(bhi[c], __shfl_xor(bhi[c], off, 64));
+      }
+    float sx = 0.f, sy = 0.f, sz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+    unsigned long long listed = 0, visits = 1;
+    int sp = 0, cnt = 0;
+    auto dmin2 = [&](const float4& r) {
+      const float ex = fmaxf(fmaxf(blo[0] - r.x, r.x - bhi[0]), 0.f);
+      const float ey = fmaxf(fmaxf(blo[1] - r.y, r.y - bhi[1]), 0.f);
+      const float ez = fmaxf(fmaxf(blo[2] - r.z, r.z - bhi[2]), 0.f);
+      return fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+    };
+    auto flush = [&]() {
+      float tx = 0.f, ty = 0.f, tz = 0.f;
+      for (int e = 0; e < cnt; ++e) {
+        const float4 q = lst[e];
+        const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
+        float r2 = fmaf(dx, dx, eps2);
+        r2 = fmaf(dy, dy, r2);
+        r2 = fmaf(dz, dz, r2);
+        float inv = __builtin_amdgcn_rsqf(r2);
+        if (MASK) inv = (r2 > 0.f) ? inv : 0.f;
+        const float f = (q.w * inv) * (inv * inv);
+        tx = fmaf(f, dx, tx);
+        ty = fmaf(f, dy, ty);
+        tz = fmaf(f, dz, tz);
+      }
+      kahan_add32(sx, cx, tx);
+      kahan_add32(sy, cy, ty);
+      kahan_add32(sz, cz, tz);
+      listed += (unsigned long long)cnt;
+      cnt = 0;
+    };
+    {
+      const int root = n;
+      const float4 r = nrec[root];
+      if (dmin2(r) > __int_as_float(nlink[root].z)) {
+        if (lane == 0) lst[0] = r;
+        cnt = 1;
+      } else {
+        if (lane == 0) stk[0] = root;
+        sp = 1;
+      }
     }
-    kahan_add32(sx, cx, tx);
-    kahan_add32(sy, cy, ty);
-    kahan_add32(sz, cz, tz);
-    listed += (unsigned long long)cnt;
-    cnt = 0;
-  };
-  {
-    const int root = n;
-    const float4 r = nrec[root];
-    if (dmin2(r) > __int_as_float(nlink[root].z)) {
-      if (lane == 0) lst[0] = r;
-      cnt = 1;
-    } else {
-      if (lane == 0) stk[0] = root;
-      sp = 1;
-    }
-  }
-  wave_sync();
-  while (sp > 0) {
-    const int take = min(sp, 32);
-    int node = -1;
-    if (lane < 2 * take) {
-      const int opened = stk[sp - take + (lane >> 1)];
-      node = (lane & 1) ? nlink[opened].x : left[opened];
-    }
-    sp -= take;
-    visits += 2 * take;
     wave_sync();
-    bool acc = false, opn = false;
-    float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (node >= 0) {
-      r = nrec[node];
-      acc = node < n || dmin2(r) > __int_as_float(nlink[node].z);
-      opn = !acc;
-    }
-    const uint64_t am = __ballot(acc), om = __ballot(opn);
-    const int nopen = (int)__popcll(om);
-    if (sp + nopen > kGrpStack) {  // cannot happen for a valid tree (see bound above)
-      if (lane == 0) atomicOr(flags, (uint32_t)FLAG_TREE);
-      break;
-    }
-    if (acc) lst[cnt + __popcll(am & lt)] = r;
-    if (opn) stk[sp + __popcll(om & lt)] = node;
-    cnt += (int)__popcll(am);
-    sp += nopen;
-    wave_sync();
-    if (cnt > kGrpList - 64) {
-      flush();
+    bool overflow = false;
+    while (sp > 0) {
+      const int take = min(sp, 32);
+      int node = -1;
+      if (lane < 2 * take) {
+        const int opened = stk[sp - take + (lane >> 1)];
+        node = (lane & 1) ? nlink[opened].x : left[opened];
+      }
+      sp -= take;
+      visits += 2 * take;
       wave_sync();
+      bool acc = false, opn = false;
+      float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (node >= 0) {
+        r = nrec[node];
+        acc = node < n || dmin2(r) > __int_as_float(nlink[node].z);
+        opn = !acc;
+      }
+      const uint64_t am = __ballot(acc), om = __ballot(opn);
+      const int nopen = (int)__popcll(om);
+      if (sp + nopen > kGrpStack) {  // cannot happen for a valid tree (see bound above)
+        if (lane == 0) atomicOr(flags, (uint32_t)FLAG_TREE);
+        overflow = true;
+        break;
+      }
+      if (acc) lst[cnt + __popcll(am & lt)] = r;
+      if (opn) stk[sp + __popcll(om & lt)] = node;
+      cnt += (int)__popcll(am);
+      sp += nopen;
+      wave_sync();
+      if (cnt > kGrpList - 64) {
+        flush();
+        wave_sync();
+      }
     }
-  }
-  if (cnt > 0) flush();
-  if (live) {
-    const int b = nlink[k].w;
-    a[b] = G * ((double)sx - (double)cx);
-    a[n + b] = G * ((double)sy - (double)cy);
-    a[2 * n + b] = G * ((double)sz - (double)cz);
-  }
-  if (lane == 0) {
-    const unsigned long long nlive = (unsigned long long)min(64, n - blockIdx.x * 64);
+    if (cnt > 0) flush();
+    wave_sync();
+    if (live) {
+      const int b = nlink[k].w;
+      a[b] = G * ((double)sx - (double)cx);
+      a[n + b] = G * ((double)sy - (double)cy);
+      a[2 * n + b] = G * ((double)sz - (double)cz);
+    }
     // every body's own leaf is in its group's list (distance 0, no force)
-    atomicAdd(&counters[BHC_INTERACTIONS], listed * nlive - nlive);
-    atomicAdd(&counters[BHC_VISITS], visits);
+    const unsigned long long nlive = (unsigned long long)(hi - lo + 1);
+    tot_inter += listed * nlive - nlive;
+    tot_visits += visits;
+    if (overflow) break;
+  }
+  if (lane == 0 && (tot_inter | tot_visits)) {
+    atomicAdd(&counters[BHC_INTERACTIONS], tot_inter);
+    atomicAdd(&counters[BHC_VISITS], tot_visits);
   }
 }
 
@@ -656,6 +750,20 @@ static int reserve_sort(aster_ctx* c) {
   ASTER_HIP(c, c->codes2.reserve(n));
   ASTER_HIP(c, c->idx2.reserve(n));
   ASTER_HIP(c, c->hist.reserve(256 * nblocks));
+  ASTER_HIP(c, c->scanpart.reserve((n + kScanChunk - 1) / kScanChunk + 1));
+  return ASTER_OK;
+}
+
+// In-place exclusive scan of data[0, count); the total goes to *total when
+// given (a device pointer).
+static int device_scan(aster_ctx* c, uint32_t* data, int count, uint32_t* total) {
+  const int chunks = (count + kScanChunk - 1) / kScanChunk;
+  if (chunks == 0) return ASTER_OK;
+  ASTER_HIP(c, c->scanpart.reserve((size_t)chunks));
+  hipLaunchKernelGGL(k_scan_reduce, dim3(chunks), dim3(kScanThreads), 0, c->stream, data, count, c->scanpart.p);
+  hipLaunchKernelGGL(k_scan_apply, dim3(chunks), dim3(kScanThreads), 0, c->stream, data, count, c->scanpart.p,
+                     total);
+  ASTER_HIP(c, hipGetLastError());
   return ASTER_OK;
 }
 
@@ -668,9 +776,10 @@ static int reserve_tree(aster_ctx* c) {
   ASTER_HIP(c, c->parent.reserve(nn));
   ASTER_HIP(c, c->escape.reserve(nn));
   ASTER_HIP(c, c->depth.reserve(nn));
-  ASTER_HIP(c, c->levels.reserve(3 * kMaxLevels));
+  ASTER_HIP(c, c->levels.reserve(kMaxLevels + 2));
   ASTER_HIP(c, c->bucket.reserve(n));
-  ASTER_HIP(c, c->counter.reserve(n));
+  ASTER_HIP(c, c->dsorted.reserve(n));
+  ASTER_HIP(c, c->grange.reserve(n));
   ASTER_HIP(c, c->nmass.reserve(nn));
   ASTER_HIP(c, c->ncom.reserve(3 * nn));
   ASTER_HIP(c, c->nvel.reserve(3 * nn));
@@ -719,7 +828,7 @@ int morton_sort(aster_ctx* c, bool user_box) {
   phase_begin(c, ASTER_PHASE_SORT);
   for (int shift = 0; shift < 32; shift += 8) {
     hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(kRsThreads), 0, c->stream, kin, n, shift, c->hist.p, nblocks);
-    hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, c->stream, c->hist.p, 256 * nblocks);
+    if ((rc = device_scan(c, c->hist.p, 256 * nblocks, nullptr))) return rc;
     hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(kRsThreads), 0, c->stream, kin, vin, kout, vout, n, shift,
                        c->hist.p, nblocks);
     std::swap(kin, kout);
@@ -738,38 +847,37 @@ static int build_tree(aster_ctx* c, int mac) {
   if ((rc = morton_sort(c, true))) return rc;
   phase_begin(c, ASTER_PHASE_BUILD);
   hipLaunchKernelGGL(k_build, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->codes.p, c->idx.p, c->x.p,
-                     c->v.p, c->m.p, c->left.p, c->right.p, c->parent.p, c->counter.p, c->nmass.p, c->ncom.p,
+                     c->v.p, c->m.p, c->left.p, c->right.p, c->parent.p, c->grange.p, c->nmass.p, c->ncom.p,
                      c->nvel.p, c->nbox.p, c->nrec.p, c->gsh.p, c->flags.p);
   ASTER_HIP(c, hipGetLastError());
   phase_end(c, ASTER_PHASE_BUILD);
-  const int nn = 2 * n - 1;
+  const int nn = 2 * n - 1, ni = n - 1;
   phase_begin(c, ASTER_PHASE_MOMENTS);
-  int32_t* lcount = c->levels.p;
-  int32_t* loff = c->levels.p + kMaxLevels;
-  int32_t* lfill = c->levels.p + 2 * kMaxLevels;
-  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(kMaxLevels), 0, c->stream, lcount, kMaxLevels);
+  // Depth census: (depth, node) pairs of the internal nodes, counting-sorted
+  // by depth with one stable radix pass (depth < 128), so every level is a
+  // contiguous bucket. The sort scratch of the Morton pass is free here.
   hipLaunchKernelGGL(k_depth, dim3((nn + 255) / 256), dim3(256), 0, c->stream, n, c->left.p, c->right.p,
-                     c->parent.p, c->depth.p, c->escape.p, lcount, c->flags.p);
-  hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(64), 0, c->stream, lcount, loff, lfill);
-  hipLaunchKernelGGL(k_bucket, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->depth.p, loff, lfill,
-                     c->bucket.p);
+                     c->parent.p, c->depth.p, c->escape.p, c->codes2.p, c->idx2.p, c->flags.p);
+  const int nblocks = (ni + kRsTile - 1) / kRsTile;
+  hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(kRsThreads), 0, c->stream, c->codes2.p, ni, 0, c->hist.p,
+                     nblocks);
+  if ((rc = device_scan(c, c->hist.p, 256 * nblocks, nullptr))) return rc;
+  hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(kRsThreads), 0, c->stream, c->codes2.p, c->idx2.p,
+                     c->dsorted.p, reinterpret_cast<uint32_t*>(c->bucket.p), ni, 0, c->hist.p, nblocks);
+  hipLaunchKernelGGL(k_level_bounds, dim3((ni + 255) / 256), dim3(256), 0, c->stream, ni, c->dsorted.p, c->levels.p);
   ASTER_HIP(c, hipGetLastError());
-  c->level_host.resize(kMaxLevels);
-  ASTER_HIP(c, hipMemcpyAsync(c->level_host.data(), lcount, kMaxLevels * sizeof(int32_t), hipMemcpyDeviceToHost,
-                              c->stream));
+  c->level_host.resize(kMaxLevels + 2);
+  ASTER_HIP(c, hipMemcpyAsync(c->level_host.data(), c->levels.p, (kMaxLevels + 2) * sizeof(int32_t),
+                              hipMemcpyDeviceToHost, c->stream));
   ASTER_HIP(c, hipStreamSynchronize(c->stream));
-  int32_t off = 0;
-  std::vector<int32_t> offs(kMaxLevels);
-  for (int d = 0; d < kMaxLevels; ++d) {
-    offs[d] = off;
-    off += c->level_host[d];
-  }
-  if (off != n - 1) return fail(c, ASTER_ERR_STATE, "radix tree depth census does not cover every internal node");
-  for (int d = kMaxLevels - 1; d >= 0; --d) {
-    const int cnt = c->level_host[d];
-    if (cnt == 0) continue;
-    hipLaunchKernelGGL(k_moments_level, dim3((cnt + 255) / 256), dim3(256), 0, c->stream, n, c->bucket.p + offs[d],
-                       cnt, c->left.p, c->right.p, c->nmass.p, c->ncom.p, c->nvel.p, c->nbox.p, c->nrec.p);
+  const int nlev = c->level_host[kMaxLevels + 1];
+  if (nlev < 1 || nlev >= kMaxLevels || c->level_host[0] != 0 || c->level_host[nlev] != ni)
+    return fail(c, ASTER_ERR_STATE, "radix tree depth census does not cover every internal node");
+  for (int d = nlev - 1; d >= 0; --d) {
+    const int off = c->level_host[d], cnt = c->level_host[d + 1] - off;
+    if (cnt <= 0) return fail(c, ASTER_ERR_STATE, "radix tree depth census has an empty level");
+    hipLaunchKernelGGL(k_moments_level, dim3((cnt + 255) / 256), dim3(256), 0, c->stream, n, c->bucket.p + off, cnt,
+                       c->left.p, c->right.p, c->nmass.p, c->ncom.p, c->nvel.p, c->nbox.p, c->nrec.p);
   }
   ASTER_HIP(c, hipGetLastError());
   phase_end(c, ASTER_PHASE_MOMENTS);
@@ -780,6 +888,26 @@ static int build_tree(aster_ctx* c, int mac) {
   phase_end(c, ASTER_PHASE_LINKS);
   c->tree_valid = true;
   c->tree_mac = mac;
+  return ASTER_OK;
+}
+
+// Groups for the wavefront traversal (subtrees of at most kGroup leaves whose
+// parent has more), as first sorted positions gstart[0..ngroups] in Morton
+// order; the count stays on the device.
+static int build_groups(aster_ctx* c) {
+  const int n = (int)c->n;
+  ASTER_HIP(c, c->gscan.reserve((size_t)n));
+  ASTER_HIP(c, c->gstart.reserve((size_t)n + 1));
+  ASTER_HIP(c, c->gcount.reserve(1));
+  phase_begin(c, ASTER_PHASE_LINKS);
+  hipLaunchKernelGGL(k_group_flags, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->parent.p, c->grange.p,
+                     c->gscan.p);
+  int rc = device_scan(c, c->gscan.p, n, c->gcount.p);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_group_emit, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->gscan.p, c->gcount.p,
+                     c->gstart.p);
+  ASTER_HIP(c, hipGetLastError());
+  phase_end(c, ASTER_PHASE_LINKS);
   return ASTER_OK;
 }
 
@@ -799,6 +927,7 @@ int bh_forces(aster_ctx* c, int method) {
   const int mac = (method == ASTER_BH_PARITY) ? 0 : 1;
   int rc = build_tree(c, mac);
   if (rc) return rc;
+  if (method == ASTER_BH_GROUP && (rc = build_groups(c))) return rc;
   phase_begin(c, ASTER_PHASE_TRAVERSE);
   const double ic2 = (1.0 / c->P.c) * (1.0 / c->P.c);
   if (f64) {
@@ -814,13 +943,17 @@ int bh_forces(aster_ctx* c, int method) {
     const float eps2 = (float)c->P.softening;
     const bool mask = !(c->P.softening > 0.0);
     if (method == ASTER_BH_GROUP) {
-      const dim3 grid((n + 63) / 64), block(64);
+      // One workgroup per group while groups average more than 32 bodies
+      // (the usual case); workgroups loop when there are more groups. The
+      // count is a multiple of 8 for the XCD-interleaved group order.
+      const int blocks = ((n / 32 + 1) + 7) & ~7;
+      const dim3 grid(blocks), block(64);
       if (mask)
-        hipLaunchKernelGGL((k_bh_group<true>), grid, block, 0, c->stream, n, c->gsh.p, c->nrec.p, c->nlink.p,
-                           c->left.p, eps2, c->P.G, c->a.p, c->bh_counters.p, c->flags.p);
+        hipLaunchKernelGGL((k_bh_group<true>), grid, block, 0, c->stream, n, c->gstart.p, c->gcount.p, c->gsh.p,
+                           c->nrec.p, c->nlink.p, c->left.p, eps2, c->P.G, c->a.p, c->bh_counters.p, c->flags.p);
       else
-        hipLaunchKernelGGL((k_bh_group<false>), grid, block, 0, c->stream, n, c->gsh.p, c->nrec.p, c->nlink.p,
-                           c->left.p, eps2, c->P.G, c->a.p, c->bh_counters.p, c->flags.p);
+        hipLaunchKernelGGL((k_bh_group<false>), grid, block, 0, c->stream, n, c->gstart.p, c->gcount.p, c->gsh.p,
+                           c->nrec.p, c->nlink.p, c->left.p, eps2, c->P.G, c->a.p, c->bh_counters.p, c->flags.p);
     } else if (mask) {
       hipLaunchKernelGGL((k_bh_f32<true>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->gsh.p, c->nrec.p,
                          c->nlink.p, eps2, c->P.G, c->a.p, c->bh_counters.p, c->flags.p);

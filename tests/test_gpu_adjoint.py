@@ -82,9 +82,17 @@ def test_gradient_finite_differences_device(ctx):
     _, gsnap = loss_and_grad_snap(snaps, target)
     gx, gv, gm = ctx.traj_backward(gsnap)
     r = np.random.default_rng(4)
-    for which, g, eps in (("x", gx, 1e-5), ("v", gv, 1e-5), ("m", gm, 1e-7)):
+    # step sizes keep the loss change well above the FP32 force rounding of
+    # the forward map (the velocity gradient is ~T/dt times smaller). Masses
+    # enter the pair forces as FP32: the mass step is snapped to whole FP32
+    # ulps of each mass, so m +- step converts to FP32 exactly and the
+    # difference sees the step that was asked for.
+    for which, g, eps in (("x", gx, 1e-5), ("v", gv, 1e-4), ("m", gm, 1e-5)):
         d = r.normal(size=g.shape)
         d /= np.linalg.norm(d)
+        if which == "m":
+            ulp = np.spacing(mass.astype(np.float32)).astype(np.float64)
+            d = np.round(eps * d / ulp) * ulp / eps
         args_p = [pos, vel, mass]
         args_m = [pos, vel, mass]
         k = "xvm".index(which)
