@@ -137,6 +137,7 @@ struct aster_ctx {
   aster::DBuf<float4> gsh;                         // bodies in sorted order (FP32), n
   bool tree_valid = false;
   int tree_mac = -1;
+  int tree_levels = 0;                             // depths of internal nodes: 0 .. tree_levels-1
 
   // Timers.
   hipEvent_t ev[16] = {};

@@ -84,6 +84,11 @@ __device__ __forceinline__ void kahan_add32(float& s, float& c, float t) {
 
 // Compiler and memory ordering point for LDS traffic between the lanes of one
 // wavefront (no cross-wave synchronisation).
+// A value known to be equal in every active lane, moved to a scalar register.
+__device__ __forceinline__ float uniform_f32(float x) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();

@@ -212,7 +212,9 @@ __device__ __forceinline__ int kdelta(const uint32_t* __restrict__ codes, int n,
 }
 
 // Thread k initialises leaf k (body perm[k]) and, for k < n-1, the internal
-// node n+k with its sorted-position range. Also writes parent links.
+// node n+k with its sorted-position range. Also writes parent links. VEL
+// also gathers the velocities (only the 1PN traversal reads node velocities).
+template <bool VEL>
 __global__ __launch_bounds__(256) void k_build(int n, const uint32_t* __restrict__ codes,
                                                const uint32_t* __restrict__ perm, const double* __restrict__ x,
                                                const double* __restrict__ v, const double* __restrict__ m,
@@ -232,9 +234,11 @@ __global__ __launch_bounds__(256) void k_build(int n, const uint32_t* __restrict
     ncom[k] = px;
     ncom[nn + k] = py;
     ncom[2 * nn + k] = pz;
-    nvel[k] = v[b];
-    nvel[nn + k] = v[n + b];
-    nvel[2 * nn + k] = v[2 * n + b];
+    if (VEL) {
+      nvel[k] = v[b];
+      nvel[nn + k] = v[n + b];
+      nvel[2 * nn + k] = v[2 * n + b];
+    }
     const float fx = (float)px, fy = (float)py, fz = (float)pz;
     nbox[k] = fx;
     nbox[nn + k] = fy;
@@ -332,7 +336,9 @@ __global__ __launch_bounds__(256) void k_level_bounds(int count, const uint32_t*
 }
 
 // Monopole moments of the internal nodes of one depth; their children are one
-// level deeper and complete (kernel boundaries order the levels).
+// level deeper and complete (kernel boundaries order the levels). VEL also
+// forms the mass-weighted velocities.
+template <bool VEL>
 __global__ __launch_bounds__(256) void k_moments_level(int n, const int32_t* __restrict__ bucket, int count,
                                                        const int32_t* __restrict__ left,
                                                        const int32_t* __restrict__ right, double* __restrict__ nmass,
@@ -358,14 +364,14 @@ __global__ __launch_bounds__(256) void k_moments_level(int n, const int32_t* __r
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         com[c] = (ncom[(size_t)c * nn + R] * mR + ncom[(size_t)c * nn + L] * mL) / mass;
-        vel[c] = (nvel[(size_t)c * nn + R] * mR + nvel[(size_t)c * nn + L] * mL) / mass;
+        if (VEL) vel[c] = (nvel[(size_t)c * nn + R] * mR + nvel[(size_t)c * nn + L] * mL) / mass;
       }
     }
     nmass[node] = mass;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       ncom[(size_t)c * nn + node] = com[c];
-      nvel[(size_t)c * nn + node] = vel[c];
+      if (VEL) nvel[(size_t)c * nn + node] = vel[c];
       nbox[(size_t)c * nn + node] = fminf(nbox[(size_t)c * nn + L], nbox[(size_t)c * nn + R]);
       nbox[(size_t)(3 + c) * nn + node] = fmaxf(nbox[(size_t)(3 + c) * nn + L], nbox[(size_t)(3 + c) * nn + R]);
     }
@@ -600,14 +606,18 @@ __global__ __launch_bounds__(256) void k_group_emit(int n, const uint32_t* __res
 // Accepted nodes are appended to an LDS interaction list that all lanes then
 // evaluate coherently (LDS broadcast reads, 12 VALU + 1 rsq per interaction),
 // FP32 sums per list flush, Kahan-compensated across flushes.
-// Stack bound: each iteration pops at most 32 opened nodes (64 children) and
-// pushes at most 64, and internal-node depth is at most 63, so fewer than
-// 32 * 63 + 64 entries are ever live.
+// Stack bound: each iteration pops the top min(sp, 32) entries and pushes
+// their opened children (at most 64) in order, so the stack stays sorted by
+// depth from bottom to top and a pop only ever leaves behind part of one
+// pushed batch: at most 32 entries per depth below a top batch of at most 64.
+// With internal nodes at depths 0..L-1 that is fewer than 32 L + 64 entries;
+// the stack is sized from L of the tree at hand (dynamic LDS).
 // Scheduling: the 8 XCDs take workgroups round-robin, so workgroup b works on
 // the groups of chunk (b mod 8), consecutive in Morton order, and neighbouring
 // groups (which open the same nodes) share an L2.
-constexpr int kGrpStack = 2304;
 constexpr int kGrpList = 256;
+
+__host__ __device__ constexpr int grp_stack_cap(int levels) { return 32 * levels + 64; }
 
 template <bool MASK>
 __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restrict__ gstart,
@@ -615,9 +625,10 @@ __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restric
                                                  const float4* __restrict__ nrec, const int4* __restrict__ nlink,
                                                  const int32_t* __restrict__ left, float eps2, double G,
                                                  double* __restrict__ a, unsigned long long* __restrict__ counters,
-                                                 uint32_t* __restrict__ flags) {
-  __shared__ int stk[kGrpStack];
-  __shared__ float4 lst[kGrpList];
+                                                 uint32_t* __restrict__ flags, int stack_cap) {
+  extern __shared__ float4 grp_lds[];
+  float4* lst = grp_lds;                                // kGrpList entries
+  int* stk = reinterpret_cast<int*>(grp_lds + kGrpList);  // stack_cap entries
   const int lane = threadIdx.x;
   const int ngroups = (int)*gcount;
   const int per_xcd = (ngroups + 7) >> 3;
@@ -634,11 +645,16 @@ __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restric
     const float4 me = gsh[live ? k : hi];
     float blo[3] = {me.x, me.y, me.z}, bhi[3] = {me.x, me.y, me.z};
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < 3; ++c) {
       for (int off = 32; off > 0; off >>= 1) {
         blo[c] = fminf(blo[c], __shfl_xor(blo[c], off, 64));
         bhi[c] = fmaxf(bhi[c], __shfl_xor(bhi[c], off, 64));
       }
+      // every lane holds the same value; saying so keeps the list count and
+      // the root test scalar
+      blo[c] = uniform_f32(blo[c]);
+      bhi[c] = uniform_f32(bhi[c]);
+    }
     float sx = 0.f, sy = 0.f, sz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
     unsigned long long listed = 0, visits = 1;
     int sp = 0, cnt = 0;
@@ -650,7 +666,9 @@ __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restric
     };
     auto flush = [&]() {
       float tx = 0.f, ty = 0.f, tz = 0.f;
-      for (int e = 0; e < cnt; ++e) {
+      const int m = __builtin_amdgcn_readfirstlane(cnt);
+#pragma unroll 4
+      for (int e = 0; e < m; ++e) {
         const float4 q = lst[e];
         const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
         float r2 = fmaf(dx, dx, eps2);
@@ -701,7 +719,7 @@ __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restric
       }
       const uint64_t am = __ballot(acc), om = __ballot(opn);
       const int nopen = (int)__popcll(om);
-      if (sp + nopen > kGrpStack) {  // cannot happen for a valid tree (see bound above)
+      if (sp + nopen > stack_cap) {  // cannot happen for a valid tree (see bound above)
         if (lane == 0) atomicOr(flags, (uint32_t)FLAG_TREE);
         overflow = true;
         break;
@@ -840,15 +858,16 @@ int morton_sort(aster_ctx* c, bool user_box) {
   return ASTER_OK;
 }
 
-static int build_tree(aster_ctx* c, int mac) {
+static int build_tree(aster_ctx* c, int mac, bool vel) {
   const int n = (int)c->n;
   int rc = reserve_tree(c);
   if (rc) return rc;
   if ((rc = morton_sort(c, true))) return rc;
   phase_begin(c, ASTER_PHASE_BUILD);
-  hipLaunchKernelGGL(k_build, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->codes.p, c->idx.p, c->x.p,
-                     c->v.p, c->m.p, c->left.p, c->right.p, c->parent.p, c->grange.p, c->nmass.p, c->ncom.p,
-                     c->nvel.p, c->nbox.p, c->nrec.p, c->gsh.p, c->flags.p);
+  auto build = vel ? k_build<true> : k_build<false>;
+  hipLaunchKernelGGL(build, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->codes.p, c->idx.p, c->x.p, c->v.p,
+                     c->m.p, c->left.p, c->right.p, c->parent.p, c->grange.p, c->nmass.p, c->ncom.p, c->nvel.p,
+                     c->nbox.p, c->nrec.p, c->gsh.p, c->flags.p);
   ASTER_HIP(c, hipGetLastError());
   phase_end(c, ASTER_PHASE_BUILD);
   const int nn = 2 * n - 1, ni = n - 1;
@@ -871,13 +890,15 @@ static int build_tree(aster_ctx* c, int mac) {
                               hipMemcpyDeviceToHost, c->stream));
   ASTER_HIP(c, hipStreamSynchronize(c->stream));
   const int nlev = c->level_host[kMaxLevels + 1];
+  c->tree_levels = nlev;
   if (nlev < 1 || nlev >= kMaxLevels || c->level_host[0] != 0 || c->level_host[nlev] != ni)
     return fail(c, ASTER_ERR_STATE, "radix tree depth census does not cover every internal node");
+  auto moments = vel ? k_moments_level<true> : k_moments_level<false>;
   for (int d = nlev - 1; d >= 0; --d) {
     const int off = c->level_host[d], cnt = c->level_host[d + 1] - off;
     if (cnt <= 0) return fail(c, ASTER_ERR_STATE, "radix tree depth census has an empty level");
-    hipLaunchKernelGGL(k_moments_level, dim3((cnt + 255) / 256), dim3(256), 0, c->stream, n, c->bucket.p + off, cnt,
-                       c->left.p, c->right.p, c->nmass.p, c->ncom.p, c->nvel.p, c->nbox.p, c->nrec.p);
+    hipLaunchKernelGGL(moments, dim3((cnt + 255) / 256), dim3(256), 0, c->stream, n, c->bucket.p + off, cnt, c->left.p,
+                       c->right.p, c->nmass.p, c->ncom.p, c->nvel.p, c->nbox.p, c->nrec.p);
   }
   ASTER_HIP(c, hipGetLastError());
   phase_end(c, ASTER_PHASE_MOMENTS);
@@ -925,7 +946,7 @@ int bh_forces(aster_ctx* c, int method) {
     return ASTER_OK;
   }
   const int mac = (method == ASTER_BH_PARITY) ? 0 : 1;
-  int rc = build_tree(c, mac);
+  int rc = build_tree(c, mac, pn);
   if (rc) return rc;
   if (method == ASTER_BH_GROUP && (rc = build_groups(c))) return rc;
   phase_begin(c, ASTER_PHASE_TRAVERSE);
@@ -948,12 +969,11 @@ int bh_forces(aster_ctx* c, int method) {
       // count is a multiple of 8 for the XCD-interleaved group order.
       const int blocks = ((n / 32 + 1) + 7) & ~7;
       const dim3 grid(blocks), block(64);
-      if (mask)
-        hipLaunchKernelGGL((k_bh_group<true>), grid, block, 0, c->stream, n, c->gstart.p, c->gcount.p, c->gsh.p,
-                           c->nrec.p, c->nlink.p, c->left.p, eps2, c->P.G, c->a.p, c->bh_counters.p, c->flags.p);
-      else
-        hipLaunchKernelGGL((k_bh_group<false>), grid, block, 0, c->stream, n, c->gstart.p, c->gcount.p, c->gsh.p,
-                           c->nrec.p, c->nlink.p, c->left.p, eps2, c->P.G, c->a.p, c->bh_counters.p, c->flags.p);
+      const int cap = grp_stack_cap(c->tree_levels);
+      const size_t lds = kGrpList * sizeof(float4) + (size_t)cap * sizeof(int);
+      auto kern = mask ? k_bh_group<true> : k_bh_group<false>;
+      hipLaunchKernelGGL(kern, grid, block, lds, c->stream, n, c->gstart.p, c->gcount.p, c->gsh.p, c->nrec.p,
+                         c->nlink.p, c->left.p, eps2, c->P.G, c->a.p, c->bh_counters.p, c->flags.p, cap);
     } else if (mask) {
       hipLaunchKernelGGL((k_bh_f32<true>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->gsh.p, c->nrec.p,
                          c->nlink.p, eps2, c->P.G, c->a.p, c->bh_counters.p, c->flags.p);

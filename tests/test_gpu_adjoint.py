@@ -83,11 +83,12 @@ def test_gradient_finite_differences_device(ctx):
     gx, gv, gm = ctx.traj_backward(gsnap)
     r = np.random.default_rng(4)
     # step sizes keep the loss change well above the FP32 force rounding of
-    # the forward map (the velocity gradient is ~T/dt times smaller). Masses
-    # enter the pair forces as FP32: the mass step is snapped to whole FP32
-    # ulps of each mass, so m +- step converts to FP32 exactly and the
-    # difference sees the step that was asked for.
-    for which, g, eps in (("x", gx, 1e-5), ("v", gv, 1e-4), ("m", gm, 1e-5)):
+    # the forward map (the velocity gradient is ~T/dt times smaller, the mass
+    # gradient smaller still: eps 1e-4 moves each mass by ~0.3%, central
+    # truncation ~1e-5). Masses enter the pair forces as FP32: the mass step
+    # is snapped to whole FP32 ulps of each mass, so m +- step converts to
+    # FP32 exactly and the difference sees the step that was asked for.
+    for which, g, eps in (("x", gx, 1e-5), ("v", gv, 1e-4), ("m", gm, 1e-4)):
         d = r.normal(size=g.shape)
         d /= np.linalg.norm(d)
         if which == "m":
