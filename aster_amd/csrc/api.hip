@@ -144,7 +144,7 @@ int aster_destroy(aster_ctx* c) {
   c->boxpart.release(); c->boxd.release();
   c->codes.release(); c->idx.release(); c->codes2.release(); c->idx2.release(); c->hist.release();
   c->left.release(); c->right.release(); c->parent.release(); c->escape.release(); c->scanpart.release();
-  c->depth.release(); c->nmass.release(); c->ncom.release(); c->nvel.release(); c->nbox.release();
+  c->depth.release(); c->nmc.release(); c->nvel.release(); c->nbox.release();
   c->levels.release(); c->bucket.release(); c->dsorted.release(); c->grange.release(); c->gscan.release();
   c->gstart.release(); c->gcount.release();
   c->nrec.release(); c->nlink.release(); c->bh_counters.release(); c->gsh.release();
@@ -348,16 +348,20 @@ int aster_tree_dump(aster_ctx* c, uint32_t* codes, uint32_t* perm, int32_t* left
   if (perm && (rc = cp(perm, c->idx.p, n * sizeof(uint32_t)))) return rc;
   if (left && (rc = cp(left, c->left.p, nn * sizeof(int32_t)))) return rc;
   if (right && (rc = cp(right, c->right.p, nn * sizeof(int32_t)))) return rc;
-  if (node_mass && (rc = cp(node_mass, c->nmass.p, nn * sizeof(double)))) return rc;
-  std::vector<double> com;
-  if (node_com) {
-    com.resize((size_t)3 * nn);
-    if ((rc = cp(com.data(), c->ncom.p, 3 * nn * sizeof(double)))) return rc;
+  std::vector<double4> mc;
+  if (node_mass || node_com) {
+    mc.resize((size_t)nn);
+    if ((rc = cp(mc.data(), c->nmc.p, nn * sizeof(double4)))) return rc;
   }
   ASTER_HIP(c, hipStreamSynchronize(c->stream));
-  if (node_com)
-    for (int64_t k = 0; k < nn; ++k)
-      for (int d = 0; d < 3; ++d) node_com[3 * k + d] = com[(size_t)d * nn + k];
+  for (int64_t k = 0; k < (int64_t)mc.size(); ++k) {
+    if (node_mass) node_mass[k] = mc[k].w;
+    if (node_com) {
+      node_com[3 * k] = mc[k].x;
+      node_com[3 * k + 1] = mc[k].y;
+      node_com[3 * k + 2] = mc[k].z;
+    }
+  }
   return ASTER_OK;
 }
 

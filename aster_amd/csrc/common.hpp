@@ -129,8 +129,9 @@ struct aster_ctx {
   aster::DBuf<int32_t> gstart;                     // first sorted position of each group (+ n)
   aster::DBuf<uint32_t> gcount;                    // number of groups (1)
   std::vector<int32_t> level_host;                 // per-depth counts on the host
-  aster::DBuf<double> nmass, ncom, nvel;           // FP64 moments: 2n-1, [3][2n-1], [3][2n-1]
-  aster::DBuf<float> nbox;                         // FP32 AABB [6][2n-1]
+  aster::DBuf<double4> nmc;                        // FP64 {com, mass} per node (2n-1)
+  aster::DBuf<double4> nvel;                       // FP64 {mass-weighted velocity, 0} per node (1PN only)
+  aster::DBuf<float4> nbox;                        // FP32 AABB: {lo, 0}, {hi, 0} per node (2 (2n-1))
   aster::DBuf<float4> nrec;                        // FP32 node record: com, mass (2n-1)
   aster::DBuf<int4> nlink;                         // right, escape, crit2 bits, leaf body (2n-1)
   aster::DBuf<unsigned long long> bh_counters;     // BHC_COUNT

@@ -220,32 +220,20 @@ __global__ __launch_bounds__(256) void k_build(int n, const uint32_t* __restrict
                                                const double* __restrict__ v, const double* __restrict__ m,
                                                int32_t* __restrict__ left, int32_t* __restrict__ right,
                                                int32_t* __restrict__ parent, int2* __restrict__ grange,
-                                               double* __restrict__ nmass, double* __restrict__ ncom,
-                                               double* __restrict__ nvel, float* __restrict__ nbox,
-                                               float4* __restrict__ nrec, float4* __restrict__ gsh,
-                                               uint32_t* __restrict__ flags) {
+                                               double4* __restrict__ nmc, double4* __restrict__ nvel,
+                                               float4* __restrict__ nbox, float4* __restrict__ nrec,
+                                               float4* __restrict__ gsh, uint32_t* __restrict__ flags) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  const int nn = 2 * n - 1;
   {
     const uint32_t b = perm[k];
     const double px = x[b], py = x[n + b], pz = x[2 * n + b], mb = m[b];
-    nmass[k] = mb;
-    ncom[k] = px;
-    ncom[nn + k] = py;
-    ncom[2 * nn + k] = pz;
-    if (VEL) {
-      nvel[k] = v[b];
-      nvel[nn + k] = v[n + b];
-      nvel[2 * nn + k] = v[2 * n + b];
-    }
+    nmc[k] = make_double4(px, py, pz, mb);
+    if (VEL) nvel[k] = make_double4(v[b], v[n + b], v[2 * n + b], 0.0);
     const float fx = (float)px, fy = (float)py, fz = (float)pz;
-    nbox[k] = fx;
-    nbox[nn + k] = fy;
-    nbox[2 * nn + k] = fz;
-    nbox[3 * nn + k] = fx;
-    nbox[4 * nn + k] = fy;
-    nbox[5 * nn + k] = fz;
+    const float4 p = make_float4(fx, fy, fz, 0.f);
+    nbox[2 * k] = p;
+    nbox[2 * k + 1] = p;
     const float4 r = make_float4(fx, fy, fz, (float)mb);
     nrec[k] = r;
     gsh[k] = r;
@@ -275,7 +263,7 @@ __global__ __launch_bounds__(256) void k_build(int n, const uint32_t* __restrict
   const int lo = min(i, j), hi = max(i, j);
   const int L = (lo == gamma) ? gamma : n + gamma;
   const int R = (hi == gamma + 1) ? gamma + 1 : n + gamma + 1;
-  if (gamma < 0 || gamma > n - 2 || R >= nn) {  // cannot happen for a sorted key array
+  if (gamma < 0 || gamma > n - 2 || R >= 2 * n - 1) {  // cannot happen for a sorted key array
     atomicOr(flags, (uint32_t)FLAG_TREE);
     return;
   }
@@ -341,42 +329,36 @@ __global__ __launch_bounds__(256) void k_level_bounds(int count, const uint32_t*
 template <bool VEL>
 __global__ __launch_bounds__(256) void k_moments_level(int n, const int32_t* __restrict__ bucket, int count,
                                                        const int32_t* __restrict__ left,
-                                                       const int32_t* __restrict__ right, double* __restrict__ nmass,
-                                                       double* __restrict__ ncom, double* __restrict__ nvel,
-                                                       float* __restrict__ nbox, float4* __restrict__ nrec) {
+                                                       const int32_t* __restrict__ right, double4* __restrict__ nmc,
+                                                       double4* __restrict__ nvel, float4* __restrict__ nbox,
+                                                       float4* __restrict__ nrec) {
 #pragma clang fp contract(off)
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= count) return;
-  const int nn = 2 * n - 1;
-  {
-    const int node = bucket[t];
-    const int L = left[node], R = right[node];
-    const double mR = nmass[R], mL = nmass[L];
-    const double total = mR + mL;
-    double com[3], vel[3];
-    double mass;
-    if (total <= 0) {
-      mass = 0.0;
-      com[0] = com[1] = com[2] = 0.0;
-      vel[0] = vel[1] = vel[2] = 0.0;
-    } else {
-      mass = total;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        com[c] = (ncom[(size_t)c * nn + R] * mR + ncom[(size_t)c * nn + L] * mL) / mass;
-        if (VEL) vel[c] = (nvel[(size_t)c * nn + R] * mR + nvel[(size_t)c * nn + L] * mL) / mass;
-      }
+  const int node = bucket[t];
+  const int L = left[node], R = right[node];
+  const double4 cR = nmc[R], cL = nmc[L];
+  const double mR = cR.w, mL = cL.w;
+  const double total = mR + mL;
+  double4 com = make_double4(0.0, 0.0, 0.0, 0.0), vel = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (total > 0) {
+    com.w = total;
+    com.x = (cR.x * mR + cL.x * mL) / total;
+    com.y = (cR.y * mR + cL.y * mL) / total;
+    com.z = (cR.z * mR + cL.z * mL) / total;
+    if (VEL) {
+      const double4 vR = nvel[R], vL = nvel[L];
+      vel.x = (vR.x * mR + vL.x * mL) / total;
+      vel.y = (vR.y * mR + vL.y * mL) / total;
+      vel.z = (vR.z * mR + vL.z * mL) / total;
     }
-    nmass[node] = mass;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      ncom[(size_t)c * nn + node] = com[c];
-      if (VEL) nvel[(size_t)c * nn + node] = vel[c];
-      nbox[(size_t)c * nn + node] = fminf(nbox[(size_t)c * nn + L], nbox[(size_t)c * nn + R]);
-      nbox[(size_t)(3 + c) * nn + node] = fmaxf(nbox[(size_t)(3 + c) * nn + L], nbox[(size_t)(3 + c) * nn + R]);
-    }
-    nrec[node] = make_float4((float)com[0], (float)com[1], (float)com[2], (float)mass);
   }
+  nmc[node] = com;
+  if (VEL) nvel[node] = vel;
+  const float4 loL = nbox[2 * L], hiL = nbox[2 * L + 1], loR = nbox[2 * R], hiR = nbox[2 * R + 1];
+  nbox[2 * node] = make_float4(fminf(loL.x, loR.x), fminf(loL.y, loR.y), fminf(loL.z, loR.z), 0.f);
+  nbox[2 * node + 1] = make_float4(fmaxf(hiL.x, hiR.x), fmaxf(hiL.y, hiR.y), fmaxf(hiL.z, hiR.z), 0.f);
+  nrec[node] = make_float4((float)com.x, (float)com.y, (float)com.z, (float)com.w);
 }
 
 // Per node: traversal record = right child (or -1), escape link, opening
@@ -386,7 +368,7 @@ __global__ __launch_bounds__(256) void k_moments_level(int n, const int32_t* __r
 //                    delta = |COM - AABB centre|.
 __global__ __launch_bounds__(256) void k_links(int n, const int32_t* __restrict__ right,
                                                const int32_t* __restrict__ escape, const int32_t* __restrict__ depth,
-                                               const float* __restrict__ nbox, const float4* __restrict__ nrec,
+                                               const float4* __restrict__ nbox, const float4* __restrict__ nrec,
                                                const double* __restrict__ boxd, double theta, int mac,
                                                int4* __restrict__ nlink, const uint32_t* __restrict__ perm) {
   const int node = blockIdx.x * blockDim.x + threadIdx.x;
@@ -400,14 +382,12 @@ __global__ __launch_bounds__(256) void k_links(int n, const int32_t* __restrict_
     const double q = s / theta;
     crit2 = (float)(q * q);
   } else {
-    const float lx = nbox[(size_t)3 * nn + node] - nbox[node];
-    const float ly = nbox[(size_t)4 * nn + node] - nbox[(size_t)nn + node];
-    const float lz = nbox[(size_t)5 * nn + node] - nbox[(size_t)2 * nn + node];
-    const float l = fmaxf(lx, fmaxf(ly, lz));
+    const float4 lo = nbox[2 * node], hi = nbox[2 * node + 1];
+    const float l = fmaxf(hi.x - lo.x, fmaxf(hi.y - lo.y, hi.z - lo.z));
     const float4 r = nrec[node];
-    const float cx = 0.5f * (nbox[(size_t)3 * nn + node] + nbox[node]);
-    const float cy = 0.5f * (nbox[(size_t)4 * nn + node] + nbox[(size_t)nn + node]);
-    const float cz = 0.5f * (nbox[(size_t)5 * nn + node] + nbox[(size_t)2 * nn + node]);
+    const float cx = 0.5f * (hi.x + lo.x);
+    const float cy = 0.5f * (hi.y + lo.y);
+    const float cz = 0.5f * (hi.z + lo.z);
     const float ex = r.x - cx, ey = r.y - cy, ez = r.z - cz;
     const float delta = sqrtf(ex * ex + ey * ey + ez * ez);
     const float q = l / (float)theta + delta;
@@ -481,9 +461,8 @@ __global__ __launch_bounds__(256) void k_bh_f32(int n, const float4* __restrict_
 // (get_node_body, barnes_hut.cpp:202-261).
 template <bool PN>
 __global__ __launch_bounds__(256) void k_bh_f64(int n, const double* __restrict__ x, const double* __restrict__ v,
-                                                const double* __restrict__ m, const double* __restrict__ nmass,
-                                                const double* __restrict__ ncom, const double* __restrict__ nvel,
-                                                const int4* __restrict__ nlink, const int32_t* __restrict__ depth,
+                                                const double* __restrict__ m, const double4* __restrict__ nmc,
+                                                const double4* __restrict__ nvel, const int4* __restrict__ nlink, const int32_t* __restrict__ depth,
                                                 const double* __restrict__ boxd, double theta, double G, double ic2,
                                                 double soft, double* __restrict__ a,
                                                 unsigned long long* __restrict__ counters,
@@ -491,7 +470,6 @@ __global__ __launch_bounds__(256) void k_bh_f64(int n, const double* __restrict_
 #pragma clang fp contract(off)
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = k < n;
-  const int nn = 2 * n - 1;
   const int b = nlink[live ? k : n - 1].w;
   const double pix = x[b], piy = x[n + b], piz = x[2 * n + b];
   double vix = 0.0, viy = 0.0, viz = 0.0;
@@ -515,7 +493,8 @@ __global__ __launch_bounds__(256) void k_bh_f64(int n, const double* __restrict_
     }
     const int4 L = nlink[node];
     ++visits;
-    const double cx = ncom[node], cy = ncom[nn + node], cz = ncom[2 * nn + node];
+    const double4 nc = nmc[node];
+    const double cx = nc.x, cy = nc.y, cz = nc.z;
     const double dx = pix - cx, dy = piy - cy, dz = piz - cz;
     const double d2 = dx * dx + dy * dy + dz * dz;
     if (d2 == 0.0) {
@@ -526,12 +505,13 @@ __global__ __launch_bounds__(256) void k_bh_f64(int n, const double* __restrict_
     if ((s * s < d2 * theta2) || node < n) {
       double vnx = 0.0, vny = 0.0, vnz = 0.0;
       if (PN) {
-        vnx = nvel[node];
-        vny = nvel[nn + node];
-        vnz = nvel[2 * nn + node];
+        const double4 nv = nvel[node];
+        vnx = nv.x;
+        vny = nv.y;
+        vnz = nv.z;
       }
       double fx, fy, fz;
-      grav64(mi, nmass[node], vix, viy, viz, vnx, vny, vnz, pix, piy, piz, cx, cy, cz, G, ic2, soft, PN, fx, fy, fz);
+      grav64(mi, nc.w, vix, viy, viz, vnx, vny, vnz, pix, piy, piz, cx, cy, cz, G, ic2, soft, PN, fx, fy, fz);
       ax += fx * inv_m;
       ay += fy * inv_m;
       az += fz * inv_m;
@@ -798,10 +778,8 @@ static int reserve_tree(aster_ctx* c) {
   ASTER_HIP(c, c->bucket.reserve(n));
   ASTER_HIP(c, c->dsorted.reserve(n));
   ASTER_HIP(c, c->grange.reserve(n));
-  ASTER_HIP(c, c->nmass.reserve(nn));
-  ASTER_HIP(c, c->ncom.reserve(3 * nn));
-  ASTER_HIP(c, c->nvel.reserve(3 * nn));
-  ASTER_HIP(c, c->nbox.reserve(6 * nn));
+  ASTER_HIP(c, c->nmc.reserve(nn));
+  ASTER_HIP(c, c->nbox.reserve(2 * nn));
   ASTER_HIP(c, c->nrec.reserve(nn));
   ASTER_HIP(c, c->nlink.reserve(nn));
   ASTER_HIP(c, c->gsh.reserve(n));
@@ -864,10 +842,11 @@ static int build_tree(aster_ctx* c, int mac, bool vel) {
   if (rc) return rc;
   if ((rc = morton_sort(c, true))) return rc;
   phase_begin(c, ASTER_PHASE_BUILD);
+  if (vel) ASTER_HIP(c, c->nvel.reserve((size_t)2 * n - 1));
   auto build = vel ? k_build<true> : k_build<false>;
   hipLaunchKernelGGL(build, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->codes.p, c->idx.p, c->x.p, c->v.p,
-                     c->m.p, c->left.p, c->right.p, c->parent.p, c->grange.p, c->nmass.p, c->ncom.p, c->nvel.p,
-                     c->nbox.p, c->nrec.p, c->gsh.p, c->flags.p);
+                     c->m.p, c->left.p, c->right.p, c->parent.p, c->grange.p, c->nmc.p, c->nvel.p, c->nbox.p,
+                     c->nrec.p, c->gsh.p, c->flags.p);
   ASTER_HIP(c, hipGetLastError());
   phase_end(c, ASTER_PHASE_BUILD);
   const int nn = 2 * n - 1, ni = n - 1;
@@ -898,7 +877,7 @@ static int build_tree(aster_ctx* c, int mac, bool vel) {
     const int off = c->level_host[d], cnt = c->level_host[d + 1] - off;
     if (cnt <= 0) return fail(c, ASTER_ERR_STATE, "radix tree depth census has an empty level");
     hipLaunchKernelGGL(moments, dim3((cnt + 255) / 256), dim3(256), 0, c->stream, n, c->bucket.p + off, cnt, c->left.p,
-                       c->right.p, c->nmass.p, c->ncom.p, c->nvel.p, c->nbox.p, c->nrec.p);
+                       c->right.p, c->nmc.p, c->nvel.p, c->nbox.p, c->nrec.p);
   }
   ASTER_HIP(c, hipGetLastError());
   phase_end(c, ASTER_PHASE_MOMENTS);
@@ -954,11 +933,11 @@ int bh_forces(aster_ctx* c, int method) {
   if (f64) {
     if (pn)
       hipLaunchKernelGGL((k_bh_f64<true>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->x.p, c->v.p, c->m.p,
-                         c->nmass.p, c->ncom.p, c->nvel.p, c->nlink.p, c->depth.p, c->boxd.p, c->P.theta, c->P.G, ic2,
+                         c->nmc.p, c->nvel.p, c->nlink.p, c->depth.p, c->boxd.p, c->P.theta, c->P.G, ic2,
                          c->P.softening, c->a.p, c->bh_counters.p, c->flags.p);
     else
       hipLaunchKernelGGL((k_bh_f64<false>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->x.p, c->v.p, c->m.p,
-                         c->nmass.p, c->ncom.p, c->nvel.p, c->nlink.p, c->depth.p, c->boxd.p, c->P.theta, c->P.G, ic2,
+                         c->nmc.p, c->nvel.p, c->nlink.p, c->depth.p, c->boxd.p, c->P.theta, c->P.G, ic2,
                          c->P.softening, c->a.p, c->bh_counters.p, c->flags.p);
   } else {
     const float eps2 = (float)c->P.softening;
