@@ -31,7 +31,10 @@ BH_GROUP = 5
 # Precision (enum aster_precision).
 FP32 = 0
 FP64 = 1
-# Integrators (enum aster_integrator; LEAPFROG = 2 as in basic.h:56).
+# Integrators (enum aster_integrator; EULER = 0, SABA = 1, LEAPFROG = 2 as in
+# basic.h:56).
+EULER = 0
+SABA = 1
 LEAPFROG = 2
 YOSHIDA4 = 5
 # Force-law flags (basic.h:39-40).
@@ -87,7 +90,9 @@ SIGNATURES = {
     "aster_forces": (ctypes.c_int, [_P, ctypes.c_int, _D]),
     "aster_get_acc": (ctypes.c_int, [_P, _D]),
     "aster_step": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_double, _I64]),
+    "aster_integrate": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _I64]),
     "aster_yoshida_weights": (ctypes.c_int, [_D]),
+    "aster_saba_coeffs": (ctypes.c_int, [ctypes.c_int, _D, _D]),
     "aster_energy": (ctypes.c_int, [_P, _D, _D, _D]),
     "aster_tree_dump": (ctypes.c_int, [_P, _PU32, _PU32, _PI32, _PI32, _D, _D]),
     "aster_morton": (ctypes.c_int, [_P, _PU32, _D]),
@@ -223,6 +228,10 @@ class Context:
     def step(self, method, integrator, dt, nsteps):
         self._ck(lib().aster_step(self.h, method, integrator, float(dt), int(nsteps)))
 
+    def integrate(self, method, integrator, order, dt, nsteps):
+        """aster_integrate: SABA of `order` (1..10), Euler, or the step integrators."""
+        self._ck(lib().aster_integrate(self.h, method, integrator, int(order), float(dt), int(nsteps)))
+
     def energy(self):
         e = ctypes.c_double()
         ep = ctypes.c_double()
@@ -347,6 +356,14 @@ def yoshida_weights():
     w = np.empty(3)
     _check(lib().aster_yoshida_weights(_dp(w)))
     return w
+
+
+def saba_coeffs(order):
+    """(drifts c, order + 1 entries; kicks d, order entries) of SABAn, order 2..10."""
+    cs = np.empty(order + 1)
+    ds = np.empty(order)
+    _check(lib().aster_saba_coeffs(int(order), _dp(cs), _dp(ds)))
+    return cs, ds
 
 
 # ---- seeded initial conditions -------------------------------------------

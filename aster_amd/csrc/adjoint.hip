@@ -320,6 +320,7 @@ int traj_backward(aster_ctx* c, const double* gsnap, double* gx, double* gv, dou
   // The state is back at step 0 (re-anchored to the first checkpoint) with
   // a = a(x_0) from the mixed-precision kernel.
   c->acc_valid = true;
+  c->acc_stale = false;
   c->acc_method = ASTER_ALLPAIRS_FP32_KAHAN;
   S.ready = false;
   auto down = [&](const double* dev, double* host) -> int {

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "saba_coeffs.hpp"
 
 namespace aster {
 
@@ -54,6 +55,7 @@ int compute_forces(aster_ctx* c, int method) {
     return fail(c, ASTER_ERR_INVALID, "unknown force method");
   if (rc) return rc;
   c->acc_valid = true;
+  c->acc_stale = false;
   c->acc_method = method;
   return ASTER_OK;
 }
@@ -279,8 +281,11 @@ int aster_yoshida_weights(double* w) {
 int aster_step(aster_ctx* c, int method, int integrator, double dt, int64_t nsteps) {
   if (!c) return ASTER_ERR_INVALID;
   if (!c->have_state) return fail(c, ASTER_ERR_STATE, "no state loaded");
+  if (integrator == ASTER_EULER) return aster_integrate(c, method, integrator, 1, dt, nsteps);
+  if (integrator == ASTER_SABA)
+    return fail(c, ASTER_ERR_INVALID, "ASTER_SABA needs an order: call aster_integrate");
   if (integrator != ASTER_LEAPFROG && integrator != ASTER_YOSHIDA4)
-    return fail(c, ASTER_ERR_INVALID, "integrator must be ASTER_LEAPFROG or ASTER_YOSHIDA4");
+    return fail(c, ASTER_ERR_INVALID, "integrator must be ASTER_EULER, ASTER_SABA, ASTER_LEAPFROG or ASTER_YOSHIDA4");
   if (!std::isfinite(dt) || nsteps < 0) return fail(c, ASTER_ERR_INVALID, "invalid dt or nsteps");
   if (nsteps == 0) return ASTER_OK;
   ASTER_HIP(c, hipSetDevice(c->device));
@@ -296,7 +301,7 @@ int aster_step(aster_ctx* c, int method, int integrator, double dt, int64_t nste
   // previous step's second evaluation: re-evaluate, as update_leapfrog does.
   // Newtonian forces depend on positions only and are reused.
   const bool vdep = (c->P.pn_flags & ASTER_PN1) != 0;
-  if (!vdep && (!c->acc_valid || c->acc_method != method))
+  if (!vdep && (!c->acc_valid || c->acc_stale || c->acc_method != method))
     if ((rc = compute_forces(c, method))) return rc;
   double pending = 0.0;
   for (int64_t s = 0; s < nsteps; ++s)
@@ -313,6 +318,47 @@ int aster_step(aster_ctx* c, int method, int integrator, double dt, int64_t nste
     }
   if ((rc = launch_kick(c, pending))) return rc;
   c->adj.ready = false;
+  return ASTER_OK;
+}
+
+int aster_integrate(aster_ctx* c, int method, int integrator, int order, double dt, int64_t nsteps) {
+  if (!c) return ASTER_ERR_INVALID;
+  if (integrator == ASTER_LEAPFROG || integrator == ASTER_YOSHIDA4)
+    return aster_step(c, method, integrator, dt, nsteps);
+  if (integrator != ASTER_EULER && integrator != ASTER_SABA)
+    return fail(c, ASTER_ERR_INVALID, "integrator must be ASTER_EULER, ASTER_SABA, ASTER_LEAPFROG or ASTER_YOSHIDA4");
+  if (integrator == ASTER_EULER) order = 1;
+  if (order < 1 || order > kSabaMaxOrder) return fail(c, ASTER_ERR_INVALID, "SABA order must be 1..10");
+  if (!c->have_state) return fail(c, ASTER_ERR_STATE, "no state loaded");
+  if (!std::isfinite(dt) || nsteps < 0) return fail(c, ASTER_ERR_INVALID, "invalid dt or nsteps");
+  if (nsteps == 0) return ASTER_OK;
+  ASTER_HIP(c, hipSetDevice(c->device));
+  int rc;
+  for (int64_t s = 0; s < nsteps; ++s) {
+    if (order == 1) {
+      if ((rc = compute_forces(c, method))) return rc;
+      if ((rc = launch_saba_stage(c, true, 1.0, 1.0, dt))) return rc;
+      continue;
+    }
+    const double* cs = kSabaC[order];
+    const double* ds = kSabaD[order];
+    if ((rc = launch_saba_stage(c, false, 0.0, cs[0], dt))) return rc;
+    for (int st = 0; st < order; ++st) {
+      if ((rc = compute_forces(c, method))) return rc;
+      if ((rc = launch_saba_stage(c, true, ds[st], cs[st + 1], dt))) return rc;
+    }
+  }
+  // every stage ends with a drift: the accelerations are those of the last
+  // evaluation, not of the current positions
+  c->acc_stale = true;
+  c->adj.ready = false;
+  return ASTER_OK;
+}
+
+int aster_saba_coeffs(int order, double* cs, double* ds) {
+  if (order < 2 || order > kSabaMaxOrder || !cs || !ds) return ASTER_ERR_INVALID;
+  for (int i = 0; i <= order; ++i) cs[i] = kSabaC[order][i];
+  for (int i = 0; i < order; ++i) ds[i] = kSabaD[order][i];
   return ASTER_OK;
 }
 

@@ -101,6 +101,7 @@ struct aster_ctx {
   int64_t npad = 0;  // n rounded up to the all-pairs block
   bool have_state = false;
   bool acc_valid = false;
+  bool acc_stale = false;  // positions moved after the last evaluation
   int acc_method = -1;
 
   // State, device resident. x, v, a: SoA planes [3][n]; m: [n].
@@ -174,6 +175,7 @@ inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 int launch_allpairs(aster_ctx* c, int method);           // writes c->a
 int launch_kick_drift(aster_ctx* c, double dt_prev, double dt);
 int launch_kick(aster_ctx* c, double dt);
+int launch_saba_stage(aster_ctx* c, bool kick, double d, double cdrift, double dt);
 int launch_shadow(aster_ctx* c);
 int launch_energy(aster_ctx* c, double* out6_host);
 int launch_reset_flags(aster_ctx* c);

@@ -34,6 +34,33 @@ __global__ __launch_bounds__(256) void k_kick_drift(int n, double* __restrict__ 
   if (!finite) atomicOr(flags, (uint32_t)FLAG_NONFINITE);
 }
 
+// One SABA stage after its force evaluation, as the per-body lambdas of
+// update_SABAn (src/integrators/SABA.cpp:19-553): v += (a d) dt, then
+// x += (v c) dt with the kicked v. kick = 0 is the sequence's opening drift.
+// update_euler (tool_chain_impl.cpp:158-167) is d = c = 1, since (a 1) dt and
+// (v 1) dt round exactly like a dt and v dt.
+__global__ __launch_bounds__(256) void k_saba_stage(int n, double* __restrict__ x, double* __restrict__ v,
+                                                    const double* __restrict__ a, int kick, double d, double cdrift,
+                                                    double dt, uint32_t* __restrict__ flags) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  bool finite = true;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t k = (size_t)c * n + i;
+    double vc = v[k];
+    if (kick) {
+      vc = vc + a[k] * d * dt;
+      v[k] = vc;
+    }
+    const double xc = x[k] + vc * cdrift * dt;
+    x[k] = xc;
+    finite = finite && isfinite(xc) && isfinite(vc);
+  }
+  if (!finite) atomicOr(flags, (uint32_t)FLAG_NONFINITE);
+}
+
 __global__ __launch_bounds__(256) void k_kick(int n, double* __restrict__ v, const double* __restrict__ a,
                                               double dt) {
 #pragma clang fp contract(off)
@@ -169,6 +196,16 @@ int launch_kick_drift(aster_ctx* c, double dt_prev, double dt) {
   phase_begin(c, ASTER_PHASE_KICKDRIFT);
   hipLaunchKernelGGL(k_kick_drift, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->x.p, c->v.p, c->a.p,
                      dt_prev, dt, c->flags.p);
+  ASTER_HIP(c, hipGetLastError());
+  phase_end(c, ASTER_PHASE_KICKDRIFT);
+  return ASTER_OK;
+}
+
+int launch_saba_stage(aster_ctx* c, bool kick, double d, double cdrift, double dt) {
+  const int n = (int)c->n;
+  phase_begin(c, ASTER_PHASE_KICKDRIFT);
+  hipLaunchKernelGGL(k_saba_stage, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->x.p, c->v.p, c->a.p,
+                     kick ? 1 : 0, d, cdrift, dt, c->flags.p);
   ASTER_HIP(c, hipGetLastError());
   phase_end(c, ASTER_PHASE_KICKDRIFT);
   return ASTER_OK;

@@ -74,8 +74,8 @@ enum aster_method {
 enum aster_precision { ASTER_FP32 = 0, ASTER_FP64 = 1 };
 
 /* Integrators: values follow the reference's update_type where one exists
- * (LEAPFROG = 2, include/Aster/simulations/basic.h:56). */
-enum aster_integrator { ASTER_LEAPFROG = 2, ASTER_YOSHIDA4 = 5 };
+ * (EULER = 0, SABA = 1, LEAPFROG = 2, include/Aster/simulations/basic.h:56). */
+enum aster_integrator { ASTER_EULER = 0, ASTER_SABA = 1, ASTER_LEAPFROG = 2, ASTER_YOSHIDA4 = 5 };
 
 typedef struct aster_params {
   double G;         /* gravitational constant (reference default 6e-11) */
@@ -130,8 +130,28 @@ int aster_get_acc(aster_ctx* ctx, double* acc);
  * synchronised at the end of the call. */
 int aster_step(aster_ctx* ctx, int method, int integrator, double dt, int64_t nsteps);
 
+/* aster_step with an order, for the integrators that take one (the
+ * reference's bake_update_function(update_type, int ord),
+ * src/utils/updater.cpp:144-152):
+ *  - ASTER_SABA, order 2..10: update_SABAn (src/integrators/SABA.cpp:19-553),
+ *    drift c1, then per stage one force evaluation, a kick (a d_k) dt and a
+ *    drift (v c_(k+1)) dt; n force evaluations per step. Order 6 uses the
+ *    symmetric kick sequence (the reference's fifth stage kicks with d2,
+ *    SABA.cpp:216, so its kicks sum to 0.946);
+ *  - ASTER_SABA order 1 and ASTER_EULER: update_euler (tool_chain_impl.cpp:
+ *    158-167), which update_SABA1 calls: v += a dt, then x += v dt;
+ *  - ASTER_LEAPFROG and ASTER_YOSHIDA4 ignore `order` (aster_step).
+ * The accelerations left on the device are those of the last evaluation. */
+int aster_integrate(aster_ctx* ctx, int method, int integrator, int order, double dt, int64_t nsteps);
+
 /* Yoshida weights used by ASTER_YOSHIDA4: w[0] = w1, w[1] = w0, w[2] = w1. */
 int aster_yoshida_weights(double* w);
+
+/* SABAn coefficients used by ASTER_SABA (order 2..10): drifts c[0..order]
+ * and kicks d[0..order-1] (GPU::saba_coeffs, include/Aster/physics/
+ * coeffs.h:8-161, stores them interleaved). They are the Gauss-Legendre
+ * nodes (as drift increments) and weights on [0, 1]. */
+int aster_saba_coeffs(int order, double* c, double* d);
 
 /* Energies and angular momentum of the current state, FP64:
  * e_ref     = sum 1/2 m v^2 - sum_{i<j} G m_i m_j / (|r_ij| + softening)

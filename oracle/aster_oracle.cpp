@@ -25,9 +25,15 @@
 //  * Yoshida-4 has no counterpart in the reference; it is the composition of
 //    three reference leapfrog steps (tool_chain_impl.cpp:169-185) with the
 //    Yoshida (1990) weights.
+//  * update_SABA6 (SABA.cpp:176-233) kicks with d2 in its fifth stage
+//    (SABA.cpp:216) where the symmetric sequence needs d3, so its kick weights
+//    sum to 0.946 instead of 1; the oracle uses d3 there. The other nine
+//    sequences are followed as written.
 //
 // Parity status of each function is stated in its comment: "pinned by" names
 // the independent checks in tests/ that would catch a plausible mistake.
+
+#include "../aster_amd/csrc/saba_coeffs.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -539,6 +545,69 @@ int orc_integrate(int64_t n, double* pos, double* vel, const double* mass, doubl
     }
   }
   if (acc_out) std::memcpy(acc_out, acc.data(), 3 * n * sizeof(double));
+  return 0;
+}
+
+// update_euler (src/physics/tool_chain_impl.cpp:158-167) for order 1, which
+// is what update_SABA1 calls (SABA.cpp:14-16), and update_SABA2..10
+// (SABA.cpp:19-553) for orders 2..10: drift c_1, then per stage one force
+// evaluation, a kick v += (a d_k) dt and a drift x += (v c_(k+1)) dt. The
+// drifts/kicks are the library's SABAn table (Gauss-Legendre nodes and
+// weights, with the reference's doubles where its literals differ).
+// Pinned by: the table against numpy's Gauss-Legendre rule and the quadrature
+// exactness conditions; order 1 / order 2 convergence on a Kepler orbit; the
+// order-2n error of SABAn on a weakly interacting fly-by; time reversibility.
+int orc_integrate_saba(int64_t n, double* pos, double* vel, const double* mass, double G, double c, double soft,
+                       int pn, int method, const double* box, double theta, int order, double dt, int64_t nsteps,
+                       double* acc_out, int nthreads) {
+  if (order < 1 || order > aster::kSabaMaxOrder) return -1;
+  std::vector<double> acc(3 * n, 0.0);
+  Law L{G, c, soft, pn};
+  auto forces = [&]() {
+    if (method == 0)
+      allpairs(n, pos, vel, mass, L, 0, n, acc.data(), nthreads);
+    else
+      orc_bh(n, pos, vel, mass, box, theta, G, c, soft, pn, acc.data(), nullptr, nthreads);
+  };
+  // one stage: v += a * d * dt; x += v * cdrift * dt (the reference's lambdas)
+  auto kick_drift = [&](double d, double cdrift) {
+    parallel_for(n, nthreads, [&](int64_t i) {
+      for (int k = 0; k < 3; ++k) {
+        vel[3 * i + k] += acc[3 * i + k] * d * dt;
+        pos[3 * i + k] += vel[3 * i + k] * cdrift * dt;
+      }
+    });
+  };
+  for (int64_t s = 0; s < nsteps; ++s) {
+    if (order == 1) {
+      forces();
+      parallel_for(n, nthreads, [&](int64_t i) {
+        for (int k = 0; k < 3; ++k) {
+          vel[3 * i + k] += acc[3 * i + k] * dt;
+          pos[3 * i + k] += vel[3 * i + k] * dt;
+        }
+      });
+      continue;
+    }
+    const double* cs = aster::kSabaC[order];
+    const double* ds = aster::kSabaD[order];
+    parallel_for(n, nthreads, [&](int64_t i) {
+      for (int k = 0; k < 3; ++k) pos[3 * i + k] += vel[3 * i + k] * cs[0] * dt;
+    });
+    for (int st = 0; st < order; ++st) {
+      forces();
+      kick_drift(ds[st], cs[st + 1]);
+    }
+  }
+  if (acc_out) std::memcpy(acc_out, acc.data(), 3 * n * sizeof(double));
+  return 0;
+}
+
+// The SABAn drift (order + 1) and kick (order) coefficients the oracle uses.
+int orc_saba_coeffs(int order, double* c_out, double* d_out) {
+  if (order < 2 || order > aster::kSabaMaxOrder) return -1;
+  for (int i = 0; i <= order; ++i) c_out[i] = aster::kSabaC[order][i];
+  for (int i = 0; i < order; ++i) d_out[i] = aster::kSabaD[order][i];
   return 0;
 }
 

@@ -35,6 +35,9 @@ _SIG = {
     "orc_yoshida_weights": [_D],
     "orc_integrate": [_I64, _D, _D, _D, _dbl, _dbl, _dbl, ctypes.c_int, ctypes.c_int, _D, _dbl, ctypes.c_int,
                       _dbl, _I64, _D, ctypes.c_int],
+    "orc_integrate_saba": [_I64, _D, _D, _D, _dbl, _dbl, _dbl, ctypes.c_int, ctypes.c_int, _D, _dbl, ctypes.c_int,
+                           _dbl, _I64, _D, ctypes.c_int],
+    "orc_saba_coeffs": [ctypes.c_int, _D, _D],
     "orc_traj_grad": [_I64, _D, _D, _D, _dbl, _dbl, _dbl, _I64, ctypes.c_int, _PI64, _D, _D, _D, _D, _D],
     "orc_traj_loss": [_I64, _D, _D, _D, _dbl, _dbl, _dbl, _I64, ctypes.c_int, _PI64, _D, _D],
     "orc_allpairs_sample": [_I64, _D, _D, _D, _dbl, _dbl, _dbl, ctypes.c_int, _I64, _I64, ctypes.c_int, _D, _D],
@@ -181,16 +184,30 @@ def yoshida_weights():
 
 
 def integrate(pos, vel, mass, dt, nsteps, integrator=2, method=0, G=1.0, c=1e4, soft=0.0, pn=1, box=None,
-              theta=0.5, threads=0):
+              theta=0.5, threads=0, order=0):
+    """integrator: 0 Euler, 1 SABA (order 1..10), 2 leapfrog, 5 Yoshida-4."""
     pos = _c(pos).copy()
     vel = _c(vel).copy()
     mass = _c(mass)
     acc = np.empty_like(pos)
     boxa = _c(box if box is not None else np.zeros(3))
-    r = lib().orc_integrate(len(mass), _dp(pos), _dp(vel), _dp(mass), G, c, soft, pn, method, _dp(boxa), theta,
-                            integrator, dt, nsteps, _dp(acc), threads)
+    if integrator in (0, 1):
+        order = 1 if integrator == 0 else order
+        r = lib().orc_integrate_saba(len(mass), _dp(pos), _dp(vel), _dp(mass), G, c, soft, pn, method, _dp(boxa),
+                                     theta, order, dt, nsteps, _dp(acc), threads)
+    else:
+        r = lib().orc_integrate(len(mass), _dp(pos), _dp(vel), _dp(mass), G, c, soft, pn, method, _dp(boxa), theta,
+                                integrator, dt, nsteps, _dp(acc), threads)
     assert r == 0
     return pos, vel, acc
+
+
+def saba_coeffs(order):
+    """(drifts c, order + 1 entries; kicks d, order entries) of SABAn, n >= 2."""
+    cs = np.empty(order + 1)
+    ds = np.empty(order)
+    assert lib().orc_saba_coeffs(order, _dp(cs), _dp(ds)) == 0
+    return cs, ds
 
 
 def traj_grad(pos, vel, mass, dt, nsteps, snap_steps, target, G=1.0, soft=0.0):

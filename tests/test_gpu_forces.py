@@ -133,6 +133,58 @@ def test_pn1_trajectory_bit_identical(ctx, orc, integrator, oi):
     assert np.array_equal(p, rp) and np.array_equal(v, rv)
 
 
+@pytest.mark.parametrize("order", range(1, 11))
+def test_saba_trajectory_bit_identical(ctx, orc, order):
+    # update_SABAn (order 1: update_euler) with FP64 all-pairs, Newton + 1PN:
+    # n force evaluations per step, each after a drift, so velocity-dependent
+    # forces need no special handling.
+    pos, vel, mass = plummer(384, seed=20 + order)
+    vel = vel * 20.0
+    ctx.set_params(G=1.0, c=40.0, softening=1e-4, pn_flags=A.NEWTON | A.PN1)
+    ctx.set_state(pos, vel, mass)
+    ctx.integrate(A.ALLPAIRS_FP64, A.SABA, order, 1e-3, 3)
+    p, v = ctx.get_state()
+    rp, rv, ra = orc.integrate(pos, vel, mass, 1e-3, 3, integrator=1, order=order, c=40.0, soft=1e-4, pn=0x11)
+    assert np.array_equal(p, rp) and np.array_equal(v, rv)
+    assert np.array_equal(ctx.get_acc(), ra)  # accelerations of the last evaluation
+
+
+def test_euler_and_saba_then_leapfrog(ctx, orc):
+    # ASTER_EULER through aster_step equals SABA order 1; a leapfrog step after
+    # a SABA call re-evaluates forces (the cached ones are from before the
+    # last drift).
+    pos, vel, mass = plummer(256, seed=31)
+    ctx.set_params(G=1.0, softening=1e-4)
+    ctx.set_state(pos, vel, mass)
+    ctx.step(A.ALLPAIRS_FP64, A.EULER, DT_PARITY, 4)
+    p, v = ctx.get_state()
+    rp, rv, _ = orc.integrate(pos, vel, mass, DT_PARITY, 4, integrator=0, soft=1e-4)
+    assert np.array_equal(p, rp) and np.array_equal(v, rv)
+    ctx.integrate(A.ALLPAIRS_FP64, A.SABA, 4, DT_PARITY, 2)
+    ctx.step(A.ALLPAIRS_FP64, A.LEAPFROG, DT_PARITY, 2)
+    rp, rv, _ = orc.integrate(rp, rv, mass, DT_PARITY, 2, integrator=1, order=4, soft=1e-4)
+    rp, rv, _ = orc.integrate(rp, rv, mass, DT_PARITY, 2, integrator=2, soft=1e-4)
+    p, v = ctx.get_state()
+    assert np.array_equal(p, rp) and np.array_equal(v, rv)
+    with pytest.raises(A.AsterError):
+        ctx.step(A.ALLPAIRS_FP64, A.SABA, DT_PARITY, 1)  # needs an order
+    with pytest.raises(A.AsterError):
+        ctx.integrate(A.ALLPAIRS_FP64, A.SABA, 11, DT_PARITY, 1)
+
+
+def test_saba_barnes_hut_bit_identical(ctx, orc):
+    # fixed Morton box, so the box does not move with the drifts
+    pos, vel, mass = A.ic_mixed(2048, 33)
+    box = np.array([20.0, 20.0, 20.0])
+    ctx.set_params(G=1.0, softening=2.5e-5, theta=0.5, precision=A.FP64, box=tuple(box))
+    ctx.set_state(pos, vel, mass)
+    ctx.integrate(A.BH_PARITY, A.SABA, 3, 5e-4, 2)
+    p, v = ctx.get_state()
+    rp, rv, _ = orc.integrate(pos, vel, mass, 5e-4, 2, integrator=1, order=3, method=1, soft=2.5e-5, box=box,
+                              theta=0.5)
+    assert np.array_equal(p, rp) and np.array_equal(v, rv)
+
+
 def test_mixed_precision_trajectory(ctx, orc):
     # FP32 forces + FP64 state vs the all-FP64 path over 50 Yoshida steps.
     pos, vel, mass = plummer(4096, seed=10, mass_mode=1)

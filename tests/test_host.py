@@ -33,6 +33,10 @@ def test_version_and_weights(orc):
     w = aster_amd.yoshida_weights()
     assert w[0] == w[2] and 2 * w[0] + w[1] == pytest.approx(1.0, abs=1e-15)
     assert np.array_equal(w, orc.yoshida_weights())  # same bits as the oracle
+    for n in range(2, 11):
+        cs, ds = aster_amd.saba_coeffs(n)
+        ocs, ods = orc.saba_coeffs(n)
+        assert np.array_equal(cs, ocs) and np.array_equal(ds, ods), n
 
 
 def test_params_struct_layout():
@@ -118,6 +122,10 @@ def test_invalid_arguments_rejected_without_device():
     assert L.aster_ic_ensemble(10, 1, 3, 65, None, None, None, None, None) == -1
     assert L.aster_create(None, 0) == -1
     assert L.aster_yoshida_weights(None) == -1
+    c11, d11 = np.empty(12), np.empty(11)
+    for order in (0, 1, 11):  # SABA tables exist for orders 2..10
+        assert L.aster_saba_coeffs(order, c11.ctypes.data_as(aster_amd._D), d11.ctypes.data_as(aster_amd._D)) == -1
+    assert L.aster_integrate(None, 0, aster_amd.SABA, 4, 1e-3, 1) == -1
     assert L.aster_destroy(None) == 0
     assert L.aster_last_error(None)  # message for failed creation
 

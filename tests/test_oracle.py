@@ -360,6 +360,80 @@ def test_integrator_time_reversible(orc):
         np.testing.assert_allclose(-v2, vel, atol=1e-11)
 
 
+# SABAn (Laskar & Robutel 2001): kick times are the n-point Gauss-Legendre
+# nodes on [0, 1], kick weights its weights. numpy's rule is computed
+# independently of the table's generator.
+@pytest.mark.parametrize("n", range(2, 11))
+def test_saba_coefficients_are_gauss_legendre(orc, n):
+    cs, ds = orc.saba_coeffs(n)
+    x, w = np.polynomial.legendre.leggauss(n)
+    nodes = np.cumsum(cs[:n])
+    np.testing.assert_allclose(nodes, (x + 1) / 2, rtol=0, atol=6e-16)
+    np.testing.assert_allclose(ds, w / 2, rtol=0, atol=6e-16)
+    assert np.array_equal(cs, cs[::-1]) and np.array_equal(ds, ds[::-1])  # symmetric sequence
+    assert cs.sum() == pytest.approx(1.0, abs=4e-16) and ds.sum() == pytest.approx(1.0, abs=4e-16)
+    # Gauss quadrature is exact for polynomials of degree < 2n
+    for k in range(2 * n):
+        assert (ds * nodes ** k).sum() == pytest.approx(1.0 / (k + 1), abs=2e-15), k
+
+
+@pytest.mark.parametrize("order", [2, 3, 4, 6, 10])
+def test_saba_second_order_on_kepler(orc, order):
+    # Kinetic/potential splitting of a Kepler orbit is not a small
+    # perturbation, so every SABAn is second order there.
+    pos, vel, mass = kepler_ic()
+    T = 2.0
+    ref, _, _ = orc.integrate(pos, vel, mass, T / 8192, 8192, integrator=1, order=order)
+    errs = [np.abs(orc.integrate(pos, vel, mass, T / k, k, integrator=1, order=order)[0] - ref).max()
+            for k in (32, 64, 128)]
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(np.abs(rates - 2.0) < 0.1), rates
+
+
+def test_euler_first_order(orc):
+    # update_euler, which update_SABA1 calls: symplectic Euler, first order.
+    pos, vel, mass = kepler_ic()
+    T = 2.0
+    ref, _, _ = orc.integrate(pos, vel, mass, T / 65536, 65536, integrator=0)
+    errs = [np.abs(orc.integrate(pos, vel, mass, T / k, k, integrator=0)[0] - ref).max() for k in (1024, 2048, 4096)]
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(np.abs(rates - 1.0) < 0.1), rates
+    p1, _, _ = orc.integrate(pos, vel, mass, 1e-3, 7, integrator=1, order=1)
+    p0, _, _ = orc.integrate(pos, vel, mass, 1e-3, 7, integrator=0)
+    assert np.array_equal(p0, p1)
+
+
+def test_saba_high_order_on_weak_flyby(orc):
+    # H = T + eps V with eps = 1e-6: the SABAn error is O(eps tau^2n + eps^2
+    # tau^2), so SABA2/3/4 show orders 4/6/8 until FP64 roundoff, and SABA6
+    # is at roundoff with two steps. A wrong kick or drift weight (e.g. a
+    # kick sequence that does not sum to 1) breaks this at O(eps tau).
+    eps = 1e-6
+    pos = np.array([[-1.0, -0.5, 0.0], [1.0, 0.5, 0.0]])
+    vel = np.array([[1.0, 0.0, 0.0], [-1.0, 0.0, 0.0]])
+    mass = np.array([eps, eps])
+    T = 1.0
+    ref, _, _ = orc.integrate(pos, vel, mass, T / 4096, 4096, integrator=1, order=10)
+
+    def err(order, k):
+        return np.abs(orc.integrate(pos, vel, mass, T / k, k, integrator=1, order=order)[0] - ref).max()
+
+    r2 = np.log2([err(2, k) / err(2, 2 * k) for k in (2, 4, 8)])
+    assert np.all((r2 > 3.8) & (r2 < 4.5)), r2
+    assert np.log2(err(3, 2) / err(3, 4)) > 5.5
+    assert np.log2(err(4, 2) / err(4, 4)) > 7.0
+    assert err(6, 2) < 1e-13
+
+
+def test_saba_time_reversible(orc):
+    pos, vel, mass = kepler_ic()
+    for order in (2, 5, 6, 10):
+        p, v, _ = orc.integrate(pos, vel, mass, 0.01, 100, integrator=1, order=order)
+        p2, v2, _ = orc.integrate(p, -v, mass, 0.01, 100, integrator=1, order=order)
+        np.testing.assert_allclose(p2, pos, atol=1e-11)
+        np.testing.assert_allclose(-v2, vel, atol=1e-11)
+
+
 def test_yoshida_weights(orc):
     w = orc.yoshida_weights()
     w1 = 1.0 / (2.0 - 2.0 ** (1.0 / 3.0))
