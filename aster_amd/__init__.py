@@ -36,6 +36,7 @@ FP64 = 1
 EULER = 0
 SABA = 1
 LEAPFROG = 2
+WH_PLANETARY = 3  # batched ensembles only
 YOSHIDA4 = 5
 # Force-law flags (basic.h:39-40).
 NEWTON = 0x1

@@ -250,6 +250,243 @@ __global__ void k_batch_sum(int64_t nsys, const int64_t* __restrict__ off, int64
 
 // ----------------------------------------------------------------- host ----
 
+// ---- Wisdom-Holman for central-mass systems ------------------------------
+//
+// Democratic-heliocentric Wisdom-Holman (Wisdom & Holman 1991; Duncan, Levison
+// & Lee 1998). The reference's update_WH_planetary (tool_chain_impl.cpp:
+// 235-274) drifts each body i >= 1 on a Kepler orbit about body 0 and then
+// kicks with the full accelerations, which include body 0's pull again; this
+// is the splitting that scheme is meant to be: H = H_Kepler(mu = G m_0) +
+// H_interaction(planet pairs) + |sum m_i u_i|^2 / (2 m_0) in heliocentric
+// positions Q_i = x_i - x_0 and barycentric velocities u_i. One step: half
+// interaction kick, half jump Q_i += (sum m_j u_j / m_0) dt/2, Kepler drift,
+// half jump, half kick. Body 0 of each system is the central mass. During a
+// call the batch buffers hold the heliocentric state (planets: Q, u, and
+// their interaction accelerations; body 0: the barycentre and its velocity),
+// so launches can split the steps anywhere without changing a bit. Segment
+// sums are taken by the head lane in body order, and pair forces are
+// compute_gravitational (tool_chain_impl.cpp:120-137) in j order, Newtonian,
+// so every system is bit-identical to the CPU restatement.
+enum { WH_ENTER = 0, WH_STEPS = 1, WH_EXIT = 2 };
+
+struct WhArgs {
+  int nwaves;
+  const int32_t* lane_body;
+  const int32_t* lane_seg;
+  double* x;
+  double* v;
+  double* a;
+  const double* m;
+  int64_t nsteps;
+  double dt, G, soft;
+  int phase;
+};
+
+__global__ __launch_bounds__(256) void k_batch_wh(WhArgs P) {
+#pragma clang fp contract(off)
+  __shared__ double sx[kBatchMaxWpb][64], sy[kBatchMaxWpb][64], sz[kBatchMaxWpb][64], sm[kBatchMaxWpb][64];
+  __shared__ double tx[kBatchMaxWpb][64], ty[kBatchMaxWpb][64], tz[kBatchMaxWpb][64];
+  const int wl = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * (blockDim.x >> 6) + wl;
+  if (wave >= P.nwaves) return;
+  const int slot = wave * 64 + lane;
+  const int b = P.lane_body[slot];
+  const int seg = P.lane_seg[slot];
+  const int base = seg >> 8, cnt = seg & 255;
+  const bool live = b >= 0;
+  const bool head = live && lane == base;   // the system's central body
+  const bool planet = live && lane != base;
+  double* lx = sx[wl];
+  double* ly = sy[wl];
+  double* lz = sz[wl];
+  double* lm = sm[wl];
+  double* ux = tx[wl];
+  double* uy = ty[wl];
+  double* uz = tz[wl];
+  double X[3] = {0.0, 0.0, 0.0}, V[3] = {0.0, 0.0, 0.0}, A[3] = {0.0, 0.0, 0.0};
+  double mi = 0.0;
+  if (live) {
+    mi = P.m[b];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      X[c] = P.x[3 * (size_t)b + c];
+      V[c] = P.v[3 * (size_t)b + c];
+      A[c] = P.a[3 * (size_t)b + c];
+    }
+  }
+  lm[lane] = mi;
+  wave_sync();
+  const double m0 = live ? lm[base] : 1.0;
+  const double mu = P.G * m0;
+  const double h = 0.5 * P.dt;
+
+  // interaction accelerations of the planets at the positions in X
+  auto interaction = [&]() {
+    lx[lane] = X[0];
+    ly[lane] = X[1];
+    lz[lane] = X[2];
+    wave_sync();
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    if (planet)
+      for (int j = 1; j < cnt; ++j) {
+        const int src = base + j;
+        if (src == lane) continue;
+        double fx, fy, fz;
+        grav64(mi, lm[src], 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, X[0], X[1], X[2], lx[src], ly[src], lz[src], P.G, 0.0,
+               P.soft, false, fx, fy, fz);
+        ax += fx / mi;
+        ay += fy / mi;
+        az += fz / mi;
+      }
+    A[0] = ax;
+    A[1] = ay;
+    A[2] = az;
+    wave_sync();
+  };
+  // sum over the system's planets of m_j w_j (j order), by the head lane
+  auto planet_sum = [&](const double* W, double* out) {
+    ux[lane] = planet ? mi * W[0] : 0.0;
+    uy[lane] = planet ? mi * W[1] : 0.0;
+    uz[lane] = planet ? mi * W[2] : 0.0;
+    wave_sync();
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    if (head)
+      for (int j = 1; j < cnt; ++j) {
+        s0 += ux[base + j];
+        s1 += uy[base + j];
+        s2 += uz[base + j];
+      }
+    wave_sync();
+    if (head) {
+      ux[lane] = s0;
+      uy[lane] = s1;
+      uz[lane] = s2;
+    }
+    wave_sync();
+    if (live) {
+      out[0] = ux[base];
+      out[1] = uy[base];
+      out[2] = uz[base];
+    }
+    wave_sync();
+  };
+  auto jump = [&]() {
+    double S[3] = {0.0, 0.0, 0.0};
+    planet_sum(V, S);
+    if (planet)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) X[c] = X[c] + S[c] / m0 * h;
+  };
+
+  if (P.phase == WH_ENTER) {
+    // M, barycentre and its velocity over all bodies (j order), then
+    // Q = x - x_0, u = v - V for the planets; body 0 keeps (X, V).
+    lx[lane] = mi * X[0];
+    ly[lane] = mi * X[1];
+    lz[lane] = mi * X[2];
+    ux[lane] = mi * V[0];
+    uy[lane] = mi * V[1];
+    uz[lane] = mi * V[2];
+    wave_sync();
+    double M = 0.0, Sx[3] = {0.0, 0.0, 0.0}, Sv[3] = {0.0, 0.0, 0.0};
+    if (head)
+      for (int j = 0; j < cnt; ++j) {
+        M += lm[base + j];
+        Sx[0] += lx[base + j];
+        Sx[1] += ly[base + j];
+        Sx[2] += lz[base + j];
+        Sv[0] += ux[base + j];
+        Sv[1] += uy[base + j];
+        Sv[2] += uz[base + j];
+      }
+    wave_sync();
+    if (head) {
+      // publish x_0 (this lane's own position) and the barycentre velocity
+      lx[lane] = X[0];
+      ly[lane] = X[1];
+      lz[lane] = X[2];
+      ux[lane] = Sv[0] / M;
+      uy[lane] = Sv[1] / M;
+      uz[lane] = Sv[2] / M;
+    }
+    wave_sync();
+    if (planet) {
+      const double x0[3] = {lx[base], ly[base], lz[base]};
+      const double vc[3] = {ux[base], uy[base], uz[base]};
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        X[c] = X[c] - x0[c];
+        V[c] = V[c] - vc[c];
+      }
+    }
+    if (head)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        X[c] = Sx[c] / M;
+        V[c] = Sv[c] / M;
+      }
+    wave_sync();
+    interaction();
+  } else if (P.phase == WH_STEPS) {
+    for (int64_t s = 0; s < P.nsteps; ++s) {
+      if (planet)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) V[c] = V[c] + A[c] * h;
+      jump();
+      if (planet) kepler_drift64(X, V, mu, P.dt);
+      jump();
+      interaction();
+      if (planet)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) V[c] = V[c] + A[c] * h;
+      if (head)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) X[c] = X[c] + V[c] * P.dt;
+    }
+  } else {
+    // x_0 = X - sum m_j Q_j / M, x_i = Q_i + x_0, v_0 = V - sum m_j u_j / m_0,
+    // v_i = u_i + V
+    double SQ[3] = {0.0, 0.0, 0.0}, SU[3] = {0.0, 0.0, 0.0};
+    planet_sum(X, SQ);
+    planet_sum(V, SU);
+    if (head) {
+      double M = 0.0;
+      for (int j = 0; j < cnt; ++j) M += lm[base + j];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) X[c] = X[c] - SQ[c] / M;
+      lx[lane] = X[0];
+      ly[lane] = X[1];
+      lz[lane] = X[2];
+      ux[lane] = V[0];
+      uy[lane] = V[1];
+      uz[lane] = V[2];
+    }
+    wave_sync();
+    if (planet) {
+      const double x0[3] = {lx[base], ly[base], lz[base]};
+      const double vb[3] = {ux[base], uy[base], uz[base]};
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        X[c] = X[c] + x0[c];
+        V[c] = V[c] + vb[c];
+      }
+    }
+    if (head)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) V[c] = V[c] - SU[c] / m0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) A[c] = 0.0;
+  }
+  if (live) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      P.x[3 * (size_t)b + c] = X[c];
+      P.v[3 * (size_t)b + c] = V[c];
+      P.a[3 * (size_t)b + c] = A[c];
+    }
+  }
+}
+
 static int upload_plan(aster_ctx* c) {
   BatchState& B = c->batch;
   const int64_t nsys = B.nsys;
@@ -329,6 +566,27 @@ static int launch_batch(aster_ctx* c, const BatchArgs& P) {
   return ASTER_OK;
 }
 
+static int launch_wh(aster_ctx* c, int phase, double dt, int64_t nsteps) {
+  BatchState& B = c->batch;
+  WhArgs P;
+  P.nwaves = B.nwaves;
+  P.lane_body = B.lane_body.p;
+  P.lane_seg = B.lane_seg.p;
+  P.x = B.x.p;
+  P.v = B.v.p;
+  P.a = B.a.p;
+  P.m = B.m.p;
+  P.nsteps = nsteps;
+  P.dt = dt;
+  P.G = c->P.G;
+  P.soft = c->P.softening;
+  P.phase = phase;
+  const int wpb = B.waves_per_block;
+  hipLaunchKernelGGL(k_batch_wh, dim3((P.nwaves + wpb - 1) / wpb), dim3(64 * wpb), 0, c->stream, P);
+  ASTER_HIP(c, hipGetLastError());
+  return ASTER_OK;
+}
+
 static int substeps(int integrator, double dt, double* h) {
   if (integrator == ASTER_LEAPFROG) {
     h[0] = dt;
@@ -344,37 +602,44 @@ static int substeps(int integrator, double dt, double* h) {
 int batch_step(aster_ctx* c, int integrator, double dt, int64_t nsteps, int spl, int use_graph) {
   BatchState& B = c->batch;
   if (B.nsys == 0) return fail(c, ASTER_ERR_STATE, "no batch loaded");
-  if (integrator != ASTER_LEAPFROG && integrator != ASTER_YOSHIDA4)
-    return fail(c, ASTER_ERR_INVALID, "integrator must be ASTER_LEAPFROG or ASTER_YOSHIDA4");
+  const bool wh = integrator == ASTER_WH_PLANETARY;
+  if (integrator != ASTER_LEAPFROG && integrator != ASTER_YOSHIDA4 && !wh)
+    return fail(c, ASTER_ERR_INVALID, "integrator must be ASTER_LEAPFROG, ASTER_YOSHIDA4 or ASTER_WH_PLANETARY");
   if (!std::isfinite(dt) || nsteps < 0 || spl < 0) return fail(c, ASTER_ERR_INVALID, "invalid dt, nsteps or spl");
-  double h[3];
-  const int nsub = substeps(integrator, dt, h);
+  const bool pn = (c->P.pn_flags & ASTER_PN1) != 0;
+  if (wh && pn) return fail(c, ASTER_ERR_INVALID, "ASTER_WH_PLANETARY is Newtonian: ASTER_PN1 is not supported");
+  double h[3] = {0.0, 0.0, 0.0};
+  const int nsub = wh ? 1 : substeps(integrator, dt, h);
   const double h0[3] = {0.0, 0.0, 0.0};
   int rc;
-  if (!B.a_valid) {
+  if (!wh && !B.a_valid) {
     if ((rc = launch_batch(c, make_args(c, 0, 1, h0, 1)))) return rc;
     B.a_valid = true;
   }
   if (nsteps == 0) return ASTER_OK;
   if (spl == 0) spl = B.steps_per_launch > 0 ? B.steps_per_launch : 256;
   const int64_t full = nsteps / spl, rem = nsteps % spl;
-  const bool pn = (c->P.pn_flags & ASTER_PN1) != 0;
   const int exact = c->P.precision == ASTER_FP64;
-  if (!use_graph) {
-    for (int64_t l = 0; l < full; ++l)
-      if ((rc = launch_batch(c, make_args(c, spl, nsub, h, 0)))) return rc;
-    if (rem && (rc = launch_batch(c, make_args(c, rem, nsub, h, 0)))) return rc;
-    return ASTER_OK;
-  }
+  // every launch of this call; Wisdom-Holman converts to heliocentric
+  // coordinates before and back after (the accelerations are then stale)
+  auto enqueue = [&]() -> int {
+    int e = ASTER_OK;
+    if (wh && (e = launch_wh(c, WH_ENTER, dt, 0))) return e;
+    for (int64_t l = 0; l < full && !e; ++l)
+      e = wh ? launch_wh(c, WH_STEPS, dt, spl) : launch_batch(c, make_args(c, spl, nsub, h, 0));
+    if (!e && rem) e = wh ? launch_wh(c, WH_STEPS, dt, rem) : launch_batch(c, make_args(c, rem, nsub, h, 0));
+    if (!e && wh) e = launch_wh(c, WH_EXIT, dt, 0);
+    return e;
+  };
+  if (wh) B.a_valid = false;
+  if (!use_graph) return enqueue();
   const bool same = B.exec && B.g_steps == nsteps && B.g_integrator == integrator && B.g_spl == spl &&
                     B.g_dt == dt && B.g_pn == (int)pn && B.g_exact == exact && B.g_wpb == B.waves_per_block &&
                     B.g_pack == B.pack && B.g_G == c->P.G && B.g_c == c->P.c && B.g_soft == c->P.softening;
   if (!same) {
     drop_graph(B);
     ASTER_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int lrc = ASTER_OK;
-    for (int64_t l = 0; l < full && !lrc; ++l) lrc = launch_batch(c, make_args(c, spl, nsub, h, 0));
-    if (!lrc && rem) lrc = launch_batch(c, make_args(c, rem, nsub, h, 0));
+    const int lrc = enqueue();
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (lrc) {
@@ -443,8 +708,9 @@ int batch_autotune(aster_ctx* c, int integrator, double dt, int32_t* spb, int32_
   ASTER_HIP(c, hipMemcpyAsync(save.p, B.x.p, nb3 * 8, hipMemcpyDeviceToDevice, c->stream));
   ASTER_HIP(c, hipMemcpyAsync(save.p + nb3, B.v.p, nb3 * 8, hipMemcpyDeviceToDevice, c->stream));
   ASTER_HIP(c, hipMemcpyAsync(save.p + 2 * nb3, B.a.p, nb3 * 8, hipMemcpyDeviceToDevice, c->stream));
-  double h[3];
-  const int nsub = substeps(integrator, dt, h);
+  const bool wh = integrator == ASTER_WH_PLANETARY;
+  double h[3] = {0.0, 0.0, 0.0};
+  const int nsub = wh ? 1 : substeps(integrator, dt, h);
   const int trial = 16;
   float best = 1e30f;
   int best_pack = B.pack, best_wpb = B.waves_per_block;
@@ -453,9 +719,12 @@ int batch_autotune(aster_ctx* c, int integrator, double dt, int32_t* spb, int32_
     if ((rc = upload_plan(c))) return rc;
     for (int wpb = 1; wpb <= kBatchMaxWpb; wpb *= 2) {
       B.waves_per_block = wpb;
-      if ((rc = launch_batch(c, make_args(c, 2, nsub, h, 0)))) return rc;  // warm
+      if (wh && (rc = launch_wh(c, WH_ENTER, dt, 0))) return rc;
+      rc = wh ? launch_wh(c, WH_STEPS, dt, 2) : launch_batch(c, make_args(c, 2, nsub, h, 0));  // warm
+      if (rc) return rc;
       ASTER_HIP(c, hipEventRecord(c->ev[14], c->stream));
-      if ((rc = launch_batch(c, make_args(c, trial, nsub, h, 0)))) return rc;
+      rc = wh ? launch_wh(c, WH_STEPS, dt, trial) : launch_batch(c, make_args(c, trial, nsub, h, 0));
+      if (rc) return rc;
       ASTER_HIP(c, hipEventRecord(c->ev[15], c->stream));
       ASTER_HIP(c, hipEventSynchronize(c->ev[15]));
       float ms = 0.f;

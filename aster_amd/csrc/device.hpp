@@ -84,6 +84,109 @@ __device__ __forceinline__ void kahan_add32(float& s, float& c, float t) {
 
 // Compiler and memory ordering point for LDS traffic between the lanes of one
 // wavefront (no cross-wave synchronisation).
+// Stumpff functions c0..c3 of z: quarter z until |z| <= 0.1, 8-term nested
+// series, then c3 = (c2 + c0 c3)/4, c2 = c1^2/2, c1 = c0 c1, c0 = 2 c0^2 - 1
+// once per quartering. Only +, -, *, / in a fixed order, so the result is
+// the same bits on any IEEE machine (contraction off).
+__device__ __forceinline__ void stumpff64(double z, double& c0, double& c1, double& c2, double& c3) {
+#pragma clang fp contract(off)
+  int nq = 0;
+  while (fabs(z) > 0.1) {
+    z = z * 0.25;
+    ++nq;
+  }
+  c2 = 0.5 * (1.0 - z / 12.0 * (1.0 - z / 30.0 * (1.0 - z / 56.0 * (1.0 - z / 90.0 *
+       (1.0 - z / 132.0 * (1.0 - z / 182.0 * (1.0 - z / 240.0 * (1.0 - z / 306.0))))))));
+  c3 = (1.0 / 6.0) * (1.0 - z / 20.0 * (1.0 - z / 42.0 * (1.0 - z / 72.0 * (1.0 - z / 110.0 *
+       (1.0 - z / 156.0 * (1.0 - z / 210.0 * (1.0 - z / 272.0 * (1.0 - z / 342.0))))))));
+  c1 = 1.0 - z * c3;
+  c0 = 1.0 - z * c2;
+  for (int k = 0; k < nq; ++k) {
+    c3 = (c2 + c0 * c3) * 0.25;
+    c2 = c1 * c1 * 0.5;
+    c1 = c0 * c1;
+    c0 = 2.0 * c0 * c0 - 1.0;
+  }
+}
+
+// Universal Kepler equation F(s) = r0 G1 + eta0 G2 + mu G3 - dt with its
+// first two derivatives (G_k = s^k c_k(beta s^2)).
+__device__ __forceinline__ void kepler_eval64(double r0, double eta0, double zeta0, double beta, double mu, double s,
+                                              double dt, double& F, double& Fp, double& Fpp) {
+#pragma clang fp contract(off)
+  double c0, c1, c2, c3;
+  stumpff64(beta * s * s, c0, c1, c2, c3);
+  const double g1 = s * c1, g2 = s * s * c2, g3 = s * s * s * c3;
+  F = r0 * g1 + eta0 * g2 + mu * g3 - dt;
+  Fp = r0 * c0 + eta0 * g1 + mu * g2;
+  Fpp = eta0 * c0 + zeta0 * g1;
+}
+
+// Kepler drift of relative position q and velocity u about mu for dt
+// (universal variables). F is increasing (F' = r > 0): on a bound orbit dt is
+// reduced modulo the period (fmod is exact) and the root lies within one
+// revolution, s in [0, 2 pi / sqrt(beta)]; otherwise the bracket is grown by
+// doubling. Halley steps from s = dt / r0, replaced by bisection whenever they
+// leave the bracket, at most 100, until |ds| <= 1e-15 |s|; then the f and g
+// functions. Every lane runs its own iteration count.
+__device__ inline void kepler_drift64(double* q, double* u, double mu, double dt) {
+#pragma clang fp contract(off)
+  const double r0 = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+  const double v2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  const double eta0 = q[0] * u[0] + q[1] * u[1] + q[2] * u[2];
+  const double beta = 2.0 * mu / r0 - v2;
+  const double zeta0 = mu - beta * r0;
+  const double sgn = dt < 0.0 ? -1.0 : 1.0;
+  double F, Fp, Fpp;
+  double lo = 0.0, hi;
+  if (beta > 0.0) {
+    const double sq = sqrt(beta);
+    dt = fmod(dt, 6.283185307179586 * mu / (beta * sq));
+    hi = sgn * (6.283185307179586 / sq);
+  } else {
+    hi = dt / r0;
+    for (int k = 0; k < 200; ++k) {
+      kepler_eval64(r0, eta0, zeta0, beta, mu, hi, dt, F, Fp, Fpp);
+      if (sgn * F >= 0.0) break;
+      lo = hi;
+      hi = 2.0 * hi;
+    }
+  }
+  double s = dt / r0;
+  if (!(sgn * (s - lo) > 0.0 && sgn * (hi - s) > 0.0)) s = 0.5 * (lo + hi);
+  for (int it = 0; it < 100; ++it) {
+    kepler_eval64(r0, eta0, zeta0, beta, mu, s, dt, F, Fp, Fpp);
+    if (sgn * F < 0.0)
+      lo = s;
+    else
+      hi = s;
+    double sn = s - F / (Fp - 0.5 * F * Fpp / Fp);
+    if (!(sgn * (sn - lo) > 0.0 && sgn * (hi - sn) > 0.0)) sn = 0.5 * (lo + hi);
+    const double ds = sn - s;
+    s = sn;
+    if (fabs(ds) <= 1e-15 * fabs(s)) break;
+  }
+  double c0, c1, c2, c3;
+  stumpff64(beta * s * s, c0, c1, c2, c3);
+  const double g1 = s * c1, g2 = s * s * c2;
+  const double r = r0 * c0 + eta0 * g1 + mu * g2;
+  const double f = 1.0 - mu * g2 / r0;
+  const double g = r0 * g1 + eta0 * g2;
+  const double fd = -mu * g1 / (r * r0);
+  const double gd = 1.0 - mu * g2 / r;
+  double qn[3], un[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    qn[k] = f * q[k] + g * u[k];
+    un[k] = fd * q[k] + gd * u[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    q[k] = qn[k];
+    u[k] = un[k];
+  }
+}
+
 // A value known to be equal in every active lane, moved to a scalar register.
 __device__ __forceinline__ float uniform_f32(float x) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));

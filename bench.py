@@ -30,6 +30,19 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0
 
 
+def traffic_per_launch(kernel):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, each scaled by
+    the same counter's reading for a copy of known size), as measured with
+    rocprofv3 --pmc and recorded in profiles/traffic.json; None if absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)["kernels"][kernel]
+        return rec["fetch_bytes"] + rec["write_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -131,7 +144,9 @@ def bench_allpairs(A, ctx, evals, n=65536):
             "kernel_ms": kernel_ms,
             "force_error_vs_fp64": errs,
             "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                         "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": "k_allpairs_f32",
+                         "frac": achieved / FP32_PEAK_TFLOPS,
+                         "traffic": traffic_per_launch("k_allpairs_f32_mixed" if vop == 15 else "k_allpairs_f32_plain"),
+                         "kernel": "k_allpairs_f32",
                          "flop_per_interaction": 20, "valu_instructions_per_interaction": vop + 1},
         }
     res = dict(out["fp32_mixed_kahan"])
@@ -139,6 +154,51 @@ def bench_allpairs(A, ctx, evals, n=65536):
     res["method"] = "ALLPAIRS_FP32_KAHAN (FP64 separations, FP32 pair force, Kahan FP32 sums)"
     res["fp32_plain"] = out["fp32"]
     return res, pos, vel, mass
+
+
+def bench_integrators(A, ctx, n=65536, dt=2.5e-4, steps=1000, steps64=20):
+    """Mixed-precision configuration: Plummer N=65536 (masses log-uniform over
+    two decades), eps=0.005, Yoshida-4 dt=2.5e-4 for 1000 steps with FP32 Kahan
+    pair forces and FP64 state: relative drift of the Plummer energy and of
+    |L|. The first `steps64` steps are repeated on the all-FP64 path (the
+    bit-exact reference path) to compare drifts and positions. Then one step
+    of each integrator of the family, timed with the mixed forces."""
+    pos, vel, mass = A.ic_plummer(n, 3, mass_mode=1)
+    soft = 0.005 ** 2
+    ctx.set_params(G=1.0, softening=soft)
+
+    def drift(method, nsteps):
+        ctx.set_state(pos, vel, mass)
+        _, e0, L0 = ctx.energy()
+        t0 = time.perf_counter()
+        ctx.step(method, A.YOSHIDA4, dt, nsteps)
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        _, e1, L1 = ctx.energy()
+        p, _ = ctx.get_state()
+        return dict(dE_rel=abs(e1 - e0) / abs(e0), dL_rel=float(np.linalg.norm(L1 - L0) / np.linalg.norm(L0)),
+                    ms_per_step=wall / nsteps * 1e3), p
+
+    mixed, _ = drift(A.ALLPAIRS_FP32_KAHAN, steps)
+    mixed_short, p32 = drift(A.ALLPAIRS_FP32_KAHAN, steps64)
+    fp64_short, p64 = drift(A.ALLPAIRS_FP64, steps64)
+    dx = float(np.abs(p32 - p64).max())
+    log(f"[bench] Yoshida-4 n={n}: mixed {steps} steps {mixed}; {steps64} steps mixed {mixed_short} "
+        f"fp64 {fp64_short}, max |dx| {dx:.2e}")
+    fam = {}
+    for name, integ, order in (("leapfrog", A.LEAPFROG, 0), ("yoshida4", A.YOSHIDA4, 0), ("saba4", A.SABA, 4),
+                               ("saba10", A.SABA, 10)):
+        ctx.set_state(pos, vel, mass)
+        ctx.integrate(A.ALLPAIRS_FP32_KAHAN, integ, order, dt, 1)  # warm
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.integrate(A.ALLPAIRS_FP32_KAHAN, integ, order, dt, 5)
+        ctx.sync()
+        fam[name] = (time.perf_counter() - t0) / 5 * 1e3
+    log(f"[bench] integrator family ms/step (mixed forces, n={n}): {fam}")
+    return {"n": n, "dt": dt, "integrator": "YOSHIDA4", "forces": "ALLPAIRS_FP32_KAHAN",
+            f"mixed_{steps}_steps": mixed, f"mixed_{steps64}_steps": mixed_short, f"fp64_{steps64}_steps": fp64_short,
+            f"max_abs_dx_mixed_vs_fp64_{steps64}_steps": dx, "ms_per_step_by_integrator": fam}
 
 
 def bench_adjoint(A, ctx, n=65536, T=512, ckpt=64, dt=1e-3):
@@ -234,7 +294,8 @@ def main():
     ap.add_argument("--method", default="BH_GROUP")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward ratio")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched ensemble")
-    ap.add_argument("--only", choices=["bh", "allpairs", "adjoint", "batch"],
+    ap.add_argument("--no-integrators", action="store_true", help="skip the integrator section")
+    ap.add_argument("--only", choices=["bh", "allpairs", "adjoint", "batch", "integrators"],
                     help="run one section only (profiling); no contract line")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -252,12 +313,15 @@ def main():
             r = bench_allpairs(A, ctx, max(args.steps, 10))[0]
         elif args.only == "adjoint":
             r = bench_adjoint(A, ctx)
+        elif args.only == "integrators":
+            r = bench_integrators(A, ctx)
         else:
             r = bench_batch(A, ctx)
         print(json.dumps({"section": args.only, "result": r}, default=float), flush=True)
         return
     bh = bench_bh(A, ctx, args.steps, args.warmup, method)
     ap_res, pos64k, vel64k, mass64k = bench_allpairs(A, ctx, max(args.steps, 10))
+    integ = None if args.no_integrators else bench_integrators(A, ctx)
     adj = None if args.no_adjoint else bench_adjoint(A, ctx)
     batch = None if args.no_batch else bench_batch(A, ctx)
 
@@ -297,7 +361,8 @@ def main():
                                "(tree rebuild + traversal + kick/drift) per step, dt=5e-4",
                    "n": bh["n"], "theta": bh["theta"], "method": args.method, "dt": bh["dt"]},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": "traversal",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic_per_launch("k_bh_group"),
+                     "kernel": "traversal",
                      "kernel_ms": trav_ms, "flops": "20 per interaction + 8 per node visit"},
         "cpu_baseline": cpu,
         "interactions_per_step": inter_per_eval,
@@ -306,6 +371,7 @@ def main():
         "phases_ms": phases,
         "event_ms_per_step": bh["event_ms"] / args.steps,
         "allpairs": ap_res,
+        "integrators": integ,
         "adjoint_forward_ratio": None if adj is None else adj["ratio"],
         "adjoint": adj,
         "batch": batch,

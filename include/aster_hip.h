@@ -74,8 +74,15 @@ enum aster_method {
 enum aster_precision { ASTER_FP32 = 0, ASTER_FP64 = 1 };
 
 /* Integrators: values follow the reference's update_type where one exists
- * (EULER = 0, SABA = 1, LEAPFROG = 2, include/Aster/simulations/basic.h:56). */
-enum aster_integrator { ASTER_EULER = 0, ASTER_SABA = 1, ASTER_LEAPFROG = 2, ASTER_YOSHIDA4 = 5 };
+ * (EULER = 0, SABA = 1, LEAPFROG = 2, WH_PLANETARY = 3,
+ * include/Aster/simulations/basic.h:56). */
+enum aster_integrator {
+  ASTER_EULER = 0,
+  ASTER_SABA = 1,
+  ASTER_LEAPFROG = 2,
+  ASTER_WH_PLANETARY = 3, /* batched ensembles only (aster_batch_step) */
+  ASTER_YOSHIDA4 = 5
+};
 
 typedef struct aster_params {
   double G;         /* gravitational constant (reference default 6e-11) */
@@ -216,7 +223,15 @@ int aster_batch_get(aster_ctx* ctx, double* pos, double* vel);
  * compute_gravitational in the reference's operation order (bit-identical to
  * the CPU path per system); ASTER_FP32 selects the fast FP64 path (rsqrt
  * estimate + Newton steps). Launches of steps_per_launch steps (0 = tuned) are
- * captured once in a HIP graph and replayed when use_graph != 0. */
+ * captured once in a HIP graph and replayed when use_graph != 0.
+ * integrator: ASTER_LEAPFROG, ASTER_YOSHIDA4, or ASTER_WH_PLANETARY, which
+ * replaces update_WH_planetary (src/physics/tool_chain_impl.cpp:235-274):
+ * body 0 of each system is the central mass; democratic-heliocentric
+ * Wisdom-Holman (half kick with the planet-planet forces, half jump, exact
+ * Kepler drift about G m_0, half jump, half kick), Newtonian, in the
+ * reference's pair-force order whatever the precision. The reference's
+ * version kicks with the full accelerations after the Kepler drift, counting
+ * the central pull twice; this one does not. */
 int aster_batch_step(aster_ctx* ctx, int integrator, double dt, int64_t nsteps, int32_t steps_per_launch,
                      int32_t use_graph);
 /* Per-system energy (kinetic + Plummer-softened potential) and angular

@@ -166,4 +166,17 @@ inline void update_yoshida4_hip(Solver& s, const Meta& meta, int64_t steps = 1, 
   if (read_back) s.download();
 }
 
+// update_SABA1..10 (src/integrators/SABA.cpp:19-553), picked by order as
+// bake_update_function(SABA, ord) does (src/utils/updater.cpp:144-152);
+// order 1 is update_euler (tool_chain_impl.cpp:158-167).
+inline void update_saba_hip(Solver& s, const Meta& meta, int order, int64_t steps = 1, bool read_back = true) {
+  check(s.context(), aster_integrate(s.context(), s.method(), ASTER_SABA, order, meta.dt, steps));
+  if (read_back) s.download();
+}
+
+inline void update_euler_hip(Solver& s, const Meta& meta, int64_t steps = 1, bool read_back = true) {
+  check(s.context(), aster_integrate(s.context(), s.method(), ASTER_EULER, 1, meta.dt, steps));
+  if (read_back) s.download();
+}
+
 }  // namespace aster_hip

@@ -603,6 +603,209 @@ int orc_integrate_saba(int64_t n, double* pos, double* vel, const double* mass, 
   return 0;
 }
 
+// ---- Wisdom-Holman (democratic heliocentric) for a central-mass system ----
+//
+// The reference's update_WH_planetary (tool_chain_impl.cpp:235-274) drifts
+// every body i >= 1 on a Kepler orbit about body 0 (update_kepler, :187-233)
+// and then kicks with the full solver accelerations, which contain body 0's
+// pull again, so the central force is applied twice per step. The oracle
+// restates the scheme that splitting is meant to be: Wisdom & Holman (1991) in
+// democratic heliocentric coordinates (Duncan, Levison & Lee 1998), with
+// heliocentric positions Q_i = x_i - x_0 and barycentric velocities u_i
+// (i >= 1) and H = H_Kepler(mu = G m_0) + H_interaction(planet pairs) +
+// |sum m_i u_i|^2 / (2 m_0). One step: half interaction kick, half "jump"
+// Q_i += (sum m_j u_j / m_0) dt/2, Kepler drift dt, half jump, half kick.
+// Newtonian pair forces (compute_gravitational, softened) between planets.
+
+// Stumpff functions c0..c3 of z: quarter z until |z| <= 0.1, series (8
+// nested terms), then c3 = (c2 + c0 c3)/4, c2 = c1^2/2, c1 = c0 c1,
+// c0 = 2 c0^2 - 1 once per quartering. Only +, -, *, / (bit-reproducible).
+static void stumpff(double z, double& c0, double& c1, double& c2, double& c3) {
+  int nq = 0;
+  while (std::fabs(z) > 0.1) {
+    z = z * 0.25;
+    ++nq;
+  }
+  c2 = 0.5 * (1.0 - z / 12.0 * (1.0 - z / 30.0 * (1.0 - z / 56.0 * (1.0 - z / 90.0 *
+       (1.0 - z / 132.0 * (1.0 - z / 182.0 * (1.0 - z / 240.0 * (1.0 - z / 306.0))))))));
+  c3 = (1.0 / 6.0) * (1.0 - z / 20.0 * (1.0 - z / 42.0 * (1.0 - z / 72.0 * (1.0 - z / 110.0 *
+       (1.0 - z / 156.0 * (1.0 - z / 210.0 * (1.0 - z / 272.0 * (1.0 - z / 342.0))))))));
+  c1 = 1.0 - z * c3;
+  c0 = 1.0 - z * c2;
+  for (int k = 0; k < nq; ++k) {
+    c3 = (c2 + c0 * c3) * 0.25;
+    c2 = c1 * c1 * 0.5;
+    c1 = c0 * c1;
+    c0 = 2.0 * c0 * c0 - 1.0;
+  }
+}
+
+// Kepler drift of (q, u) about mu for dt in universal variables (G-functions
+// G_k = s^k c_k(beta s^2)). The universal Kepler equation
+// F(s) = r0 G1 + eta0 G2 + mu G3 - dt is increasing (F' = r > 0), so the root
+// is bracketed: on a bound orbit dt is reduced modulo the period (fmod is
+// exact) and s lies in [0, 2 pi / sqrt(beta)] (one revolution), otherwise the
+// bracket is grown by doubling. Halley steps from s = dt / r0, replaced by
+// bisection whenever they leave the bracket, at most 100, until
+// |ds| <= 1e-15 |s|; then the f and g functions.
+struct KepState {
+  double r0, eta0, zeta0, beta, mu;
+};
+
+static void kepler_eval(const KepState& K, double s, double dt, double& F, double& Fp, double& Fpp) {
+  double c0, c1, c2, c3;
+  stumpff(K.beta * s * s, c0, c1, c2, c3);
+  const double g1 = s * c1, g2 = s * s * c2, g3 = s * s * s * c3;
+  F = K.r0 * g1 + K.eta0 * g2 + K.mu * g3 - dt;
+  Fp = K.r0 * c0 + K.eta0 * g1 + K.mu * g2;
+  Fpp = K.eta0 * c0 + K.zeta0 * g1;
+}
+
+static void kepler_drift(double* q, double* u, double mu, double dt) {
+  KepState K;
+  K.mu = mu;
+  K.r0 = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+  const double v2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  K.eta0 = q[0] * u[0] + q[1] * u[1] + q[2] * u[2];
+  K.beta = 2.0 * mu / K.r0 - v2;
+  K.zeta0 = mu - K.beta * K.r0;
+  const double sgn = dt < 0.0 ? -1.0 : 1.0;
+  double F, Fp, Fpp;
+  double lo = 0.0, hi;  // the root lies between lo and hi (in the direction of dt)
+  if (K.beta > 0.0) {
+    const double sq = std::sqrt(K.beta);
+    dt = std::fmod(dt, 6.283185307179586 * mu / (K.beta * sq));
+    hi = sgn * (6.283185307179586 / sq);
+  } else {
+    hi = dt / K.r0;
+    for (int k = 0; k < 200; ++k) {
+      kepler_eval(K, hi, dt, F, Fp, Fpp);
+      if (sgn * F >= 0.0) break;
+      lo = hi;
+      hi = 2.0 * hi;
+    }
+  }
+  double s = dt / K.r0;
+  if (!(sgn * (s - lo) > 0.0 && sgn * (hi - s) > 0.0)) s = 0.5 * (lo + hi);
+  for (int it = 0; it < 100; ++it) {
+    kepler_eval(K, s, dt, F, Fp, Fpp);
+    if (sgn * F < 0.0)
+      lo = s;
+    else
+      hi = s;
+    double sn = s - F / (Fp - 0.5 * F * Fpp / Fp);
+    if (!(sgn * (sn - lo) > 0.0 && sgn * (hi - sn) > 0.0)) sn = 0.5 * (lo + hi);
+    const double ds = sn - s;
+    s = sn;
+    if (std::fabs(ds) <= 1e-15 * std::fabs(s)) break;
+  }
+  double c0, c1, c2, c3;
+  stumpff(K.beta * s * s, c0, c1, c2, c3);
+  const double g1 = s * c1, g2 = s * s * c2;
+  const double r = K.r0 * c0 + K.eta0 * g1 + mu * g2;
+  const double f = 1.0 - mu * g2 / K.r0;
+  const double g = K.r0 * g1 + K.eta0 * g2;
+  const double fd = -mu * g1 / (r * K.r0);
+  const double gd = 1.0 - mu * g2 / r;
+  double qn[3], un[3];
+  for (int k = 0; k < 3; ++k) {
+    qn[k] = f * q[k] + g * u[k];
+    un[k] = fd * q[k] + gd * u[k];
+  }
+  for (int k = 0; k < 3; ++k) {
+    q[k] = qn[k];
+    u[k] = un[k];
+  }
+}
+
+int orc_kepler_drift(double* q, double* u, double mu, double dt) {
+  kepler_drift(q, u, mu, dt);
+  return 0;
+}
+
+// One system, body 0 central, n >= 1 (pos/vel AoS-d3, updated in place).
+// Conversions: M = sum m_j (j order); X = sum m_j x_j / M, V likewise;
+// Q_i = x_i - x_0, u_i = v_i - V. Back: x_0 = X - sum_{j>=1} m_j Q_j / M,
+// x_i = Q_i + x_0, v_0 = V - sum_{j>=1} m_j u_j / m_0, v_i = u_i + V; X
+// advances by V dt per step.
+int orc_integrate_wh(int64_t n, double* pos, double* vel, const double* mass, double G, double soft, double dt,
+                     int64_t nsteps) {
+  if (n < 1) return -1;
+  Law L{G, 0.0, soft, kNewton};
+  const double m0 = mass[0];
+  const double mu = G * m0;
+  double M = 0.0, X[3] = {0.0, 0.0, 0.0}, Vc[3] = {0.0, 0.0, 0.0};
+  for (int64_t j = 0; j < n; ++j) {
+    M += mass[j];
+    for (int k = 0; k < 3; ++k) {
+      X[k] += mass[j] * pos[3 * j + k];
+      Vc[k] += mass[j] * vel[3 * j + k];
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    X[k] = X[k] / M;
+    Vc[k] = Vc[k] / M;
+  }
+  std::vector<double> Q(3 * n, 0.0), U(3 * n, 0.0), A(3 * n, 0.0);
+  for (int64_t i = 1; i < n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      Q[3 * i + k] = pos[3 * i + k] - pos[k];
+      U[3 * i + k] = vel[3 * i + k] - Vc[k];
+    }
+  auto interaction = [&]() {
+    for (int64_t i = 1; i < n; ++i) {
+      V3 a{0.0, 0.0, 0.0};
+      const V3 zero{0.0, 0.0, 0.0};
+      for (int64_t j = 1; j < n; ++j) {
+        if (j == i) continue;
+        V3 f = gravitational(mass[i], mass[j], zero, zero, load3(Q.data(), i), load3(Q.data(), j), L);
+        a.x += f.x / mass[i];
+        a.y += f.y / mass[i];
+        a.z += f.z / mass[i];
+      }
+      store3(A.data(), i, a);
+    }
+  };
+  const double h = 0.5 * dt;
+  auto kick = [&]() {
+    for (int64_t i = 1; i < n; ++i)
+      for (int k = 0; k < 3; ++k) U[3 * i + k] = U[3 * i + k] + A[3 * i + k] * h;
+  };
+  auto jump = [&]() {
+    double P[3] = {0.0, 0.0, 0.0};
+    for (int64_t j = 1; j < n; ++j)
+      for (int k = 0; k < 3; ++k) P[k] += mass[j] * U[3 * j + k];
+    for (int64_t i = 1; i < n; ++i)
+      for (int k = 0; k < 3; ++k) Q[3 * i + k] = Q[3 * i + k] + P[k] / m0 * h;
+  };
+  interaction();
+  for (int64_t s = 0; s < nsteps; ++s) {
+    kick();
+    jump();
+    for (int64_t i = 1; i < n; ++i) kepler_drift(&Q[3 * i], &U[3 * i], mu, dt);
+    jump();
+    interaction();
+    kick();
+    for (int k = 0; k < 3; ++k) X[k] = X[k] + Vc[k] * dt;
+  }
+  double SQ[3] = {0.0, 0.0, 0.0}, SU[3] = {0.0, 0.0, 0.0};
+  for (int64_t j = 1; j < n; ++j)
+    for (int k = 0; k < 3; ++k) {
+      SQ[k] += mass[j] * Q[3 * j + k];
+      SU[k] += mass[j] * U[3 * j + k];
+    }
+  for (int k = 0; k < 3; ++k) {
+    pos[k] = X[k] - SQ[k] / M;
+    vel[k] = Vc[k] - SU[k] / m0;
+  }
+  for (int64_t i = 1; i < n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      pos[3 * i + k] = Q[3 * i + k] + pos[k];
+      vel[3 * i + k] = U[3 * i + k] + Vc[k];
+    }
+  return 0;
+}
+
 // The SABAn drift (order + 1) and kick (order) coefficients the oracle uses.
 int orc_saba_coeffs(int order, double* c_out, double* d_out) {
   if (order < 2 || order > aster::kSabaMaxOrder) return -1;

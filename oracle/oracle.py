@@ -38,6 +38,8 @@ _SIG = {
     "orc_integrate_saba": [_I64, _D, _D, _D, _dbl, _dbl, _dbl, ctypes.c_int, ctypes.c_int, _D, _dbl, ctypes.c_int,
                            _dbl, _I64, _D, ctypes.c_int],
     "orc_saba_coeffs": [ctypes.c_int, _D, _D],
+    "orc_kepler_drift": [_D, _D, _dbl, _dbl],
+    "orc_integrate_wh": [_I64, _D, _D, _D, _dbl, _dbl, _dbl, _I64],
     "orc_traj_grad": [_I64, _D, _D, _D, _dbl, _dbl, _dbl, _I64, ctypes.c_int, _PI64, _D, _D, _D, _D, _D],
     "orc_traj_loss": [_I64, _D, _D, _D, _dbl, _dbl, _dbl, _I64, ctypes.c_int, _PI64, _D, _D],
     "orc_allpairs_sample": [_I64, _D, _D, _D, _dbl, _dbl, _dbl, ctypes.c_int, _I64, _I64, ctypes.c_int, _D, _D],
@@ -200,6 +202,23 @@ def integrate(pos, vel, mass, dt, nsteps, integrator=2, method=0, G=1.0, c=1e4, 
                                 integrator, dt, nsteps, _dp(acc), threads)
     assert r == 0
     return pos, vel, acc
+
+
+def kepler_drift(q, u, mu, dt):
+    """Kepler drift of relative position q and velocity u about mu for dt."""
+    q = _c(q).copy()
+    u = _c(u).copy()
+    assert lib().orc_kepler_drift(_dp(q), _dp(u), mu, dt) == 0
+    return q, u
+
+
+def integrate_wh(pos, vel, mass, dt, nsteps, G=1.0, soft=0.0):
+    """Democratic-heliocentric Wisdom-Holman steps of one system (body 0 central)."""
+    pos = _c(pos).copy()
+    vel = _c(vel).copy()
+    mass = _c(mass)
+    assert lib().orc_integrate_wh(len(mass), _dp(pos), _dp(vel), _dp(mass), G, soft, dt, nsteps) == 0
+    return pos, vel
 
 
 def saba_coeffs(order):

@@ -16,6 +16,8 @@ int orc_bh(int64_t, const double*, const double*, const double*, const double*, 
            double*, int64_t*, int);
 int orc_integrate(int64_t, double*, double*, const double*, double, double, double, int, int, const double*, double,
                   int, double, int64_t, double*, int);
+int orc_integrate_saba(int64_t, double*, double*, const double*, double, double, double, int, int, const double*,
+                       double, int, double, int64_t, double*, int);
 }
 
 using namespace aster_hip;
@@ -66,6 +68,11 @@ int main() {
     orc_integrate(n, px.data(), vx.data(), b.masses.data(), 1.0, meta.c, 1e-4, ASTER_NEWTON, 0, box, 0.5, 2, meta.dt,
                   5, nullptr, 0);
     if (!same(b.positions, px) || !same(b.velocities, vx)) { std::printf("leapfrog differs\n"); ++fails; }
+
+    update_saba_hip(ap2, meta, 4, 2);
+    orc_integrate_saba(n, px.data(), vx.data(), b.masses.data(), 1.0, meta.c, 1e-4, ASTER_NEWTON, 0, box, 0.5, 4,
+                       meta.dt, 2, nullptr, 0);
+    if (!same(b.positions, px) || !same(b.velocities, vx)) { std::printf("SABA4 differs\n"); ++fails; }
   } catch (const std::exception& e) {
     std::printf("exception: %s\n", e.what());
     return 3;

@@ -61,6 +61,52 @@ def test_batch_graph_replays_consistently(ctx):
     assert np.array_equal(pg, pe) and np.array_equal(vg, ve)
 
 
+@pytest.mark.parametrize("precision", [A.FP64, A.FP32])
+def test_batch_wisdom_holman_bit_identical_to_oracle(ctx, orc, precision):
+    # Central-mass ensemble (body 0 of each system is the central mass):
+    # democratic-heliocentric Wisdom-Holman, Newtonian, bit-identical per
+    # system to the oracle whatever the precision flag, launch split or graph.
+    off, pos, vel, mass = ensemble(64, 6)
+    load(ctx, off, pos, vel, mass, pn_flags=A.NEWTON, precision=precision)
+    ctx.batch_step(A.WH_PLANETARY, 1e-3, 25, steps_per_launch=7, use_graph=False)
+    p, v = ctx.batch_get()
+    for s in range(64):
+        a, b = off[s], off[s + 1]
+        rp, rv = orc.integrate_wh(pos[a:b], vel[a:b], mass[a:b], 1e-3, 25, G=1.0, soft=0.0)
+        assert np.array_equal(p[a:b], rp) and np.array_equal(v[a:b], rv), s
+    for spl, graph in ((25, True), (3, True), (3, True)):  # the last replays the captured graph
+        load(ctx, off, pos, vel, mass, pn_flags=A.NEWTON, precision=precision)
+        ctx.batch_step(A.WH_PLANETARY, 1e-3, 25, steps_per_launch=spl, use_graph=graph)
+        p2, v2 = ctx.batch_get()
+        assert np.array_equal(p, p2) and np.array_equal(v, v2), (spl, graph)
+
+
+def test_batch_wisdom_holman_two_calls_and_errors(ctx, orc):
+    # Two calls convert back and forth once each; the oracle does the same.
+    off, pos, vel, mass = ensemble(16, 7)
+    load(ctx, off, pos, vel, mass, pn_flags=A.NEWTON)
+    ctx.batch_step(A.WH_PLANETARY, 2e-3, 10)
+    ctx.batch_step(A.WH_PLANETARY, 2e-3, 5)
+    p, v = ctx.batch_get()
+    for s in range(16):
+        a, b = off[s], off[s + 1]
+        rp, rv = orc.integrate_wh(pos[a:b], vel[a:b], mass[a:b], 2e-3, 10)
+        rp, rv = orc.integrate_wh(rp, rv, mass[a:b], 2e-3, 5)
+        assert np.array_equal(p[a:b], rp) and np.array_equal(v[a:b], rv), s
+    # a leapfrog call after WH recomputes its accelerations
+    ctx.batch_step(A.LEAPFROG, 1e-3, 3, use_graph=False)
+    p, v = ctx.batch_get()
+    s = 3
+    a, b = off[s], off[s + 1]
+    rp, rv = orc.integrate_wh(pos[a:b], vel[a:b], mass[a:b], 2e-3, 10)
+    rp, rv = orc.integrate_wh(rp, rv, mass[a:b], 2e-3, 5)
+    rp, rv, _ = orc.integrate(rp, rv, mass[a:b], 1e-3, 3, integrator=2, G=1.0, c=1e4, soft=0.0, pn=0x1)
+    assert np.array_equal(p[a:b], rp) and np.array_equal(v[a:b], rv)
+    load(ctx, off, pos, vel, mass, pn_flags=A.NEWTON | A.PN1)
+    with pytest.raises(A.AsterError):
+        ctx.batch_step(A.WH_PLANETARY, 1e-3, 1)
+
+
 def test_batch_fast_mode_close_to_exact(ctx):
     off, pos, vel, mass = ensemble(256, 4)
     load(ctx, off, pos, vel, mass, precision=A.FP64)

@@ -434,6 +434,124 @@ def test_saba_time_reversible(orc):
         np.testing.assert_allclose(-v2, vel, atol=1e-11)
 
 
+# ---- Kepler drift and Wisdom-Holman ----------------------------------------
+
+def classical_kepler(q, u, mu, dt):
+    """Two-body propagation through the eccentric (or hyperbolic) anomaly and
+    the perifocal frame, independent of the universal-variable solver."""
+    r = np.linalg.norm(q)
+    h = np.cross(q, u)
+    evec = np.cross(u, h) / mu - q / r
+    e = np.linalg.norm(evec)
+    a = 1.0 / (2.0 / r - (u @ u) / mu)
+    P = evec / e
+    W = h / np.linalg.norm(h)
+    Qh = np.cross(W, P)
+    if a > 0:
+        n = np.sqrt(mu / a ** 3)
+        E0 = np.arctan2((q @ u) / (e * np.sqrt(mu * a)), (1 - r / a) / e)
+        M = E0 - e * np.sin(E0) + n * dt
+        E = M
+        for _ in range(100):
+            E = E - (E - e * np.sin(E) - M) / (1 - e * np.cos(E))
+        b = a * np.sqrt(1 - e * e)
+        rr = a * (1 - e * np.cos(E))
+        return (a * (np.cos(E) - e) * P + b * np.sin(E) * Qh,
+                (-a * np.sin(E) * P + b * np.cos(E) * Qh) * n * a / rr)
+    aa = -a
+    n = np.sqrt(mu / aa ** 3)
+    F0 = np.arcsinh((q @ u) / (e * np.sqrt(mu * aa)))
+    M = e * np.sinh(F0) - F0 + n * dt
+    F = np.arcsinh(M / e)
+    for _ in range(200):
+        F = F - (e * np.sinh(F) - F - M) / (e * np.cosh(F) - 1)
+    b = aa * np.sqrt(e * e - 1)
+    rr = aa * (e * np.cosh(F) - 1)
+    return (aa * (e - np.cosh(F)) * P + b * np.sinh(F) * Qh,
+            (-aa * np.sinh(F) * P + b * np.cosh(F) * Qh) * n * aa / rr)
+
+
+def test_kepler_drift_matches_anomaly_solution(orc):
+    # Random bound orbits (e up to 0.95, drifts up to 3 periods, either
+    # direction) and hyperbolic ones; tolerance 1e-11 relative.
+    rng = np.random.default_rng(11)
+    worst = 0.0
+    for t in range(300):
+        mu = rng.uniform(0.5, 2.0)
+        r = rng.uniform(0.3, 3.0)
+        q = rng.normal(size=3)
+        q *= r / np.linalg.norm(q)
+        vc = np.sqrt(mu / r)
+        u = rng.normal(size=3)
+        u -= (u @ q) / (q @ q) * q
+        u *= vc * rng.uniform(0.5, 1.9 if t % 3 == 0 else 1.35) / np.linalg.norm(u)
+        u += rng.normal(scale=0.1 * vc) * q / r
+        a = 1.0 / (2.0 / r - (u @ u) / mu)
+        scale = 2 * np.pi * np.sqrt(abs(a) ** 3 / mu) if a > 0 else r / vc
+        dt = rng.uniform(0.01, 3.0) * scale * (1 if t % 2 else -1)
+        q1, u1 = orc.kepler_drift(q, u, mu, dt)
+        q2, u2 = classical_kepler(q, u, mu, dt)
+        worst = max(worst, np.abs(q1 - q2).max() / np.linalg.norm(q2), np.abs(u1 - u2).max() / np.linalg.norm(u2))
+    assert worst < 1e-11, worst
+
+
+def test_kepler_drift_conserves_and_reverses(orc):
+    mu = 1.3
+    q = np.array([0.7, -0.2, 0.1])
+    u = np.array([0.3, 1.2, -0.2])
+    E = lambda q, u: 0.5 * (u @ u) - mu / np.linalg.norm(q)
+    q1, u1 = q, u
+    for _ in range(500):
+        q1, u1 = orc.kepler_drift(q1, u1, mu, 0.037)
+    assert E(q1, u1) == pytest.approx(E(q, u), rel=1e-13)
+    np.testing.assert_allclose(np.cross(q1, u1), np.cross(q, u), rtol=1e-13)
+    q2, u2 = orc.kepler_drift(q1, u1, mu, -0.037 * 500)
+    np.testing.assert_allclose(q2, q, atol=1e-12)
+    np.testing.assert_allclose(u2, u, atol=1e-12)
+
+
+def wh_system(eps):
+    pos = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.6, 0.05]])
+    vel = np.array([[0.0, 0.0, 0.0], [0.0, 1.0, 0.02], [-0.8, 0.0, 0.0]])
+    return pos, vel, np.array([1.0, eps, eps])
+
+
+def test_wh_test_particle_is_exact_kepler(orc):
+    # A massless planet feels only the central body: the jump vanishes with
+    # m/m0 and the drift is exact, so WH reproduces the two-body orbit.
+    pos = np.array([[0.2, -0.1, 0.0], [1.2, -0.1, 0.0]])
+    vel = np.array([[0.01, 0.0, 0.0], [0.01, 1.1, 0.1]])
+    mass = np.array([1.0, 1e-20])
+    p, v = orc.integrate_wh(pos, vel, mass, 0.05, 400)
+    q2, u2 = classical_kepler(pos[1] - pos[0], vel[1] - vel[0], 1.0, 20.0)
+    np.testing.assert_allclose(p[1] - p[0], q2, atol=2e-13)
+    np.testing.assert_allclose(v[1] - v[0], u2, atol=2e-13)
+    np.testing.assert_allclose(p[0], pos[0] + vel[0] * 20.0, atol=1e-14)
+
+
+def test_wh_second_order_error_proportional_to_planet_mass(orc):
+    # Wisdom-Holman's error is O(eps dt^2): second order in dt, 100x smaller
+    # for planets 100x lighter. Reference: SABA10 with 65536 steps.
+    T = 8.0
+    err = {}
+    for eps in (1e-3, 1e-5):
+        pos, vel, mass = wh_system(eps)
+        ref, _, _ = orc.integrate(pos, vel, mass, T / 65536, 65536, integrator=1, order=10)
+        err[eps] = [np.abs(orc.integrate_wh(pos, vel, mass, T / k, k)[0] - ref).max() for k in (64, 128, 256)]
+    rates = np.log2(np.array(err[1e-3][:-1]) / np.array(err[1e-3][1:]))
+    assert np.all(np.abs(rates - 2.0) < 0.2), rates
+    ratio = err[1e-3][0] / err[1e-5][0]
+    assert 50 < ratio < 200, ratio
+
+
+def test_wh_time_reversible(orc):
+    pos, vel, mass = wh_system(1e-3)
+    p, v = orc.integrate_wh(pos, vel, mass, 0.01, 300)
+    p2, v2 = orc.integrate_wh(p, -v, mass, 0.01, 300)
+    np.testing.assert_allclose(p2, pos, atol=1e-13)
+    np.testing.assert_allclose(-v2, vel, atol=1e-13)
+
+
 def test_yoshida_weights(orc):
     w = orc.yoshida_weights()
     w1 = 1.0 / (2.0 - 2.0 ** (1.0 / 3.0))
