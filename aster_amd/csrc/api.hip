@@ -144,7 +144,7 @@ int aster_destroy(aster_ctx* c) {
   c->x.release(); c->v.release(); c->m.release(); c->a.release(); c->sh.release(); c->src64.release();
   c->apart.release(); c->apn.release(); c->ework.release(); c->eout.release(); c->flags.release();
   c->boxpart.release(); c->boxd.release();
-  c->codes.release(); c->idx.release(); c->codes2.release(); c->idx2.release(); c->hist.release();
+  c->codes.release(); c->xm4.release(); c->idx.release(); c->codes2.release(); c->idx2.release(); c->hist.release();
   c->left.release(); c->right.release(); c->parent.release(); c->escape.release(); c->scanpart.release();
   c->depth.release(); c->nmc.release(); c->nvel.release(); c->nbox.release();
   c->levels.release(); c->bucket.release(); c->dsorted.release(); c->grange.release(); c->gscan.release();

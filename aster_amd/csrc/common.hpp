@@ -118,6 +118,7 @@ struct aster_ctx {
 
   // Barnes-Hut / LBVH.
   aster::DBuf<uint32_t> codes, idx, codes2, idx2;  // sort ping-pong, n
+  aster::DBuf<double4> xm4;                        // {x, y, z, m} per body (body order), n
   aster::DBuf<uint32_t> hist;                      // per-block digit counts
   aster::DBuf<int32_t> left, right, parent, escape;// 2n-1
   aster::DBuf<uint32_t> scanpart;                  // per-chunk totals of the device scan

@@ -53,15 +53,21 @@ __device__ __forceinline__ uint32_t cell_of(double p, double b, uint32_t* bad) {
   return c < 1023u ? c : 1023u;
 }
 
-__global__ __launch_bounds__(256) void k_morton(int n, const double* __restrict__ x, const double* __restrict__ box,
-                                                uint32_t* __restrict__ codes, uint32_t* __restrict__ idx,
+// Also packs {x, y, z, m} of each body into one 32-byte record, so the leaf
+// initialisation gathers one record through the sort permutation instead of
+// four scattered planes.
+__global__ __launch_bounds__(256) void k_morton(int n, const double* __restrict__ x, const double* __restrict__ m,
+                                                const double* __restrict__ box, uint32_t* __restrict__ codes,
+                                                uint32_t* __restrict__ idx, double4* __restrict__ xm,
                                                 uint32_t* __restrict__ flags) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t bad = 0;
-  const uint32_t ix = cell_of(x[i], box[0], &bad);
-  const uint32_t iy = cell_of(x[n + i], box[1], &bad);
-  const uint32_t iz = cell_of(x[2 * n + i], box[2], &bad);
+  const double px = x[i], py = x[n + i], pz = x[2 * n + i];
+  xm[i] = make_double4(px, py, pz, m[i]);
+  const uint32_t ix = cell_of(px, box[0], &bad);
+  const uint32_t iy = cell_of(py, box[1], &bad);
+  const uint32_t iz = cell_of(pz, box[2], &bad);
   codes[i] = (part1by2(iz) << 2) | (part1by2(iy) << 1) | part1by2(ix);
   idx[i] = (uint32_t)i;
   if (bad) atomicOr(flags, (uint32_t)FLAG_OUT_OF_BOX);
@@ -216,8 +222,8 @@ __device__ __forceinline__ int kdelta(const uint32_t* __restrict__ codes, int n,
 // also gathers the velocities (only the 1PN traversal reads node velocities).
 template <bool VEL>
 __global__ __launch_bounds__(256) void k_build(int n, const uint32_t* __restrict__ codes,
-                                               const uint32_t* __restrict__ perm, const double* __restrict__ x,
-                                               const double* __restrict__ v, const double* __restrict__ m,
+                                               const uint32_t* __restrict__ perm, const double4* __restrict__ xm,
+                                               const double* __restrict__ v,
                                                int32_t* __restrict__ left, int32_t* __restrict__ right,
                                                int32_t* __restrict__ parent, int2* __restrict__ grange,
                                                double4* __restrict__ nmc, double4* __restrict__ nvel,
@@ -227,8 +233,9 @@ __global__ __launch_bounds__(256) void k_build(int n, const uint32_t* __restrict
   if (k >= n) return;
   {
     const uint32_t b = perm[k];
-    const double px = x[b], py = x[n + b], pz = x[2 * n + b], mb = m[b];
-    nmc[k] = make_double4(px, py, pz, mb);
+    const double4 pm = xm[b];
+    const double px = pm.x, py = pm.y, pz = pm.z, mb = pm.w;
+    nmc[k] = pm;
     if (VEL) nvel[k] = make_double4(v[b], v[n + b], v[2 * n + b], 0.0);
     const float fx = (float)px, fy = (float)py, fz = (float)pz;
     const float4 p = make_float4(fx, fy, fz, 0.f);
@@ -747,6 +754,7 @@ static int reserve_sort(aster_ctx* c) {
   ASTER_HIP(c, c->idx.reserve(n));
   ASTER_HIP(c, c->codes2.reserve(n));
   ASTER_HIP(c, c->idx2.reserve(n));
+  ASTER_HIP(c, c->xm4.reserve(n));
   ASTER_HIP(c, c->hist.reserve(256 * nblocks));
   ASTER_HIP(c, c->scanpart.reserve((n + kScanChunk - 1) / kScanChunk + 1));
   return ASTER_OK;
@@ -797,8 +805,8 @@ static int morton_codes(aster_ctx* c, bool user_box) {
   int rc = launch_box(c, user_box);
   if (rc) return rc;
   const int n = (int)c->n;
-  hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->x.p, c->boxd.p, c->codes.p,
-                     c->idx.p, c->flags.p);
+  hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->x.p, c->m.p, c->boxd.p,
+                     c->codes.p, c->idx.p, c->xm4.p, c->flags.p);
   ASTER_HIP(c, hipGetLastError());
   phase_end(c, ASTER_PHASE_MORTON);
   c->tree_valid = false;
@@ -844,8 +852,8 @@ static int build_tree(aster_ctx* c, int mac, bool vel) {
   phase_begin(c, ASTER_PHASE_BUILD);
   if (vel) ASTER_HIP(c, c->nvel.reserve((size_t)2 * n - 1));
   auto build = vel ? k_build<true> : k_build<false>;
-  hipLaunchKernelGGL(build, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->codes.p, c->idx.p, c->x.p, c->v.p,
-                     c->m.p, c->left.p, c->right.p, c->parent.p, c->grange.p, c->nmc.p, c->nvel.p, c->nbox.p,
+  hipLaunchKernelGGL(build, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->codes.p, c->idx.p, c->xm4.p,
+                     c->v.p, c->left.p, c->right.p, c->parent.p, c->grange.p, c->nmc.p, c->nvel.p, c->nbox.p,
                      c->nrec.p, c->gsh.p, c->flags.p);
   ASTER_HIP(c, hipGetLastError());
   phase_end(c, ASTER_PHASE_BUILD);

@@ -88,6 +88,14 @@ def bench_bh(A, ctx, steps, warmup, method, n=1 << 20, theta=0.5, dt=5e-4, refer
         ctx.bh_stats()
         errs = err_stats(rel_err(acc, ref))
         log(f"[bench] BH force error vs FP64 all-pairs: {errs}")
+        # the reference's own MAC (s^2 < theta^2 d^2 with s = |box| 2^-depth) at the same theta
+        ctx.forces(A.BH_PARITY, out=False)
+        ctx.bh_stats()
+        acc = ctx.forces(A.BH_PARITY)
+        inter_ref, _ = ctx.bh_stats()
+        errs["reference_mac"] = err_stats(rel_err(acc, ref))
+        errs["reference_mac"]["interactions_per_body"] = inter_ref / n
+        log(f"[bench] reference MAC at theta={theta}: {errs['reference_mac']}")
     # warmup, then K timed leapfrog steps
     ctx.step(method, A.LEAPFROG, dt, warmup)
     ctx.sync()
@@ -254,12 +262,31 @@ def bench_batch(A, ctx, nsys=8192, steps=10000, dt=1e-3):
     inter = pairs * steps * 3 * 2
     log(f"[bench] batch {nsys} systems x {steps} steps: {wall:.3f}s, {inter / wall:.3e} pair interactions/s, "
         f"wpb {wpb} spl {spl}, |dE/E| median {np.median(de):.2e}")
+    # Newtonian comparison on the same ensemble: Wisdom-Holman (central mass =
+    # body 0) against Yoshida-4 at the same dt, 2000 steps each.
+    wh = {}
+    ctx.set_params(G=1.0, softening=0.0, pn_flags=A.NEWTON, precision=A.FP32)
+    for name, integ in (("yoshida4", A.YOSHIDA4), ("wisdom_holman", A.WH_PLANETARY)):
+        ctx.batch_set(off, pos, vel, mass)
+        en0, _ = ctx.batch_energy()
+        ctx.batch_step(integ, dt, 0)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.batch_step(integ, dt, 2000, steps_per_launch=spl, use_graph=True)
+        ctx.sync()
+        tw = time.perf_counter() - t0
+        en1, _ = ctx.batch_energy()
+        d = np.abs(en1 - en0) / np.abs(en0)
+        wh[name] = {"ms_per_step": tw / 2000 * 1e3, "dE_rel_median": float(np.median(d)),
+                    "dE_rel_p90": float(np.quantile(d, 0.9))}
+    log(f"[bench] Newtonian ensemble, 2000 steps dt={dt}: {wh}")
     return {"systems": nsys, "bodies": int(off[-1]), "steps": steps, "wall_s": wall,
             "system_steps_per_s": nsys * steps / wall, "pair_interactions_per_s": inter / wall,
             "waves_per_block": wpb, "steps_per_launch": spl,
             "energy_drift": {"median": float(np.median(de)), "p90": float(np.quantile(de, 0.9)),
                              "max": float(de.max())},
-            "L_drift_max": float((np.abs(L1 - L0).max(1) / np.abs(L0).max(1)).max())}
+            "L_drift_max": float((np.abs(L1 - L0).max(1) / np.abs(L0).max(1)).max()),
+            "newtonian_wh_vs_yoshida_2000_steps": wh}
 
 
 def cpu_baseline_bh(orc, A, threads, n=1 << 20, theta=0.5, stride=1):
