@@ -196,7 +196,7 @@ int traj_forward(aster_ctx* c, double dt, int64_t nsteps, int64_t ckpt, int32_t 
   for (int q = 0; q < nsnap; ++q)
     if (snaps[q] < 1 || snaps[q] > nsteps || (q > 0 && snaps[q] <= snaps[q - 1]))
       return fail(c, ASTER_ERR_INVALID, "snapshot steps must be ascending within [1, nsteps]");
-  if (c->P.pn_flags & ASTER_PN1) return fail(c, ASTER_ERR_INVALID, "the adjoint covers Newtonian forces");
+  if (c->P.pn_flags & ASTER_PN_TERMS) return fail(c, ASTER_ERR_INVALID, "the adjoint covers Newtonian forces");
   if (c->P.lower > 0 || (c->P.upper >= 0 && c->P.upper < c->n))
     return fail(c, ASTER_ERR_INVALID, "the adjoint evaluates the full body range only");
   const int64_t n = c->n;

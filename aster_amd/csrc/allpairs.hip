@@ -154,12 +154,14 @@ __global__ void k_combine(int n, int S, const double* __restrict__ part, const d
 
 // ---------------------------------------------------------------- FP64 ----
 
-// MODE 0: Newton (+1PN if PN); MODE 1: only the 1PN term (added to an FP32
-// Newtonian result). x, v: SoA planes [3][n].
+// MODE 0: Newton (+ the post-Newtonian terms pnf selects if PN); MODE 1: only
+// the post-Newtonian terms (added to an FP32 Newtonian result). x, v: SoA
+// planes [3][n]. ic = 1/c.
 template <bool PN, int MODE>
 __global__ __launch_bounds__(256) void k_allpairs_f64(int n, int lower, int upper, const double* __restrict__ x,
                                                       const double* __restrict__ v, const double* __restrict__ m,
-                                                      double G, double ic2, double soft, double* __restrict__ a) {
+                                                      double G, double ic, int pnf, double soft,
+                                                      double* __restrict__ a) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int ii = i < n ? i : n - 1;
@@ -190,11 +192,7 @@ __global__ __launch_bounds__(256) void k_allpairs_f64(int n, int lower, int uppe
     }
     if (PN) {
       const double v2x = v[j], v2y = v[n + j], v2z = v[2 * n + j];
-      double px, py, pz;
-      pn1_term64(m1, m2, v1x, v1y, v1z, v2x, v2y, v2z, inv_r, -nx, -ny, -nz, G, px, py, pz);
-      fx = fx + (px * ic2) * m1;
-      fy = fy + (py * ic2) * m1;
-      fz = fz + (pz * ic2) * m1;
+      pn_add64(pnf, m1, m2, v1x, v1y, v1z, v2x, v2y, v2z, inv_r, -nx, -ny, -nz, G, ic, fx, fy, fz);
     }
     ax += fx / m1;
     ay += fy / m1;
@@ -229,17 +227,18 @@ int launch_allpairs(aster_ctx* c, int method) {
   const int lower = c->P.lower < 0 ? 0 : c->P.lower;
   const int upper = (c->P.upper < 0 || c->P.upper > n) ? n : c->P.upper;
   if (lower > upper) return fail(c, ASTER_ERR_INVALID, "source bounds: lower > upper");
-  const bool pn = (c->P.pn_flags & ASTER_PN1) != 0;
-  const double ic2 = (1.0 / c->P.c) * (1.0 / c->P.c);
+  const int pnf = c->P.pn_flags & ASTER_PN_TERMS;
+  const bool pn = pnf != 0;
+  const double ic = 1.0 / c->P.c;
   if (method == ASTER_ALLPAIRS_FP64) {
     dim3 grid((n + 255) / 256);
     phase_begin(c, ASTER_PHASE_ALLPAIRS);
     if (pn)
       hipLaunchKernelGGL((k_allpairs_f64<true, 0>), grid, dim3(256), 0, c->stream, n, lower, upper, c->x.p, c->v.p,
-                         c->m.p, c->P.G, ic2, c->P.softening, c->a.p);
+                         c->m.p, c->P.G, ic, pnf, c->P.softening, c->a.p);
     else
       hipLaunchKernelGGL((k_allpairs_f64<false, 0>), grid, dim3(256), 0, c->stream, n, lower, upper, c->x.p, c->v.p,
-                         c->m.p, c->P.G, ic2, c->P.softening, c->a.p);
+                         c->m.p, c->P.G, ic, pnf, c->P.softening, c->a.p);
     ASTER_HIP(c, hipGetLastError());
     phase_end(c, ASTER_PHASE_ALLPAIRS);
     return ASTER_OK;
@@ -290,7 +289,7 @@ int launch_allpairs(aster_ctx* c, int method) {
   if (pn) {
     ASTER_HIP(c, c->apn.reserve((size_t)3 * n));
     hipLaunchKernelGGL((k_allpairs_f64<true, 1>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, 0, n, c->x.p,
-                       c->v.p, c->m.p, c->P.G, ic2, c->P.softening, c->apn.p);
+                       c->v.p, c->m.p, c->P.G, ic, pnf, c->P.softening, c->apn.p);
     ASTER_HIP(c, hipGetLastError());
     pnp = c->apn.p;
   }

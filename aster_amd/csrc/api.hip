@@ -170,8 +170,8 @@ int aster_set_params(aster_ctx* c, const aster_params* p) {
     return fail(c, ASTER_ERR_INVALID, "invalid G, c, softening or theta");
   if (p->precision != ASTER_FP32 && p->precision != ASTER_FP64)
     return fail(c, ASTER_ERR_INVALID, "precision must be ASTER_FP32 or ASTER_FP64");
-  if ((p->pn_flags & ~(ASTER_NEWTON | ASTER_PN1)) != 0)
-    return fail(c, ASTER_ERR_INVALID, "only ASTER_NEWTON and ASTER_PN1 force terms are provided");
+  if ((p->pn_flags & ~(ASTER_NEWTON | ASTER_PN_TERMS)) != 0)
+    return fail(c, ASTER_ERR_INVALID, "force terms are ASTER_NEWTON, ASTER_PN1, ASTER_PN2 and ASTER_PN25");
   const bool any = p->box[0] != 0.0 || p->box[1] != 0.0 || p->box[2] != 0.0;
   if (any && !(p->box[0] > 0.0 && p->box[1] > 0.0 && p->box[2] > 0.0))
     return fail(c, ASTER_ERR_INVALID, "box half-extents must all be > 0 (or all 0 for the automatic box)");
@@ -300,7 +300,7 @@ int aster_step(aster_ctx* c, int method, int integrator, double dt, int64_t nste
   // evaluation of a step sees the fully kicked velocities, so it is not the
   // previous step's second evaluation: re-evaluate, as update_leapfrog does.
   // Newtonian forces depend on positions only and are reused.
-  const bool vdep = (c->P.pn_flags & ASTER_PN1) != 0;
+  const bool vdep = (c->P.pn_flags & ASTER_PN_TERMS) != 0;
   if (!vdep && (!c->acc_valid || c->acc_stale || c->acc_method != method))
     if ((rc = compute_forces(c, method))) return rc;
   double pending = 0.0;

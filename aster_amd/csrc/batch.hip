@@ -48,14 +48,16 @@ struct BatchArgs {
   int nsub;
   double h[3];
   int init;
-  double G, ic2, soft;
+  double G, ic2, ic, soft;
+  int pnf;  // post-Newtonian terms (exact mode; the fast mode has PN1 only)
 };
 
 template <bool PN, bool EXACT>
 __device__ __forceinline__ void batch_accel(int lane, int base, int cnt, int maxcnt, bool live, const double* lx,
                                             const double* ly, const double* lz, const double* lm, const double* lvx,
                                             const double* lvy, const double* lvz, const double* X, const double* V,
-                                            double mi, double G, double ic2, double soft, double* A) {
+                                            double mi, double G, double ic2, double ic, int pnf, double soft,
+                                            double* A) {
   double ax = 0.0, ay = 0.0, az = 0.0;
   for (int j = 0; j < maxcnt; ++j) {
     const int src = min(base + j, 63);
@@ -69,7 +71,8 @@ __device__ __forceinline__ void batch_accel(int lane, int base, int cnt, int max
     }
     if (EXACT) {
       double fx, fy, fz;
-      grav64(mi, m2, V[0], V[1], V[2], v2x, v2y, v2z, X[0], X[1], X[2], x2, y2, z2, G, ic2, soft, PN, fx, fy, fz);
+      grav64(mi, m2, V[0], V[1], V[2], v2x, v2y, v2z, X[0], X[1], X[2], x2, y2, z2, G, ic, soft, PN ? pnf : 0, fx, fy,
+             fz);
       if (valid) {
 #pragma clang fp contract(off)
         ax += fx / mi;
@@ -160,7 +163,7 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs P) {
   if (P.init) {
     publish();
     batch_accel<PN, EXACT>(lane, base, cnt, maxcnt, live, lx, ly, lz, sm[wl], lvx, lvy, lvz, X, V, mi, P.G, P.ic2,
-                           P.soft, A);
+                           P.ic, P.pnf, P.soft, A);
     wave_sync();
   }
   double pending = 0.0;
@@ -174,13 +177,13 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs P) {
         pending = 0.0;
         publish();
         batch_accel<PN, EXACT>(lane, base, cnt, maxcnt, live, lx, ly, lz, sm[wl], lvx, lvy, lvz, X, V, mi, P.G,
-                               P.ic2, P.soft, A);
+                               P.ic2, P.ic, P.pnf, P.soft, A);
         wave_sync();
       }
       kick_drift_reg(X, V, A, pending, h);
       publish();
       batch_accel<PN, EXACT>(lane, base, cnt, maxcnt, live, lx, ly, lz, sm[wl], lvx, lvy, lvz, X, V, mi, P.G,
-                             P.ic2, P.soft, A);
+                             P.ic2, P.ic, P.pnf, P.soft, A);
       wave_sync();
       pending = h;
     }
@@ -549,6 +552,8 @@ static BatchArgs make_args(aster_ctx* c, int64_t nsteps, int nsub, const double*
   P.init = init;
   P.G = c->P.G;
   P.ic2 = (1.0 / c->P.c) * (1.0 / c->P.c);
+  P.ic = 1.0 / c->P.c;
+  P.pnf = c->P.pn_flags & ASTER_PN_TERMS;
   P.soft = c->P.softening;
   return P;
 }
@@ -556,7 +561,7 @@ static BatchArgs make_args(aster_ctx* c, int64_t nsteps, int nsub, const double*
 static int launch_batch(aster_ctx* c, const BatchArgs& P) {
   const int wpb = c->batch.waves_per_block;
   const dim3 grid((P.nwaves + wpb - 1) / wpb), block(64 * wpb);
-  const bool pn = (c->P.pn_flags & ASTER_PN1) != 0;
+  const bool pn = (c->P.pn_flags & ASTER_PN_TERMS) != 0;
   const bool exact = c->P.precision == ASTER_FP64;
   if (pn && exact) hipLaunchKernelGGL((k_batch<true, true>), grid, block, 0, c->stream, P);
   else if (pn) hipLaunchKernelGGL((k_batch<true, false>), grid, block, 0, c->stream, P);
@@ -606,8 +611,10 @@ int batch_step(aster_ctx* c, int integrator, double dt, int64_t nsteps, int spl,
   if (integrator != ASTER_LEAPFROG && integrator != ASTER_YOSHIDA4 && !wh)
     return fail(c, ASTER_ERR_INVALID, "integrator must be ASTER_LEAPFROG, ASTER_YOSHIDA4 or ASTER_WH_PLANETARY");
   if (!std::isfinite(dt) || nsteps < 0 || spl < 0) return fail(c, ASTER_ERR_INVALID, "invalid dt, nsteps or spl");
-  const bool pn = (c->P.pn_flags & ASTER_PN1) != 0;
-  if (wh && pn) return fail(c, ASTER_ERR_INVALID, "ASTER_WH_PLANETARY is Newtonian: ASTER_PN1 is not supported");
+  const bool pn = (c->P.pn_flags & ASTER_PN_TERMS) != 0;
+  if (wh && pn) return fail(c, ASTER_ERR_INVALID, "ASTER_WH_PLANETARY is Newtonian: post-Newtonian terms are not supported");
+  if ((c->P.pn_flags & (ASTER_PN2 | ASTER_PN25)) && c->P.precision != ASTER_FP64)
+    return fail(c, ASTER_ERR_INVALID, "ASTER_PN2 / ASTER_PN25 in ensembles need precision ASTER_FP64 (exact mode)");
   double h[3] = {0.0, 0.0, 0.0};
   const int nsub = wh ? 1 : substeps(integrator, dt, h);
   const double h0[3] = {0.0, 0.0, 0.0};
@@ -797,7 +804,7 @@ __global__ __launch_bounds__(256) void k_batch_traj_fwd(BatchArgs P, double* __r
   };
   publish();
   batch_accel<false, false>(lane, base, cnt, maxcnt, live, sx[wl], sy[wl], sz[wl], sm[wl], nullptr, nullptr,
-                            nullptr, X, V, mi, P.G, P.ic2, P.soft, A);
+                            nullptr, X, V, mi, P.G, P.ic2, P.ic, P.pnf, P.soft, A);
   wave_sync();
   store(0);
   const double dt = P.h[0];
@@ -806,7 +813,7 @@ __global__ __launch_bounds__(256) void k_batch_traj_fwd(BatchArgs P, double* __r
     kick_drift_reg(X, V, A, pending, dt);
     publish();
     batch_accel<false, false>(lane, base, cnt, maxcnt, live, sx[wl], sy[wl], sz[wl], sm[wl], nullptr, nullptr,
-                              nullptr, X, V, mi, P.G, P.ic2, P.soft, A);
+                              nullptr, X, V, mi, P.G, P.ic2, P.ic, P.pnf, P.soft, A);
     wave_sync();
     pending = dt;
     store(s + 1);
@@ -908,7 +915,7 @@ int batch_traj_forward(aster_ctx* c, double dt, int64_t nsteps, int32_t nsnap, c
   for (int q = 0; q < nsnap; ++q)
     if (snaps[q] < 1 || snaps[q] > nsteps || (q > 0 && snaps[q] <= snaps[q - 1]))
       return fail(c, ASTER_ERR_INVALID, "snapshot steps must be ascending within [1, nsteps]");
-  if (c->P.pn_flags & ASTER_PN1) return fail(c, ASTER_ERR_INVALID, "the adjoint covers Newtonian forces");
+  if (c->P.pn_flags & ASTER_PN_TERMS) return fail(c, ASTER_ERR_INVALID, "the adjoint covers Newtonian forces");
   const size_t nb3 = (size_t)3 * B.nbodies;
   if ((double)(nsteps + 1) * nb3 * 8.0 > 64e9) return fail(c, ASTER_ERR_CAPACITY, "trajectory would exceed 64 GB");
   ASTER_HIP(c, B.traj.reserve((size_t)(nsteps + 1) * nb3));

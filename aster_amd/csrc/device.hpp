@@ -34,13 +34,119 @@ __device__ __forceinline__ void pn1_term64(double m1, double m2, double v1x, dou
   az = az * G * m2 * inv_r * inv_r;
 }
 
+// get_pn2 (tool_chain_impl.cpp:87-107), the two-body 2PN acceleration of
+// body 1, in the reference's operation order: every product left to right,
+// every sum left to right, the fraction constants folded as written (3.0/2.0,
+// 52.0/3.0, ...). `n` is the caller's -norm.
+__device__ __forceinline__ void pn2_term64(double m1, double m2, double v1x, double v1y, double v1z, double v2x,
+                                           double v2y, double v2z, double inv_r, double nx, double ny, double nz,
+                                           double G, double& ax, double& ay, double& az) {
+#pragma clang fp contract(off)
+  const double v2_sq = v2x * v2x + v2y * v2y + v2z * v2z;
+  const double v1_sq = v1x * v1x + v1y * v1y + v1z * v1z;
+  const double v_prod = v1x * v2x + v1y * v2y + v1z * v2z;
+  const double nv1 = nx * v1x + ny * v1y + nz * v1z;
+  const double nv2 = nx * v2x + ny * v2y + nz * v2z;
+  const double nv22 = nv2 * nv2;
+  const double pn_1 = G * m1 * inv_r;
+  const double pn_2 = G * m2 * inv_r;
+  const double nv12 = nv1 * nv1;
+  double A = -2.0 * v2_sq * v2_sq + 4.0 * v2_sq * v_prod;
+  A = A - 2.0 * v_prod * v_prod;
+  A = A + 3.0 / 2.0 * v1_sq * nv22;
+  A = A + 9.0 / 2.0 * v2_sq * nv22;
+  A = A - 6.0 * v_prod * nv22;
+  A = A - 15.0 / 8.0 * nv22 * nv22;
+  double g1 = -15.0 / 4.0 * v1_sq + 5.0 / 4.0 * v2_sq;
+  g1 = g1 - 5.0 / 2.0 * v_prod;
+  g1 = g1 + 39.0 / 2.0 * nv12;
+  g1 = g1 - 39.0 * nv1 * nv2;
+  g1 = g1 + 17.0 / 2.0 * nv22;
+  A = A + pn_1 * g1;
+  double g2 = 4.0 * v2_sq - 8.0 * v_prod;
+  g2 = g2 + 2.0 * nv12;
+  g2 = g2 - 4.0 * nv1 * nv2;
+  g2 = g2 - 6.0 * nv22;
+  A = A + pn_2 * g2;
+  double B = v1_sq * nv2 + 4.0 * v2_sq * nv1;
+  B = B - 5.0 * v2_sq * nv2;
+  B = B - 4.0 * v_prod * nv1;
+  B = B + 4.0 * v_prod * nv2;
+  B = B - 6.0 * nv1 * nv22;
+  B = B + 9.0 / 2.0 * nv22 * nv2;
+  B = B + pn_1 * (-63.0 / 4.0 * nv1 + 55.0 / 4.0 * nv2);
+  B = B + pn_2 * (-2.0 * nv1 - 2.0 * nv2);
+  const double s1 = G * m2 * inv_r * inv_r;
+  const double s2 = G * G * G * m2 * inv_r * inv_r * inv_r * inv_r;
+  double C = -57.0 / 4.0 * m1 * m1 - 9.0 * m2 * m2;
+  C = C - 69.0 / 2.0 * m1 * m2;
+  ax = (nx * A + (v1x - v2x) * B) * s1 + (nx * s2) * C;
+  ay = (ny * A + (v1y - v2y) * B) * s1 + (ny * s2) * C;
+  az = (nz * A + (v1z - v2z) * B) * s1 + (nz * s2) * C;
+}
+
+// get_pn25 (tool_chain_impl.cpp:73-84), the two-body 2.5PN radiation-reaction
+// acceleration of body 1, in the reference's operation order. `n` is the
+// caller's -norm.
+__device__ __forceinline__ void pn25_term64(double m1, double m2, double v1x, double v1y, double v1z, double v2x,
+                                            double v2y, double v2z, double inv_r, double nx, double ny, double nz,
+                                            double G, double& ax, double& ay, double& az) {
+#pragma clang fp contract(off)
+  const double pn_1 = G * m1 * inv_r;
+  const double pn_2 = G * m2 * inv_r;
+  const double dvx = v1x - v2x, dvy = v1y - v2y, dvz = v1z - v2z;
+  const double sq = dvx * dvx + dvy * dvy + dvz * dvz;
+  const double k1 = -sq + 2.0 * pn_1 - 8.0 * pn_2;
+  const double ndv = (nx * v1x + ny * v1y + nz * v1z) - (nx * v2x + ny * v2y + nz * v2z);
+  const double k2 = 3.0 * sq - 6.0 * pn_1 + 52.0 / 3.0 * pn_2;
+  double w[3] = {dvx * k1 + (nx * ndv) * k2, dvy * k1 + (ny * ndv) * k2, dvz * k1 + (nz * ndv) * k2};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) w[c] = w[c] * 4.0 / 5.0 * G * G * m1 * m2 * inv_r * inv_r * inv_r;
+  ax = w[0];
+  ay = w[1];
+  az = w[2];
+}
+
+// The post-Newtonian terms of compute_gravitational (tool_chain_impl.cpp:
+// 129-136) that `pnf` selects (ASTER_PN1 0x10, ASTER_PN2 0x100, ASTER_PN25
+// 0x1000), each scaled by 1/c^k (products of ic = 1/c taken left to right)
+// and by m1, added to the force f in the reference's order.
+__device__ __forceinline__ void pn_add64(int pnf, double m1, double m2, double v1x, double v1y, double v1z,
+                                         double v2x, double v2y, double v2z, double inv_r, double nx, double ny,
+                                         double nz, double G, double ic, double& fx, double& fy, double& fz) {
+#pragma clang fp contract(off)
+  double px, py, pz;
+  if (pnf & 0x10) {
+    const double ic2 = ic * ic;
+    pn1_term64(m1, m2, v1x, v1y, v1z, v2x, v2y, v2z, inv_r, nx, ny, nz, G, px, py, pz);
+    fx = fx + (px * ic2) * m1;
+    fy = fy + (py * ic2) * m1;
+    fz = fz + (pz * ic2) * m1;
+  }
+  if (pnf & 0x100) {
+    const double ic4 = ic * ic * ic * ic;
+    pn2_term64(m1, m2, v1x, v1y, v1z, v2x, v2y, v2z, inv_r, nx, ny, nz, G, px, py, pz);
+    fx = fx + (px * ic4) * m1;
+    fy = fy + (py * ic4) * m1;
+    fz = fz + (pz * ic4) * m1;
+  }
+  if (pnf & 0x1000) {
+    const double ic5 = ic * ic * ic * ic * ic;
+    pn25_term64(m1, m2, v1x, v1y, v1z, v2x, v2y, v2z, inv_r, nx, ny, nz, G, px, py, pz);
+    fx = fx + (px * ic5) * m1;
+    fy = fy + (py * ic5) * m1;
+    fz = fz + (pz * ic5) * m1;
+  }
+}
+
 // compute_gravitational (tool_chain_impl.cpp:120-137): force on body 1 from
 // body 2 in the reference's operation order (IEEE division and square root,
-// no contraction). The caller divides by m1 (builder_endpoint.cpp:747-752) or
-// multiplies by 1/m1 (barnes_hut.cpp:238-239).
+// no contraction), with the post-Newtonian terms `pnf` selects. ic = 1/c.
+// The caller divides by m1 (builder_endpoint.cpp:747-752) or multiplies by
+// 1/m1 (barnes_hut.cpp:238-239).
 __device__ __forceinline__ void grav64(double m1, double m2, double v1x, double v1y, double v1z, double v2x,
                                        double v2y, double v2z, double p1x, double p1y, double p1z, double p2x,
-                                       double p2y, double p2z, double G, double ic2, double soft, bool pn,
+                                       double p2y, double p2z, double G, double ic, double soft, int pnf,
                                        double& fx, double& fy, double& fz) {
 #pragma clang fp contract(off)
   const double dx = p2x - p1x, dy = p2y - p1y, dz = p2z - p1z;
@@ -52,13 +158,7 @@ __device__ __forceinline__ void grav64(double m1, double m2, double v1x, double 
   fx = nx * scal;
   fy = ny * scal;
   fz = nz * scal;
-  if (pn) {
-    double px, py, pz;
-    pn1_term64(m1, m2, v1x, v1y, v1z, v2x, v2y, v2z, inv_r, -nx, -ny, -nz, G, px, py, pz);
-    fx = fx + (px * ic2) * m1;
-    fy = fy + (py * ic2) * m1;
-    fz = fz + (pz * ic2) * m1;
-  }
+  if (pnf) pn_add64(pnf, m1, m2, v1x, v1y, v1z, v2x, v2y, v2z, inv_r, -nx, -ny, -nz, G, ic, fx, fy, fz);
 }
 
 // 1/sqrt(x) in FP64: hardware estimate plus two Newton steps (relative error

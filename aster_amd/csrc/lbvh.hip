@@ -470,7 +470,8 @@ template <bool PN>
 __global__ __launch_bounds__(256) void k_bh_f64(int n, const double* __restrict__ x, const double* __restrict__ v,
                                                 const double* __restrict__ m, const double4* __restrict__ nmc,
                                                 const double4* __restrict__ nvel, const int4* __restrict__ nlink, const int32_t* __restrict__ depth,
-                                                const double* __restrict__ boxd, double theta, double G, double ic2,
+                                                const double* __restrict__ boxd, double theta, double G, double ic,
+                                                int pnf,
                                                 double soft, double* __restrict__ a,
                                                 unsigned long long* __restrict__ counters,
                                                 uint32_t* __restrict__ flags) {
@@ -518,7 +519,8 @@ __global__ __launch_bounds__(256) void k_bh_f64(int n, const double* __restrict_
         vnz = nv.z;
       }
       double fx, fy, fz;
-      grav64(mi, nc.w, vix, viy, viz, vnx, vny, vnz, pix, piy, piz, cx, cy, cz, G, ic2, soft, PN, fx, fy, fz);
+      grav64(mi, nc.w, vix, viy, viz, vnx, vny, vnz, pix, piy, piz, cx, cy, cz, G, ic, soft, PN ? pnf : 0, fx, fy,
+             fz);
       ax += fx * inv_m;
       ay += fy * inv_m;
       az += fz * inv_m;
@@ -921,10 +923,11 @@ static int build_groups(aster_ctx* c) {
 
 int bh_forces(aster_ctx* c, int method) {
   const int n = (int)c->n;
-  const bool pn = (c->P.pn_flags & ASTER_PN1) != 0;
+  const int pnf = c->P.pn_flags & ASTER_PN_TERMS;
+  const bool pn = pnf != 0;
   const bool f64 = method == ASTER_BH_PARITY && c->P.precision == ASTER_FP64;
   if (pn && !f64)
-    return fail(c, ASTER_ERR_INVALID, "Barnes-Hut with 1PN needs ASTER_BH_PARITY with precision ASTER_FP64");
+    return fail(c, ASTER_ERR_INVALID, "Barnes-Hut with post-Newtonian terms needs ASTER_BH_PARITY with precision ASTER_FP64");
   if (!(c->P.theta >= 0.0)) return fail(c, ASTER_ERR_INVALID, "theta must be >= 0");
   if (c->P.lower > 0 || (c->P.upper >= 0 && c->P.upper < n))
     return fail(c, ASTER_ERR_INVALID, "Barnes-Hut evaluates the full body range only");
@@ -937,15 +940,15 @@ int bh_forces(aster_ctx* c, int method) {
   if (rc) return rc;
   if (method == ASTER_BH_GROUP && (rc = build_groups(c))) return rc;
   phase_begin(c, ASTER_PHASE_TRAVERSE);
-  const double ic2 = (1.0 / c->P.c) * (1.0 / c->P.c);
+  const double ic = 1.0 / c->P.c;
   if (f64) {
     if (pn)
       hipLaunchKernelGGL((k_bh_f64<true>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->x.p, c->v.p, c->m.p,
-                         c->nmc.p, c->nvel.p, c->nlink.p, c->depth.p, c->boxd.p, c->P.theta, c->P.G, ic2,
+                         c->nmc.p, c->nvel.p, c->nlink.p, c->depth.p, c->boxd.p, c->P.theta, c->P.G, ic, pnf,
                          c->P.softening, c->a.p, c->bh_counters.p, c->flags.p);
     else
       hipLaunchKernelGGL((k_bh_f64<false>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->x.p, c->v.p, c->m.p,
-                         c->nmc.p, c->nvel.p, c->nlink.p, c->depth.p, c->boxd.p, c->P.theta, c->P.G, ic2,
+                         c->nmc.p, c->nvel.p, c->nlink.p, c->depth.p, c->boxd.p, c->P.theta, c->P.G, ic, pnf,
                          c->P.softening, c->a.p, c->bh_counters.p, c->flags.p);
   } else {
     const float eps2 = (float)c->P.softening;

@@ -43,6 +43,10 @@ enum aster_status {
  * (include/Aster/simulations/basic.h:39-40). */
 #define ASTER_NEWTON 0x1
 #define ASTER_PN1 0x10
+#define ASTER_PN2 0x100
+#define ASTER_PN25 0x1000
+/* every post-Newtonian term (velocity-dependent forces) */
+#define ASTER_PN_TERMS (ASTER_PN1 | ASTER_PN2 | ASTER_PN25)
 
 /* Force methods. */
 enum aster_method {
@@ -91,7 +95,7 @@ typedef struct aster_params {
   double theta;     /* opening parameter (reference default 0.2) */
   double box[3];    /* Morton box half-extent (Barnes_Hut::bounding_box);
                        all zero = symmetric max |coordinate| of the state */
-  int32_t pn_flags; /* ASTER_NEWTON | ASTER_PN1 */
+  int32_t pn_flags; /* ASTER_NEWTON | ASTER_PN1 | ASTER_PN2 | ASTER_PN25 */
   int32_t precision;/* ASTER_FP32 / ASTER_FP64 for ASTER_BH_PARITY */
   int32_t lower;    /* source range [lower, upper) for all-pairs, as */
   int32_t upper;    /* Solver::set_bounds (sim_obj.h:45); upper < 0 = n */

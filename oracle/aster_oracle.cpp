@@ -48,6 +48,8 @@ namespace {
 
 constexpr int kNewton = 0x1;   // basic.h:39
 constexpr int kPN1 = 0x10;     // basic.h:40
+constexpr int kPN2 = 0x100;    // basic.h:41
+constexpr int kPN25 = 0x1000;  // basic.h:42
 
 struct V3 {
   double x, y, z;
@@ -110,9 +112,57 @@ inline V3 pn1_term(double m1, double m2, V3 v1, V3 v2, double inv_r, V3 n, doubl
   return a;
 }
 
+// get_pn2, tool_chain_impl.cpp:87-107: the two-body 2PN acceleration of body
+// 1 (Blanchet, Living Rev. Rel. 17 (2014) 2, eq. 203, 2PN part). `n` is the
+// caller's -norm. Every product and sum left to right, fraction constants
+// folded as written.
+inline V3 pn2_term(double m1, double m2, V3 v1, V3 v2, double inv_r, V3 n, double G) {
+  const double v2_sq = v2.x * v2.x + v2.y * v2.y + v2.z * v2.z;
+  const double v1_sq = v1.x * v1.x + v1.y * v1.y + v1.z * v1.z;
+  const double v_prod = v1.x * v2.x + v1.y * v2.y + v1.z * v2.z;
+  const double nv1 = n.x * v1.x + n.y * v1.y + n.z * v1.z;
+  const double nv2 = n.x * v2.x + n.y * v2.y + n.z * v2.z;
+  const double nv22 = nv2 * nv2;
+  const double pn_1 = G * m1 * inv_r;
+  const double pn_2 = G * m2 * inv_r;
+  const double nv12 = nv1 * nv1;
+  const double A = -2 * v2_sq * v2_sq + 4 * v2_sq * v_prod - 2 * v_prod * v_prod + 3.0 / 2.0 * v1_sq * nv22 +
+                   9.0 / 2.0 * v2_sq * nv22 - 6 * v_prod * nv22 - 15.0 / 8.0 * nv22 * nv22 +
+                   pn_1 * (-15.0 / 4.0 * v1_sq + 5.0 / 4.0 * v2_sq - 5.0 / 2.0 * v_prod + 39.0 / 2.0 * nv12 -
+                           39.0 * nv1 * nv2 + 17.0 / 2.0 * nv22) +
+                   pn_2 * (4 * v2_sq - 8 * v_prod + 2 * nv12 - 4 * nv1 * nv2 - 6 * nv22);
+  const double B = v1_sq * nv2 + 4 * v2_sq * nv1 - 5 * v2_sq * nv2 - 4 * v_prod * nv1 + 4 * v_prod * nv2 -
+                   6 * nv1 * nv22 + 9.0 / 2.0 * nv22 * nv2 + pn_1 * (-63.0 / 4.0 * nv1 + 55.0 / 4.0 * nv2) +
+                   pn_2 * (-2 * nv1 - 2 * nv2);
+  const double C = -57.0 / 4.0 * m1 * m1 - 9 * m2 * m2 - 69.0 / 2.0 * m1 * m2;
+  const double s1 = G * m2 * inv_r * inv_r;
+  const double s2 = G * G * G * m2 * inv_r * inv_r * inv_r * inv_r;
+  // s1 * (n A + (v1 - v2) B) + (s2 n) C, scalar*vec3 being vec3*scalar
+  V3 u{n.x * A + (v1.x - v2.x) * B, n.y * A + (v1.y - v2.y) * B, n.z * A + (v1.z - v2.z) * B};
+  V3 w{n.x * s2, n.y * s2, n.z * s2};
+  return V3{u.x * s1 + w.x * C, u.y * s1 + w.y * C, u.z * s1 + w.z * C};
+}
+
+// get_pn25, tool_chain_impl.cpp:73-84: the two-body 2.5PN radiation-reaction
+// acceleration of body 1 (Blanchet 2014, eq. 203, 2.5PN part). `n` is the
+// caller's -norm.
+inline V3 pn25_term(double m1, double m2, V3 v1, V3 v2, double inv_r, V3 n, double G) {
+  const double pn_1 = G * m1 * inv_r;
+  const double pn_2 = G * m2 * inv_r;
+  const V3 dv{v1.x - v2.x, v1.y - v2.y, v1.z - v2.z};
+  const double sq = dv.x * dv.x + dv.y * dv.y + dv.z * dv.z;
+  const double k1 = -sq + 2 * pn_1 - 8 * pn_2;
+  const double ndv = (n.x * v1.x + n.y * v1.y + n.z * v1.z) - (n.x * v2.x + n.y * v2.y + n.z * v2.z);
+  const double k2 = 3 * sq - 6 * pn_1 + 52.0 / 3.0 * pn_2;
+  V3 in{dv.x * k1 + (n.x * ndv) * k2, dv.y * k1 + (n.y * ndv) * k2, dv.z * k1 + (n.z * ndv) * k2};
+  auto scale = [&](double c) { return c * 4.0 / 5.0 * G * G * m1 * m2 * inv_r * inv_r * inv_r; };
+  return V3{scale(in.x), scale(in.y), scale(in.z)};
+}
+
 // compute_gravitational, tool_chain_impl.cpp:120-137: force on body 1 from
 // body 2 (G m1 m2 / r^2 along d = p2 - p1, Plummer-softened with
-// softening = eps^2), plus the pairwise 1PN term scaled by m1.
+// softening = eps^2), plus the pairwise 1PN, 2PN and 2.5PN terms the flags
+// select, each times 1/c^k and m1.
 inline V3 gravitational(double m1, double m2, V3 v1, V3 v2, V3 p1, V3 p2, const Law& L) {
   V3 d{p2.x - p1.x, p2.y - p1.y, p2.z - p1.z};
   double r2 = d.x * d.x + d.y * d.y + d.z * d.z + L.soft;
@@ -121,14 +171,31 @@ inline V3 gravitational(double m1, double m2, V3 v1, V3 v2, V3 p1, V3 p2, const 
   V3 norm{d.x * inv_r, d.y * inv_r, d.z * inv_r};
   double scal = L.G * m2 * m1 * inv_r2;
   V3 f{norm.x * scal, norm.y * scal, norm.z * scal};
+  const double inv_c = 1.0 / L.c;
+  const V3 mn{-norm.x, -norm.y, -norm.z};
   if (L.pn & kPN1) {
-    double inv_c = 1.0 / L.c;
     double ic2 = inv_c * inv_c;
-    V3 p = pn1_term(m1, m2, v1, v2, inv_r, V3{-norm.x, -norm.y, -norm.z}, L.G);
+    V3 p = pn1_term(m1, m2, v1, v2, inv_r, mn, L.G);
     V3 a1{p.x * ic2, p.y * ic2, p.z * ic2};
     f.x = f.x + a1.x * m1;
     f.y = f.y + a1.y * m1;
     f.z = f.z + a1.z * m1;
+  }
+  if (L.pn & kPN2) {
+    double ic4 = inv_c * inv_c * inv_c * inv_c;
+    V3 p = pn2_term(m1, m2, v1, v2, inv_r, mn, L.G);
+    V3 a2{p.x * ic4, p.y * ic4, p.z * ic4};
+    f.x = f.x + a2.x * m1;
+    f.y = f.y + a2.y * m1;
+    f.z = f.z + a2.z * m1;
+  }
+  if (L.pn & kPN25) {
+    double ic5 = inv_c * inv_c * inv_c * inv_c * inv_c;
+    V3 p = pn25_term(m1, m2, v1, v2, inv_r, mn, L.G);
+    V3 a25{p.x * ic5, p.y * ic5, p.z * ic5};
+    f.x = f.x + a25.x * m1;
+    f.y = f.y + a25.y * m1;
+    f.z = f.z + a25.z * m1;
   }
   return f;
 }

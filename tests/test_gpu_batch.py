@@ -61,6 +61,22 @@ def test_batch_graph_replays_consistently(ctx):
     assert np.array_equal(pg, pe) and np.array_equal(vg, ve)
 
 
+def test_batch_exact_pn2_pn25_bit_identical(ctx, orc):
+    flags = A.NEWTON | A.PN1 | A.PN2 | A.PN25
+    off, pos, vel, mass = ensemble(24, 8)
+    load(ctx, off, pos, vel, mass, c=30.0, pn_flags=flags)
+    ctx.batch_step(A.LEAPFROG, 1e-3, 6, steps_per_launch=4, use_graph=False)
+    p, v = ctx.batch_get()
+    for s in range(24):
+        a, b = off[s], off[s + 1]
+        rp, rv, _ = orc.integrate(pos[a:b], vel[a:b], mass[a:b], 1e-3, 6, integrator=2, G=1.0, c=30.0, soft=0.0,
+                                  pn=flags)
+        assert np.array_equal(p[a:b], rp) and np.array_equal(v[a:b], rv), s
+    load(ctx, off, pos, vel, mass, c=30.0, pn_flags=flags, precision=A.FP32)
+    with pytest.raises(A.AsterError):
+        ctx.batch_step(A.LEAPFROG, 1e-3, 1)  # the fast mode has 1PN only
+
+
 @pytest.mark.parametrize("precision", [A.FP64, A.FP32])
 def test_batch_wisdom_holman_bit_identical_to_oracle(ctx, orc, precision):
     # Central-mass ensemble (body 0 of each system is the central mass):

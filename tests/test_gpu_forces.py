@@ -40,6 +40,44 @@ def test_allpairs_fp64_pn1_and_bounds_bit_identical(ctx, orc):
     assert np.array_equal(acc, ref)
 
 
+ALL_PN = A.NEWTON | A.PN1 | A.PN2 | A.PN25
+
+
+@pytest.mark.parametrize("flags", [A.NEWTON | A.PN2, A.NEWTON | A.PN25, ALL_PN])
+def test_allpairs_fp64_pn2_pn25_bit_identical(ctx, orc, flags):
+    # get_pn2 / get_pn25 in the reference's operation order, FP64.
+    pos, vel, mass = plummer(600, seed=13, mass_mode=1)
+    vel = vel * 30.0
+    ctx.set_params(G=1.0, c=20.0, softening=2.5e-5, pn_flags=flags)
+    ctx.set_state(pos, vel, mass)
+    acc = ctx.forces(A.ALLPAIRS_FP64)
+    ref = orc.allpairs(pos, vel, mass, G=1.0, c=20.0, soft=2.5e-5, pn=flags)
+    assert np.array_equal(acc, ref)
+
+
+def test_all_pn_trajectory_bit_identical(ctx, orc):
+    pos, vel, mass = plummer(256, seed=14)
+    vel = vel * 20.0
+    ctx.set_params(G=1.0, c=20.0, softening=1e-4, pn_flags=ALL_PN)
+    ctx.set_state(pos, vel, mass)
+    ctx.step(A.ALLPAIRS_FP64, A.YOSHIDA4, 1e-3, 3)
+    p, v = ctx.get_state()
+    rp, rv, _ = orc.integrate(pos, vel, mass, 1e-3, 3, integrator=5, c=20.0, soft=1e-4, pn=ALL_PN)
+    assert np.array_equal(p, rp) and np.array_equal(v, rv)
+
+
+def test_mixed_precision_with_pn_terms(ctx, orc):
+    # FP32 Newtonian part + FP64 post-Newtonian correction; tolerance 1e-5
+    # max relative error against the FP64 path.
+    pos, vel, mass = plummer(4096, seed=15, mass_mode=1)
+    vel = vel * 30.0
+    ctx.set_params(G=1.0, c=20.0, softening=2.5e-5, pn_flags=ALL_PN)
+    ctx.set_state(pos, vel, mass)
+    ref = ctx.forces(A.ALLPAIRS_FP64)
+    assert np.array_equal(ref, orc.allpairs(pos, vel, mass, G=1.0, c=20.0, soft=2.5e-5, pn=ALL_PN))
+    assert rel_err(ctx.forces(A.ALLPAIRS_FP32_KAHAN), ref).max() < 1e-5
+
+
 def test_allpairs_no_softening_bit_identical(ctx, orc):
     pos, vel, mass = plummer(300, seed=4)
     ctx.set_params(G=1.0, softening=0.0)
