@@ -141,6 +141,8 @@ int aster_destroy(aster_ctx* c) {
     if (e) (void)hipEventDestroy(e);
   for (auto& e : c->prof_pool)
     if (e) (void)hipEventDestroy(e);
+  if (c->tree_ev) (void)hipEventDestroy(c->tree_ev);
+  if (c->host_words) (void)hipHostFree(c->host_words);
   c->x.release(); c->v.release(); c->m.release(); c->a.release(); c->sh.release(); c->src64.release();
   c->apart.release(); c->apn.release(); c->ework.release(); c->eout.release(); c->flags.release();
   c->boxpart.release(); c->boxd.release();
@@ -148,7 +150,7 @@ int aster_destroy(aster_ctx* c) {
   c->left.release(); c->right.release(); c->parent.release(); c->escape.release(); c->scanpart.release();
   c->depth.release(); c->nmc.release(); c->nvel.release(); c->nbox.release();
   c->levels.release(); c->bucket.release(); c->dsorted.release(); c->grange.release(); c->gscan.release();
-  c->gstart.release(); c->gcount.release();
+  c->gstart.release(); c->gcount.release(); c->dmax.release();
   c->nrec.release(); c->nlink.release(); c->bh_counters.release(); c->gsh.release();
   auto& Bt = c->batch;
   Bt.x.release(); Bt.v.release(); Bt.a.release(); Bt.m.release(); Bt.ebody.release(); Bt.esys.release();

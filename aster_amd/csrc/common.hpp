@@ -124,13 +124,13 @@ struct aster_ctx {
   aster::DBuf<uint32_t> scanpart;                  // per-chunk totals of the device scan
   aster::DBuf<int32_t> depth;                      // 2n-1
   aster::DBuf<uint32_t> dsorted;                   // internal-node depths in bucket order (n-1)
-  aster::DBuf<int32_t> levels;                     // per-depth bucket offsets + level count
+  aster::DBuf<int32_t> levels;                     // per-depth bucket offsets, level count, max depth
+  aster::DBuf<int32_t> dmax;                       // per-block largest internal-node depth
   aster::DBuf<int32_t> bucket;                     // internal nodes grouped by depth (n-1)
   aster::DBuf<int2> grange;                        // sorted-position range of internal node n+i
   aster::DBuf<uint32_t> gscan;                     // group-start flags, then their exclusive scan (n)
   aster::DBuf<int32_t> gstart;                     // first sorted position of each group (+ n)
   aster::DBuf<uint32_t> gcount;                    // number of groups (1)
-  std::vector<int32_t> level_host;                 // per-depth counts on the host
   aster::DBuf<double4> nmc;                        // FP64 {com, mass} per node (2n-1)
   aster::DBuf<double4> nvel;                       // FP64 {mass-weighted velocity, 0} per node (1PN only)
   aster::DBuf<float4> nbox;                        // FP32 AABB: {lo, 0}, {hi, 0} per node (2 (2n-1))
@@ -141,6 +141,8 @@ struct aster_ctx {
   bool tree_valid = false;
   int tree_mac = -1;
   int tree_levels = 0;                             // depths of internal nodes: 0 .. tree_levels-1
+  int32_t* host_words = nullptr;                   // pinned: census level bounds + largest depth
+  hipEvent_t tree_ev = nullptr;                    // recorded after their readback
 
   // Timers.
   hipEvent_t ev[16] = {};

@@ -31,6 +31,9 @@ constexpr int kRsTile = 2048;  // keys per block (8 per thread)
 // Depth bound of the radix tree: an internal node's split bit strictly
 // increases with depth over 30 code bits + 32 tie-break bits, so depth <= 63.
 constexpr int kMaxLevels = 128;
+// Traversal groups (and the moment kernel's per-wavefront subtrees): subtrees
+// of at most kGroup leaves whose parent has more.
+constexpr int kGroup = 64;
 
 // ------------------------------------------------------------ Morton -------
 
@@ -283,50 +286,88 @@ __global__ __launch_bounds__(256) void k_build(int n, const uint32_t* __restrict
 
 // Depth of every node and its escape link in the right-first preorder (the
 // next node after its subtree: the left sibling if it is a right child, else
-// its parent's escape). Internal node n+i also writes (depth, n+i) as a
-// key/value pair for the counting sort that buckets nodes by depth.
+// its parent's escape). Internal node n+i also writes (key, n+i) for the
+// counting sort that buckets nodes by depth: key = depth for nodes above the
+// traversal groups (more than kGroup leaves), kInGroup for the others, whose
+// moments the group kernel computes. The block's largest internal-node depth
+// goes to dmax[block].
+constexpr uint32_t kInGroup = 255u;
+
 __global__ __launch_bounds__(256) void k_depth(int n, const int32_t* __restrict__ left,
                                                const int32_t* __restrict__ right, const int32_t* __restrict__ parent,
-                                               int32_t* __restrict__ depth, int32_t* __restrict__ escape,
-                                               uint32_t* __restrict__ dkey, uint32_t* __restrict__ dval,
+                                               const int2* __restrict__ grange, int32_t* __restrict__ depth,
+                                               int32_t* __restrict__ escape, uint32_t* __restrict__ dkey,
+                                               uint32_t* __restrict__ dval, int32_t* __restrict__ dmax,
                                                uint32_t* __restrict__ flags) {
+  __shared__ int wmax[4];
   const int node = blockIdx.x * blockDim.x + threadIdx.x;
   const int nn = 2 * n - 1;
-  if (node >= nn) return;
-  int esc = -1, dep = 0;
-  int cur = node;
-  for (;;) {
-    const int p = parent[cur];
-    if (p < 0) break;
-    if (dep >= kMaxLevels - 1) {
-      atomicOr(flags, (uint32_t)FLAG_TREE);
-      break;
+  int dep = 0;
+  if (node < nn) {
+    int esc = -1;
+    int cur = node;
+    for (;;) {
+      const int p = parent[cur];
+      if (p < 0) break;
+      if (dep >= kMaxLevels - 1) {
+        atomicOr(flags, (uint32_t)FLAG_TREE);
+        break;
+      }
+      if (esc < 0 && right[p] == cur) esc = left[p];
+      ++dep;
+      cur = p;
     }
-    if (esc < 0 && right[p] == cur) esc = left[p];
-    ++dep;
-    cur = p;
+    depth[node] = dep;
+    escape[node] = esc;
+    if (node >= n) {
+      const int2 r = grange[node - n];
+      dkey[node - n] = (r.y - r.x + 1 > kGroup) ? (uint32_t)dep : kInGroup;
+      dval[node - n] = (uint32_t)node;
+    }
   }
-  depth[node] = dep;
-  escape[node] = esc;
-  if (node >= n) {
-    dkey[node - n] = (uint32_t)dep;
-    dval[node - n] = (uint32_t)node;
+  int m = (node >= n && node < nn) ? dep : 0;
+  for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) dmax[blockIdx.x] = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+}
+
+// Largest of count ints (one block), to out.
+__global__ __launch_bounds__(1024) void k_max_i32(const int32_t* __restrict__ in, int count, int32_t* __restrict__ out) {
+  __shared__ int wmax[16];
+  int m = 0;
+  for (int i = threadIdx.x; i < count; i += blockDim.x) m = max(m, in[i]);
+  for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int r = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r = max(r, wmax[w]);
+    *out = r;
   }
 }
 
-// Bucket bounds from the depth-sorted keys: off[d] = first bucket slot of
-// depth d for d <= max depth, off[max depth + 1] = count, and
-// off[kMaxLevels + 1] = number of depths. Internal-node depths have no gaps
-// (every internal node below the root has an internal parent one level up).
+// Bucket bounds from the sorted keys (depths of the nodes above the groups,
+// then kInGroup): off[d] = first slot of depth d for d < L, off[L] = first
+// in-group slot (or count), off[kMaxLevels + 1] = L, the number of depths
+// above the groups. Those depths have no gaps (the parent of such a node is
+// one too, one level up).
 __global__ __launch_bounds__(256) void k_level_bounds(int count, const uint32_t* __restrict__ key,
                                                       int32_t* __restrict__ off) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= count) return;
   const uint32_t d = key[p];
-  if (p == 0 || key[p - 1] != d) off[d] = p;
-  if (p == count - 1) {
-    off[d + 1] = count;
-    off[kMaxLevels + 1] = (int32_t)d + 1;
+  const bool first = p == 0 || key[p - 1] != d;
+  if (d != kInGroup) {
+    if (first) off[d] = p;
+    if (p == count - 1) {
+      off[d + 1] = count;
+      off[kMaxLevels + 1] = (int32_t)d + 1;
+    }
+  } else if (first) {
+    const int32_t L = p == 0 ? 0 : (int32_t)key[p - 1] + 1;
+    off[L] = p;
+    off[kMaxLevels + 1] = L;
   }
 }
 
@@ -366,6 +407,124 @@ __global__ __launch_bounds__(256) void k_moments_level(int n, const int32_t* __r
   nbox[2 * node] = make_float4(fminf(loL.x, loR.x), fminf(loL.y, loR.y), fminf(loL.z, loR.z), 0.f);
   nbox[2 * node + 1] = make_float4(fmaxf(hiL.x, hiR.x), fmaxf(hiL.y, hiR.y), fmaxf(hiL.z, hiR.z), 0.f);
   nrec[node] = make_float4((float)com.x, (float)com.y, (float)com.z, (float)com.w);
+}
+
+// Moments of every internal node inside a traversal group, one wavefront per
+// group. Internal node n+i's range starts or ends at i (Karras), so the
+// hi - lo internal nodes of a group with leaf range [lo, hi] are n+lo ..
+// n+hi-1 when the group's root is n+lo, and n+lo+1 .. n+hi when it is n+hi.
+// Lane t owns node n+base+t; the lanes sweep bottom-up, a node being ready
+// once both children are (leaves always are). Finished records stay in LDS
+// for the parents in the same wavefront; leaves come from k_build. Same
+// arithmetic as k_moments_level, so the results are the same bits.
+template <bool VEL>
+__global__ __launch_bounds__(256) void k_group_moments(int n, const int32_t* __restrict__ gstart,
+                                                       const uint32_t* __restrict__ gcount,
+                                                       const int2* __restrict__ grange,
+                                                       const int32_t* __restrict__ left,
+                                                       const int32_t* __restrict__ right, double4* __restrict__ nmc,
+                                                       double4* __restrict__ nvel, float4* __restrict__ nbox,
+                                                       float4* __restrict__ nrec) {
+#pragma clang fp contract(off)
+  __shared__ double4 smc[4][64];
+  __shared__ double4 svel[VEL ? 4 : 1][64];
+  __shared__ float4 slo[4][64], shi[4][64];
+  __shared__ int sdone[4][64];
+  const int wl = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ngroups = (int)*gcount;
+  const int stride = (int)gridDim.x * 4;
+  for (int g = blockIdx.x * 4 + wl; g < ngroups; g += stride) {
+    const int lo = gstart[g], hi = gstart[g + 1] - 1;
+    const int nint = hi - lo;  // internal nodes of the group
+    if (nint == 0) continue;   // a single leaf (uniform across the wavefront)
+    bool root_lo = lo <= n - 2;
+    if (root_lo) {
+      const int2 r = grange[lo];
+      root_lo = r.x == lo && r.y == hi;
+    }
+    const int base = root_lo ? lo : lo + 1;
+    const bool mine = lane < nint;
+    const int node = n + base + lane;
+    int L = 0, R = 0;
+    // leaf children are loaded before the sweep (one round trip); internal
+    // children come from LDS as their lanes finish
+    double4 cL = make_double4(0.0, 0.0, 0.0, 0.0), cR = cL, vL = cL, vR = cL;
+    float4 loL = make_float4(0.f, 0.f, 0.f, 0.f), hiL = loL, loR = loL, hiR = loL;
+    if (mine) {
+      L = left[node];
+      R = right[node];
+      if (L < n) {
+        cL = nmc[L];
+        if (VEL) vL = nvel[L];
+        loL = nbox[2 * L];
+        hiL = nbox[2 * L + 1];
+      }
+      if (R < n) {
+        cR = nmc[R];
+        if (VEL) vR = nvel[R];
+        loR = nbox[2 * R];
+        hiR = nbox[2 * R + 1];
+      }
+    }
+    bool done = !mine;
+    sdone[wl][lane] = done ? 1 : 0;
+    wave_sync();
+    double4 com = make_double4(0.0, 0.0, 0.0, 0.0), vel = com;
+    float4 blo = loL, bhi = loL;
+    for (int it = 0; it < 64; ++it) {
+      bool ready = false;
+      if (!done) {
+        const bool rl = L < n || sdone[wl][L - n - base];
+        const bool rr = R < n || sdone[wl][R - n - base];
+        ready = rl && rr;
+      }
+      if (ready) {
+        if (L >= n) {
+          cL = smc[wl][L - n - base];
+          if (VEL) vL = svel[wl][L - n - base];
+          loL = slo[wl][L - n - base];
+          hiL = shi[wl][L - n - base];
+        }
+        if (R >= n) {
+          cR = smc[wl][R - n - base];
+          if (VEL) vR = svel[wl][R - n - base];
+          loR = slo[wl][R - n - base];
+          hiR = shi[wl][R - n - base];
+        }
+        const double mR = cR.w, mL = cL.w;
+        const double total = mR + mL;
+        if (total > 0) {
+          com.w = total;
+          com.x = (cR.x * mR + cL.x * mL) / total;
+          com.y = (cR.y * mR + cL.y * mL) / total;
+          com.z = (cR.z * mR + cL.z * mL) / total;
+          if (VEL) {
+            vel.x = (vR.x * mR + vL.x * mL) / total;
+            vel.y = (vR.y * mR + vL.y * mL) / total;
+            vel.z = (vR.z * mR + vL.z * mL) / total;
+          }
+        }
+        blo = make_float4(fminf(loL.x, loR.x), fminf(loL.y, loR.y), fminf(loL.z, loR.z), 0.f);
+        bhi = make_float4(fmaxf(hiL.x, hiR.x), fmaxf(hiL.y, hiR.y), fmaxf(hiL.z, hiR.z), 0.f);
+        smc[wl][lane] = com;
+        if (VEL) svel[wl][lane] = vel;
+        slo[wl][lane] = blo;
+        shi[wl][lane] = bhi;
+        done = true;
+      }
+      wave_sync();
+      if (ready) sdone[wl][lane] = 1;
+      wave_sync();
+      if (__ballot(!done) == 0ull) break;
+    }
+    if (mine) {
+      nmc[node] = com;
+      if (VEL) nvel[node] = vel;
+      nbox[2 * node] = blo;
+      nbox[2 * node + 1] = bhi;
+      nrec[node] = make_float4((float)com.x, (float)com.y, (float)com.z, (float)com.w);
+    }
+  }
 }
 
 // Per node: traversal record = right child (or -1), escape link, opening
@@ -553,7 +712,6 @@ __global__ __launch_bounds__(256) void k_bh_f64(int n, const double* __restrict_
 // parent has more: its bodies are contiguous in Morton order and lie in one
 // Morton cell (the subtree's common code prefix), so the group box is compact
 // wherever the curve jumps. Groups partition the sorted bodies.
-constexpr int kGroup = 64;
 
 // Leaf k climbs while the parent still has at most kGroup leaves; k starts a
 // group when it is the first position of the subtree it stops at.
@@ -784,7 +942,7 @@ static int reserve_tree(aster_ctx* c) {
   ASTER_HIP(c, c->parent.reserve(nn));
   ASTER_HIP(c, c->escape.reserve(nn));
   ASTER_HIP(c, c->depth.reserve(nn));
-  ASTER_HIP(c, c->levels.reserve(kMaxLevels + 2));
+  ASTER_HIP(c, c->levels.reserve(kMaxLevels + 3));
   ASTER_HIP(c, c->bucket.reserve(n));
   ASTER_HIP(c, c->dsorted.reserve(n));
   ASTER_HIP(c, c->grange.reserve(n));
@@ -846,6 +1004,24 @@ int morton_sort(aster_ctx* c, bool user_box) {
   return ASTER_OK;
 }
 
+// Groups for the wavefront traversal (subtrees of at most kGroup leaves whose
+// parent has more), as first sorted positions gstart[0..ngroups] in Morton
+// order; the count stays on the device.
+static int build_groups(aster_ctx* c) {
+  const int n = (int)c->n;
+  ASTER_HIP(c, c->gscan.reserve((size_t)n));
+  ASTER_HIP(c, c->gstart.reserve((size_t)n + 1));
+  ASTER_HIP(c, c->gcount.reserve(1));
+  hipLaunchKernelGGL(k_group_flags, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->parent.p, c->grange.p,
+                     c->gscan.p);
+  int rc = device_scan(c, c->gscan.p, n, c->gcount.p);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_group_emit, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->gscan.p, c->gcount.p,
+                     c->gstart.p);
+  ASTER_HIP(c, hipGetLastError());
+  return ASTER_OK;
+}
+
 static int build_tree(aster_ctx* c, int mac, bool vel) {
   const int n = (int)c->n;
   int rc = reserve_tree(c);
@@ -858,14 +1034,20 @@ static int build_tree(aster_ctx* c, int mac, bool vel) {
                      c->v.p, c->left.p, c->right.p, c->parent.p, c->grange.p, c->nmc.p, c->nvel.p, c->nbox.p,
                      c->nrec.p, c->gsh.p, c->flags.p);
   ASTER_HIP(c, hipGetLastError());
+  // traversal groups (also the per-wavefront subtrees of the moment pass)
+  if ((rc = build_groups(c))) return rc;
   phase_end(c, ASTER_PHASE_BUILD);
   const int nn = 2 * n - 1, ni = n - 1;
   phase_begin(c, ASTER_PHASE_MOMENTS);
-  // Depth census: (depth, node) pairs of the internal nodes, counting-sorted
-  // by depth with one stable radix pass (depth < 128), so every level is a
-  // contiguous bucket. The sort scratch of the Morton pass is free here.
-  hipLaunchKernelGGL(k_depth, dim3((nn + 255) / 256), dim3(256), 0, c->stream, n, c->left.p, c->right.p,
-                     c->parent.p, c->depth.p, c->escape.p, c->codes2.p, c->idx2.p, c->flags.p);
+  // Depth census of the internal nodes above the groups: (depth, node) pairs,
+  // the in-group nodes keyed last, counting-sorted with one stable radix pass
+  // (depth < 128), so every level is a contiguous bucket. The sort scratch of
+  // the Morton pass is free here.
+  const int dblocks = (nn + 255) / 256;
+  ASTER_HIP(c, c->dmax.reserve((size_t)dblocks));
+  hipLaunchKernelGGL(k_depth, dim3(dblocks), dim3(256), 0, c->stream, n, c->left.p, c->right.p, c->parent.p,
+                     c->grange.p, c->depth.p, c->escape.p, c->codes2.p, c->idx2.p, c->dmax.p, c->flags.p);
+  hipLaunchKernelGGL(k_max_i32, dim3(1), dim3(1024), 0, c->stream, c->dmax.p, dblocks, c->levels.p + kMaxLevels + 2);
   const int nblocks = (ni + kRsTile - 1) / kRsTile;
   hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(kRsThreads), 0, c->stream, c->codes2.p, ni, 0, c->hist.p,
                      nblocks);
@@ -874,17 +1056,34 @@ static int build_tree(aster_ctx* c, int mac, bool vel) {
                      c->dsorted.p, reinterpret_cast<uint32_t*>(c->bucket.p), ni, 0, c->hist.p, nblocks);
   hipLaunchKernelGGL(k_level_bounds, dim3((ni + 255) / 256), dim3(256), 0, c->stream, ni, c->dsorted.p, c->levels.p);
   ASTER_HIP(c, hipGetLastError());
-  c->level_host.resize(kMaxLevels + 2);
-  ASTER_HIP(c, hipMemcpyAsync(c->level_host.data(), c->levels.p, (kMaxLevels + 2) * sizeof(int32_t),
-                              hipMemcpyDeviceToHost, c->stream));
-  ASTER_HIP(c, hipStreamSynchronize(c->stream));
-  const int nlev = c->level_host[kMaxLevels + 1];
-  c->tree_levels = nlev;
-  if (nlev < 1 || nlev >= kMaxLevels || c->level_host[0] != 0 || c->level_host[nlev] != ni)
-    return fail(c, ASTER_ERR_STATE, "radix tree depth census does not cover every internal node");
+  // The level bounds and the largest depth go to pinned host memory
+  // asynchronously; the host waits for them only after the group kernel below
+  // is queued, so the device keeps working meanwhile.
+  if (!c->host_words) {
+    ASTER_HIP(c, hipHostMalloc((void**)&c->host_words, (kMaxLevels + 3) * sizeof(int32_t), hipHostMallocDefault));
+    ASTER_HIP(c, hipEventCreateWithFlags(&c->tree_ev, hipEventDisableTiming));
+  }
+  ASTER_HIP(c, hipMemcpyAsync(c->host_words, c->levels.p, (kMaxLevels + 3) * sizeof(int32_t), hipMemcpyDeviceToHost,
+                              c->stream));
+  ASTER_HIP(c, hipEventRecord(c->tree_ev, c->stream));
+  // every node inside a group: one wavefront per group, bottom-up in LDS
+  const int gblocks = ((n / 32 + 1) + 3) / 4;
+  auto gmom = vel ? k_group_moments<true> : k_group_moments<false>;
+  hipLaunchKernelGGL(gmom, dim3(gblocks), dim3(256), 0, c->stream, n, c->gstart.p, c->gcount.p, c->grange.p, c->left.p,
+                     c->right.p, c->nmc.p, c->nvel.p, c->nbox.p, c->nrec.p);
+  ASTER_HIP(c, hipGetLastError());
+  ASTER_HIP(c, hipEventSynchronize(c->tree_ev));
+  const int32_t* lv = c->host_words;
+  const int nlev = lv[kMaxLevels + 1];  // depths above the groups
+  const int maxdep = lv[kMaxLevels + 2];
+  if (nlev < 0 || nlev >= kMaxLevels || maxdep < nlev - 1 || maxdep >= kMaxLevels - 1 || (nlev > 0 && lv[0] != 0))
+    return fail(c, ASTER_ERR_STATE, "radix tree depth census is inconsistent");
+  c->tree_levels = maxdep + 1;
+  // the nodes above the groups, deepest level first (their children are
+  // group roots, leaves or nodes one level deeper)
   auto moments = vel ? k_moments_level<true> : k_moments_level<false>;
   for (int d = nlev - 1; d >= 0; --d) {
-    const int off = c->level_host[d], cnt = c->level_host[d + 1] - off;
+    const int off = lv[d], cnt = lv[d + 1] - off;
     if (cnt <= 0) return fail(c, ASTER_ERR_STATE, "radix tree depth census has an empty level");
     hipLaunchKernelGGL(moments, dim3((cnt + 255) / 256), dim3(256), 0, c->stream, n, c->bucket.p + off, cnt, c->left.p,
                        c->right.p, c->nmc.p, c->nvel.p, c->nbox.p, c->nrec.p);
@@ -898,26 +1097,6 @@ static int build_tree(aster_ctx* c, int mac, bool vel) {
   phase_end(c, ASTER_PHASE_LINKS);
   c->tree_valid = true;
   c->tree_mac = mac;
-  return ASTER_OK;
-}
-
-// Groups for the wavefront traversal (subtrees of at most kGroup leaves whose
-// parent has more), as first sorted positions gstart[0..ngroups] in Morton
-// order; the count stays on the device.
-static int build_groups(aster_ctx* c) {
-  const int n = (int)c->n;
-  ASTER_HIP(c, c->gscan.reserve((size_t)n));
-  ASTER_HIP(c, c->gstart.reserve((size_t)n + 1));
-  ASTER_HIP(c, c->gcount.reserve(1));
-  phase_begin(c, ASTER_PHASE_LINKS);
-  hipLaunchKernelGGL(k_group_flags, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->parent.p, c->grange.p,
-                     c->gscan.p);
-  int rc = device_scan(c, c->gscan.p, n, c->gcount.p);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_group_emit, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->gscan.p, c->gcount.p,
-                     c->gstart.p);
-  ASTER_HIP(c, hipGetLastError());
-  phase_end(c, ASTER_PHASE_LINKS);
   return ASTER_OK;
 }
 
@@ -938,7 +1117,6 @@ int bh_forces(aster_ctx* c, int method) {
   const int mac = (method == ASTER_BH_PARITY) ? 0 : 1;
   int rc = build_tree(c, mac, pn);
   if (rc) return rc;
-  if (method == ASTER_BH_GROUP && (rc = build_groups(c))) return rc;
   phase_begin(c, ASTER_PHASE_TRAVERSE);
   const double ic = 1.0 / c->P.c;
   if (f64) {
