@@ -37,6 +37,12 @@ EULER = 0
 SABA = 1
 LEAPFROG = 2
 WH_PLANETARY = 3  # batched ensembles only
+
+# Graphs collectors (enum aster_collector).
+COLLECT_HAMILTONIAN = 0
+COLLECT_ERROR = 1
+COLLECT_DISTANCE = 2
+COLLECT_ANGULAR_MOMENTUM = 3
 YOSHIDA4 = 5
 # Force-law flags (basic.h:39-40).
 NEWTON = 0x1
@@ -89,6 +95,10 @@ SIGNATURES = {
     "aster_get_params": (ctypes.c_int, [_P, ctypes.POINTER(Params)]),
     "aster_set_state": (ctypes.c_int, [_P, _I64, _D, _D, _D]),
     "aster_get_state": (ctypes.c_int, [_P, _D, _D]),
+    "aster_set_state_device": (ctypes.c_int, [_P, _I64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "aster_get_state_device": (ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_void_p]),
+    "aster_collect": (ctypes.c_int, [_P, ctypes.c_int, _I64, _D]),
+    "aster_center_of_mass": (ctypes.c_int, [_P, _D]),
     "aster_num_bodies": (ctypes.c_int64, [_P]),
     "aster_forces": (ctypes.c_int, [_P, ctypes.c_int, _D]),
     "aster_get_acc": (ctypes.c_int, [_P, _D]),
@@ -216,6 +226,26 @@ class Context:
         vel = np.empty((self.n, 3))
         self._ck(lib().aster_get_state(self.h, _dp(pos), _dp(vel)))
         return pos, vel
+
+    def set_state_device(self, n, pos_ptr, vel_ptr, mass_ptr):
+        """Load n bodies from AoS-d3 device memory (raw device addresses, vel may be None)."""
+        self._ck(lib().aster_set_state_device(self.h, int(n), pos_ptr, vel_ptr, mass_ptr))
+        self.n = int(n)
+
+    def get_state_device(self, pos_ptr, vel_ptr):
+        """Write positions/velocities as AoS-d3 into device memory (raw addresses, either may be None)."""
+        self._ck(lib().aster_get_state_device(self.h, pos_ptr, vel_ptr))
+
+    def collect(self, kind, body=0):
+        """Graphs collector on the device: COLLECT_HAMILTONIAN, COLLECT_ERROR, COLLECT_DISTANCE, COLLECT_ANGULAR_MOMENTUM."""
+        out = ctypes.c_double()
+        self._ck(lib().aster_collect(self.h, int(kind), int(body), ctypes.byref(out)))
+        return out.value
+
+    def center_of_mass(self):
+        com = np.empty(3)
+        self._ck(lib().aster_center_of_mass(self.h, _dp(com)))
+        return com
 
     # ---- forces and integration
     def forces(self, method, out=True):

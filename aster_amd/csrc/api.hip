@@ -60,17 +60,14 @@ int compute_forces(aster_ctx* c, int method) {
   return ASTER_OK;
 }
 
-static void soa_to_aos(int64_t n, const std::vector<double>& soa, double* aos) {
-  for (int64_t i = 0; i < n; ++i)
-    for (int k = 0; k < 3; ++k) aos[3 * i + k] = soa[(size_t)k * n + i];
-}
-
+// SoA plane on the device -> AoS-d3 host array (transposed on the device).
 static int download3(aster_ctx* c, const double* dev, double* host_aos) {
   const int64_t n = c->n;
-  std::vector<double> tmp((size_t)3 * n);
-  ASTER_HIP(c, hipMemcpyAsync(tmp.data(), dev, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  ASTER_HIP(c, c->stage.reserve((size_t)7 * n));
+  int rc = export_plane(c, dev, c->stage.p);
+  if (rc) return rc;
+  ASTER_HIP(c, hipMemcpyAsync(host_aos, c->stage.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   ASTER_HIP(c, hipStreamSynchronize(c->stream));
-  soa_to_aos(n, tmp, host_aos);
   return ASTER_OK;
 }
 
@@ -145,7 +142,7 @@ int aster_destroy(aster_ctx* c) {
   if (c->host_words) (void)hipHostFree(c->host_words);
   c->x.release(); c->v.release(); c->m.release(); c->a.release(); c->sh.release(); c->src64.release();
   c->apart.release(); c->apn.release(); c->ework.release(); c->eout.release(); c->flags.release();
-  c->boxpart.release(); c->boxd.release();
+  c->boxpart.release(); c->boxd.release(); c->stage.release();
   c->codes.release(); c->xm4.release(); c->idx.release(); c->codes2.release(); c->idx2.release(); c->hist.release();
   c->left.release(); c->right.release(); c->parent.release(); c->escape.release(); c->scanpart.release();
   c->depth.release(); c->nmc.release(); c->nvel.release(); c->nbox.release();
@@ -200,51 +197,97 @@ int aster_set_state(aster_ctx* c, int64_t n, const double* pos, const double* ve
   if (!c) return ASTER_ERR_INVALID;
   if (n < 1 || !pos || !mass) return fail(c, ASTER_ERR_INVALID, "need n >= 1, positions and masses");
   if (n > (int64_t(1) << 28)) return fail(c, ASTER_ERR_CAPACITY, "at most 2^28 bodies per context");
-  for (int64_t i = 0; i < n; ++i) {
-    if (!(mass[i] > 0.0) || !std::isfinite(mass[i]))
-      return fail(c, ASTER_ERR_INVALID, "masses must be finite and > 0 (accelerations divide by m_i)");
-    for (int k = 0; k < 3; ++k)
-      if (!std::isfinite(pos[3 * i + k]) || (vel && !std::isfinite(vel[3 * i + k])))
-        return fail(c, ASTER_ERR_NONFINITE, "non-finite position or velocity in input");
-  }
   ASTER_HIP(c, hipSetDevice(c->device));
-  const int64_t npad = round_up(n, kApBlock);
-  ASTER_HIP(c, c->x.reserve(3 * n));
-  ASTER_HIP(c, c->v.reserve(3 * n));
-  ASTER_HIP(c, c->a.reserve(3 * n));
-  ASTER_HIP(c, c->m.reserve(n));
-  ASTER_HIP(c, c->sh.reserve(npad));
-  std::vector<double> xs((size_t)3 * n), vs((size_t)3 * n, 0.0);
-  for (int64_t i = 0; i < n; ++i)
-    for (int k = 0; k < 3; ++k) {
-      xs[(size_t)k * n + i] = pos[3 * i + k];
-      if (vel) vs[(size_t)k * n + i] = vel[3 * i + k];
-    }
+  // the host arrays go over as they are (AoS-d3) and are transposed and
+  // checked on the device
+  ASTER_HIP(c, c->stage.reserve((size_t)7 * n));
+  double* sp = c->stage.p;
   ASTER_HIP(c, hipStreamSynchronize(c->stream));
-  ASTER_HIP(c, hipMemcpyAsync(c->x.p, xs.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  ASTER_HIP(c, hipMemcpyAsync(c->v.p, vs.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  ASTER_HIP(c, hipMemcpyAsync(c->m.p, mass, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  ASTER_HIP(c, hipMemsetAsync(c->a.p, 0, 3 * n * sizeof(double), c->stream));
-  c->n = n;
-  c->npad = npad;
-  int rc = launch_reset_flags(c);
-  if (rc) return rc;
-  ASTER_HIP(c, hipStreamSynchronize(c->stream));
-  c->have_state = true;
-  c->acc_valid = false;
-  c->tree_valid = false;
-  c->adj.ready = false;
-  return ASTER_OK;
+  ASTER_HIP(c, hipMemcpyAsync(sp, pos, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (vel) ASTER_HIP(c, hipMemcpyAsync(sp + 3 * n, vel, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  ASTER_HIP(c, hipMemcpyAsync(sp + 6 * n, mass, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  return import_state(c, n, sp, vel ? sp + 3 * n : nullptr, sp + 6 * n);
+}
+
+int aster_set_state_device(aster_ctx* c, int64_t n, const double* d_pos, const double* d_vel, const double* d_mass) {
+  if (!c) return ASTER_ERR_INVALID;
+  if (n < 1 || !d_pos || !d_mass) return fail(c, ASTER_ERR_INVALID, "need n >= 1, positions and masses");
+  if (n > (int64_t(1) << 28)) return fail(c, ASTER_ERR_CAPACITY, "at most 2^28 bodies per context");
+  ASTER_HIP(c, hipSetDevice(c->device));
+  return import_state(c, n, d_pos, d_vel, d_mass);
 }
 
 int aster_get_state(aster_ctx* c, double* pos, double* vel) {
   if (!c) return ASTER_ERR_INVALID;
   if (!c->have_state) return fail(c, ASTER_ERR_STATE, "no state loaded");
   ASTER_HIP(c, hipSetDevice(c->device));
+  const int64_t n = c->n;
+  ASTER_HIP(c, c->stage.reserve((size_t)7 * n));
   int rc;
-  if (pos && (rc = download3(c, c->x.p, pos))) return rc;
-  if (vel && (rc = download3(c, c->v.p, vel))) return rc;
+  if (pos) {
+    if ((rc = export_plane(c, c->x.p, c->stage.p))) return rc;
+    ASTER_HIP(c, hipMemcpyAsync(pos, c->stage.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  if (vel) {
+    if ((rc = export_plane(c, c->v.p, c->stage.p + 3 * n))) return rc;
+    ASTER_HIP(c, hipMemcpyAsync(vel, c->stage.p + 3 * n, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  ASTER_HIP(c, hipStreamSynchronize(c->stream));
   return check_flags(c);
+}
+
+int aster_get_state_device(aster_ctx* c, double* d_pos, double* d_vel) {
+  if (!c) return ASTER_ERR_INVALID;
+  if (!c->have_state) return fail(c, ASTER_ERR_STATE, "no state loaded");
+  ASTER_HIP(c, hipSetDevice(c->device));
+  int rc;
+  if (d_pos && (rc = export_plane(c, c->x.p, d_pos))) return rc;
+  if (d_vel && (rc = export_plane(c, c->v.p, d_vel))) return rc;
+  ASTER_HIP(c, hipStreamSynchronize(c->stream));
+  return check_flags(c);
+}
+
+int aster_collect(aster_ctx* c, int kind, int64_t body, double* out) {
+  if (!c || !out) return ASTER_ERR_INVALID;
+  if (!c->have_state) return fail(c, ASTER_ERR_STATE, "no state loaded");
+  ASTER_HIP(c, hipSetDevice(c->device));
+  int rc;
+  if (kind == ASTER_COLLECT_HAMILTONIAN || kind == ASTER_COLLECT_ERROR || kind == ASTER_COLLECT_ANGULAR_MOMENTUM) {
+    double e_ref, L[3];
+    if ((rc = aster_energy(c, &e_ref, nullptr, L))) return rc;
+    if (kind == ASTER_COLLECT_HAMILTONIAN) {
+      *out = e_ref;
+    } else if (kind == ASTER_COLLECT_ANGULAR_MOMENTUM) {
+      *out = std::sqrt(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
+    } else if (!c->e0_valid) {  // error_collector: the first call records E0 and reports 0
+      c->e0 = e_ref;
+      c->e0_valid = true;
+      *out = 0.0;
+    } else {
+      *out = std::fabs(c->e0 - e_ref) / std::fabs(c->e0);
+    }
+    return ASTER_OK;
+  }
+  if (kind == ASTER_COLLECT_DISTANCE) {
+    if (body < 0 || body >= c->n) return fail(c, ASTER_ERR_INVALID, "body index out of range");
+    double com[3], p[3];
+    if ((rc = center_of_mass(c, com))) return rc;
+    for (int k = 0; k < 3; ++k)
+      ASTER_HIP(c, hipMemcpyAsync(&p[k], c->x.p + (size_t)k * c->n + body, sizeof(double), hipMemcpyDeviceToHost,
+                                  c->stream));
+    ASTER_HIP(c, hipStreamSynchronize(c->stream));
+    const double dx = com[0] - p[0] + 0.1, dy = com[1] - p[1] + 0.1, dz = com[2] - p[2] + 0.1;
+    *out = std::sqrt(dx * dx + dy * dy + dz * dz);
+    return ASTER_OK;
+  }
+  return fail(c, ASTER_ERR_INVALID, "unknown collector");
+}
+
+int aster_center_of_mass(aster_ctx* c, double* com) {
+  if (!c || !com) return ASTER_ERR_INVALID;
+  if (!c->have_state) return fail(c, ASTER_ERR_STATE, "no state loaded");
+  ASTER_HIP(c, hipSetDevice(c->device));
+  return center_of_mass(c, com);
 }
 
 int64_t aster_num_bodies(const aster_ctx* c) { return c ? c->n : 0; }

@@ -38,7 +38,7 @@ struct DBuf {
 };
 
 // Device-side error word bits (ctx->flags[0]).
-enum : uint32_t { FLAG_OUT_OF_BOX = 1u, FLAG_NONFINITE = 2u, FLAG_TREE = 4u };
+enum : uint32_t { FLAG_OUT_OF_BOX = 1u, FLAG_NONFINITE = 2u, FLAG_TREE = 4u, FLAG_BADMASS = 8u };
 
 // Sizes of the Barnes-Hut counters block (ctx->bh_counters, u64).
 enum { BHC_INTERACTIONS = 0, BHC_VISITS = 1, BHC_COUNT = 4 };
@@ -114,6 +114,9 @@ struct aster_ctx {
   aster::DBuf<double> eout;     // 8 doubles
   aster::DBuf<uint32_t> flags;  // device error word(s), 4 words
   aster::DBuf<double> boxpart;  // per-block |x| maxima for the auto box [blocks][3]
+  aster::DBuf<double> stage;    // AoS-d3 staging for host transfers (7 n)
+  double e0 = 0.0;              // first Hamiltonian after a state load (error collector)
+  bool e0_valid = false;
   aster::DBuf<double> boxd;     // box half-extent used (3) + |box| (1)
 
   // Barnes-Hut / LBVH.
@@ -181,6 +184,9 @@ int launch_kick(aster_ctx* c, double dt);
 int launch_saba_stage(aster_ctx* c, bool kick, double d, double cdrift, double dt);
 int launch_shadow(aster_ctx* c);
 int launch_energy(aster_ctx* c, double* out6_host);
+int import_state(aster_ctx* c, int64_t n, const double* d_pos, const double* d_vel, const double* d_mass);
+int export_plane(aster_ctx* c, const double* soa, double* d_aos);
+int center_of_mass(aster_ctx* c, double* com);
 int launch_reset_flags(aster_ctx* c);
 int launch_box(aster_ctx* c, bool user_box);             // fills c->boxd
 int bh_forces(aster_ctx* c, int method);                 // writes c->a

@@ -123,6 +123,15 @@ int aster_set_state(aster_ctx* ctx, int64_t n, const double* pos, const double* 
 /* Download state (Simulation::read_bodies_gpu, builder_endpoint.cpp:424-434);
  * either pointer may be NULL. */
 int aster_get_state(aster_ctx* ctx, double* pos, double* vel);
+/* The same for bodies already in device memory on this context's device (a
+ * framework tensor, another HIP buffer): AoS-d3 device pointers, transposed
+ * and checked on the device, with no host copy (the GPU state transfer of
+ * load_gpu_buffers, builder_endpoint.cpp:102-119, and read_bodies_gpu,
+ * :424-434). vel may be NULL (zero velocities). The call orders itself on the
+ * context's stream and returns when the copy is done. */
+int aster_set_state_device(aster_ctx* ctx, int64_t n, const double* d_pos, const double* d_vel,
+                           const double* d_mass);
+int aster_get_state_device(aster_ctx* ctx, double* d_pos, double* d_vel);
 int64_t aster_num_bodies(const aster_ctx* ctx);
 
 /* Evaluate accelerations with `method` (Solver::compute_forces postcondition:
@@ -157,6 +166,27 @@ int aster_integrate(aster_ctx* ctx, int method, int integrator, int order, doubl
 
 /* Yoshida weights used by ASTER_YOSHIDA4: w[0] = w1, w[1] = w0, w[2] = w1. */
 int aster_yoshida_weights(double* w);
+
+/* Graphs collectors (src/utils/collectors.cpp:19-79), evaluated on the device.
+ * ASTER_COLLECT_HAMILTONIAN: get_total_energy (the reference's definition,
+ *   e_ref of aster_energy);
+ * ASTER_COLLECT_ERROR: error_collector, |E0 - E| / |E0| with E0 the first
+ *   value collected after the state was loaded (that call returns 0; the
+ *   reference keeps E0 in a function static);
+ * ASTER_COLLECT_DISTANCE: distance_collector, |COM - x_body + (0.1, 0.1, 0.1)|;
+ *   COM is the true barycentre (the reference's get_center_of_mass,
+ *   builder_endpoint.cpp:262-269, returns its x component in all three);
+ * ASTER_COLLECT_ANGULAR_MOMENTUM: |L|. `body` is used by DISTANCE only. */
+enum aster_collector {
+  ASTER_COLLECT_HAMILTONIAN = 0,
+  ASTER_COLLECT_ERROR = 1,
+  ASTER_COLLECT_DISTANCE = 2,
+  ASTER_COLLECT_ANGULAR_MOMENTUM = 3
+};
+int aster_collect(aster_ctx* ctx, int kind, int64_t body, double* out);
+/* Barycentre sum m x / sum m, summed on the device in a fixed order
+ * (Simulation::get_center_of_mass, builder_endpoint.cpp:262-269). */
+int aster_center_of_mass(aster_ctx* ctx, double* com);
 
 /* SABAn coefficients used by ASTER_SABA (order 2..10): drifts c[0..order]
  * and kicks d[0..order-1] (GPU::saba_coeffs, include/Aster/physics/

@@ -126,6 +126,10 @@ def test_invalid_arguments_rejected_without_device():
     for order in (0, 1, 11):  # SABA tables exist for orders 2..10
         assert L.aster_saba_coeffs(order, c11.ctypes.data_as(aster_amd._D), d11.ctypes.data_as(aster_amd._D)) == -1
     assert L.aster_integrate(None, 0, aster_amd.SABA, 4, 1e-3, 1) == -1
+    assert L.aster_set_state_device(None, 4, None, None, None) == -1
+    assert L.aster_get_state_device(None, None, None) == -1
+    assert L.aster_collect(None, aster_amd.COLLECT_HAMILTONIAN, 0, None) == -1
+    assert L.aster_center_of_mass(None, None) == -1
     assert L.aster_destroy(None) == 0
     assert L.aster_last_error(None)  # message for failed creation
 
