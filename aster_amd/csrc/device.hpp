@@ -185,8 +185,9 @@ __device__ __forceinline__ void kahan_add32(float& s, float& c, float t) {
 // Compiler and memory ordering point for LDS traffic between the lanes of one
 // wavefront (no cross-wave synchronisation).
 // Stumpff functions c0..c3 of z: quarter z until |z| <= 0.1, 8-term nested
-// series, then c3 = (c2 + c0 c3)/4, c2 = c1^2/2, c1 = c0 c1, c0 = 2 c0^2 - 1
-// once per quartering. Only +, -, *, / in a fixed order, so the result is
+// series (multiplying by reciprocal constants folded at compile time, no
+// division), then c3 = (c2 + c0 c3)/4, c2 = c1^2/2, c1 = c0 c1,
+// c0 = 2 c0^2 - 1 once per quartering. Only +, -, *, / in a fixed order, so the result is
 // the same bits on any IEEE machine (contraction off).
 __device__ __forceinline__ void stumpff64(double z, double& c0, double& c1, double& c2, double& c3) {
 #pragma clang fp contract(off)
@@ -195,10 +196,12 @@ __device__ __forceinline__ void stumpff64(double z, double& c0, double& c1, doub
     z = z * 0.25;
     ++nq;
   }
-  c2 = 0.5 * (1.0 - z / 12.0 * (1.0 - z / 30.0 * (1.0 - z / 56.0 * (1.0 - z / 90.0 *
-       (1.0 - z / 132.0 * (1.0 - z / 182.0 * (1.0 - z / 240.0 * (1.0 - z / 306.0))))))));
-  c3 = (1.0 / 6.0) * (1.0 - z / 20.0 * (1.0 - z / 42.0 * (1.0 - z / 72.0 * (1.0 - z / 110.0 *
-       (1.0 - z / 156.0 * (1.0 - z / 210.0 * (1.0 - z / 272.0 * (1.0 - z / 342.0))))))));
+  c2 = 0.5 * (1.0 - z * (1.0 / 12.0) * (1.0 - z * (1.0 / 30.0) * (1.0 - z * (1.0 / 56.0) *
+       (1.0 - z * (1.0 / 90.0) * (1.0 - z * (1.0 / 132.0) * (1.0 - z * (1.0 / 182.0) *
+       (1.0 - z * (1.0 / 240.0) * (1.0 - z * (1.0 / 306.0)))))))));
+  c3 = (1.0 / 6.0) * (1.0 - z * (1.0 / 20.0) * (1.0 - z * (1.0 / 42.0) * (1.0 - z * (1.0 / 72.0) *
+       (1.0 - z * (1.0 / 110.0) * (1.0 - z * (1.0 / 156.0) * (1.0 - z * (1.0 / 210.0) *
+       (1.0 - z * (1.0 / 272.0) * (1.0 - z * (1.0 / 342.0)))))))));
   c1 = 1.0 - z * c3;
   c0 = 1.0 - z * c2;
   for (int k = 0; k < nq; ++k) {

@@ -693,10 +693,12 @@ static void stumpff(double z, double& c0, double& c1, double& c2, double& c3) {
     z = z * 0.25;
     ++nq;
   }
-  c2 = 0.5 * (1.0 - z / 12.0 * (1.0 - z / 30.0 * (1.0 - z / 56.0 * (1.0 - z / 90.0 *
-       (1.0 - z / 132.0 * (1.0 - z / 182.0 * (1.0 - z / 240.0 * (1.0 - z / 306.0))))))));
-  c3 = (1.0 / 6.0) * (1.0 - z / 20.0 * (1.0 - z / 42.0 * (1.0 - z / 72.0 * (1.0 - z / 110.0 *
-       (1.0 - z / 156.0 * (1.0 - z / 210.0 * (1.0 - z / 272.0 * (1.0 - z / 342.0))))))));
+  c2 = 0.5 * (1.0 - z * (1.0 / 12.0) * (1.0 - z * (1.0 / 30.0) * (1.0 - z * (1.0 / 56.0) *
+       (1.0 - z * (1.0 / 90.0) * (1.0 - z * (1.0 / 132.0) * (1.0 - z * (1.0 / 182.0) *
+       (1.0 - z * (1.0 / 240.0) * (1.0 - z * (1.0 / 306.0)))))))));
+  c3 = (1.0 / 6.0) * (1.0 - z * (1.0 / 20.0) * (1.0 - z * (1.0 / 42.0) * (1.0 - z * (1.0 / 72.0) *
+       (1.0 - z * (1.0 / 110.0) * (1.0 - z * (1.0 / 156.0) * (1.0 - z * (1.0 / 210.0) *
+       (1.0 - z * (1.0 / 272.0) * (1.0 - z * (1.0 / 342.0)))))))));
   c1 = 1.0 - z * c3;
   c0 = 1.0 - z * c2;
   for (int k = 0; k < nq; ++k) {
