@@ -43,6 +43,10 @@ COLLECT_HAMILTONIAN = 0
 COLLECT_ERROR = 1
 COLLECT_DISTANCE = 2
 COLLECT_ANGULAR_MOMENTUM = 3
+
+CLUSTER_PLUMMER = 1
+CLUSTER_CUBE = 2
+CLUSTER_DISK = 3
 YOSHIDA4 = 5
 # Force-law flags (basic.h:39-40).
 NEWTON = 0x1
@@ -72,6 +76,16 @@ class Params(ctypes.Structure):
         ("theta", ctypes.c_double), ("box", ctypes.c_double * 3), ("pn_flags", ctypes.c_int32),
         ("precision", ctypes.c_int32), ("lower", ctypes.c_int32), ("upper", ctypes.c_int32),
         ("split", ctypes.c_int32), ("reserved", ctypes.c_int32),
+    ]
+
+
+class Cluster(ctypes.Structure):
+    """aster_cluster (include/aster_hip.h)."""
+    _fields_ = [
+        ("kind", ctypes.c_int32), ("mass_mode", ctypes.c_int32), ("number", ctypes.c_int64),
+        ("seed", ctypes.c_uint64), ("position", ctypes.c_double * 3), ("velocity", ctypes.c_double * 3),
+        ("rotation", ctypes.c_double * 3), ("size", ctypes.c_double * 3), ("mass", ctypes.c_double),
+        ("center_mass", ctypes.c_double), ("scale", ctypes.c_double),
     ]
 
 
@@ -129,6 +143,10 @@ SIGNATURES = {
     "aster_ic_cube": (ctypes.c_int, [_I64, _U64, ctypes.c_double, ctypes.c_double, _D, _D, _D]),
     "aster_ic_ensemble": (ctypes.c_int, [_I64, _U64, _I32, _I32, _PI64, _PI64, _D, _D, _D]),
     "aster_ic_threebody": (ctypes.c_int, [_I64, _U64, _D, _D, _D]),
+    "aster_cluster_add": (ctypes.c_int, [_P, ctypes.POINTER(Cluster)]),
+    "aster_cluster_pop": (ctypes.c_int, [_P]),
+    "aster_cluster_queued": (ctypes.c_int, [_P, _PI64, _PI64]),
+    "aster_cluster_load": (ctypes.c_int, [_P]),
 }
 
 _lib = None
@@ -246,6 +264,38 @@ class Context:
         com = np.empty(3)
         self._ck(lib().aster_center_of_mass(self.h, _dp(com)))
         return com
+
+    # ---- cluster queue (generated on the device)
+    def cluster_add(self, kind, number, seed, position=(0, 0, 0), velocity=(0, 0, 0), rotation=(0, 0, 0),
+                    size=(1.0, 10.0, 0.0), mass=1.0, center_mass=0.0, scale=1.0, mass_mode=0):
+        k = Cluster(kind=kind, mass_mode=mass_mode, number=int(number), seed=int(seed), mass=float(mass),
+                    center_mass=float(center_mass), scale=float(scale))
+        for i in range(3):
+            k.position[i] = float(position[i])
+            k.velocity[i] = float(velocity[i])
+            k.rotation[i] = float(rotation[i])
+            k.size[i] = float(size[i])
+        self._ck(lib().aster_cluster_add(self.h, ctypes.byref(k)))
+        self.n = int(lib().aster_num_bodies(self.h))
+
+    def add_disk(self, nums, center, radius, thickness, rotation, avr_mass=10e6, center_mass=10e20, v=(0, 0, 0),
+                 seed=0, scale=1.0, inner=10.0):
+        """The reference's add_disk (presets_impl.cpp:207-255) on this context's queue."""
+        self.cluster_add(CLUSTER_DISK, nums, seed, position=center, velocity=v, rotation=rotation,
+                         size=(radius, thickness, inner), mass=avr_mass, center_mass=center_mass, scale=scale)
+
+    def cluster_pop(self):
+        self._ck(lib().aster_cluster_pop(self.h))
+
+    def cluster_queued(self):
+        nc = ctypes.c_int64()
+        nb = ctypes.c_int64()
+        self._ck(lib().aster_cluster_queued(self.h, ctypes.byref(nc), ctypes.byref(nb)))
+        return nc.value, nb.value
+
+    def cluster_load(self):
+        self._ck(lib().aster_cluster_load(self.h))
+        self.n = int(lib().aster_num_bodies(self.h))
 
     # ---- forces and integration
     def forces(self, method, out=True):

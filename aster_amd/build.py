@@ -15,7 +15,7 @@ OUT_DIR = os.path.join(HERE, "lib")
 OUT = os.path.join(OUT_DIR, "libaster_hip.so")
 BUILD = os.path.join(HERE, "_build")
 
-SOURCES = ["api.hip", "allpairs.hip", "integrate.hip", "lbvh.hip", "batch.hip", "adjoint.hip", "state.hip", "ic.cpp"]
+SOURCES = ["api.hip", "allpairs.hip", "integrate.hip", "lbvh.hip", "batch.hip", "adjoint.hip", "state.hip", "ic.hip", "ic.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ASTER_OFFLOAD_ARCH", "gfx950")

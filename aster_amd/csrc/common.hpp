@@ -158,6 +158,7 @@ struct aster_ctx {
 
   aster::BatchState batch;
   aster::AdjointState adj;
+  std::vector<aster_cluster> clusters;  // queued, not yet generated (aster_cluster_load)
   int tuned_split = 0;
 };
 
@@ -186,6 +187,7 @@ int launch_shadow(aster_ctx* c);
 int launch_energy(aster_ctx* c, double* out6_host);
 int import_state(aster_ctx* c, int64_t n, const double* d_pos, const double* d_vel, const double* d_mass);
 int export_plane(aster_ctx* c, const double* soa, double* d_aos);
+int grow_state(aster_ctx* c, int64_t add, int64_t* base);  // appends `add` bodies; *base = first new index
 int center_of_mass(aster_ctx* c, double* com);
 int launch_reset_flags(aster_ctx* c);
 int launch_box(aster_ctx* c, bool user_box);             // fills c->boxd

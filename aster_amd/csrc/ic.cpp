@@ -3,43 +3,31 @@
 // nor reproducible across machines; these generators replace them for the
 // workloads this library is measured on.
 //
-// Random numbers: counter-based SplitMix64. Stream key for body/system i is
-// mix64(seed ^ mix64(i + C1)); draw k of that stream is
-// mix64(key + (k + 1) * 0x9E3779B97F4A7C15) >> 11 scaled by 2^-53, a double in
-// [0, 1). Every body is generated independently of the others, so any host
-// (or a future device generator) reproduces identical inputs from the seed.
+// Random numbers: counter-based SplitMix64 streams, one per body or system
+// (icmath.hpp). The Plummer sphere and the cube draw each body with the
+// functions the device cluster loader uses (icgen.hpp) and reduce in the
+// device's order, so aster_ic_plummer / aster_ic_cube and a cluster of the
+// same kind and seed loaded on the device hold the same bits.
 #include <cmath>
 #include <cstdint>
 #include <vector>
 
 #include "../../include/aster_hip.h"
+#include "icgen.hpp"
 
 namespace {
 
-constexpr double kPi = 3.14159265358979323846;
+using aster::icm::Stream;
+constexpr double kPi = aster::icm::kPi;
 
-inline uint64_t mix64(uint64_t z) {
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
-
-struct Stream {
-  uint64_t key, k = 0;
-  Stream(uint64_t seed, uint64_t i) : key(mix64(seed ^ mix64(i + 0x632BE59BD9B4E019ull))) {}
-  double u() {
-    ++k;
-    return (double)(mix64(key + k * 0x9E3779B97F4A7C15ull) >> 11) * 0x1.0p-53;
-  }
-};
-
+// Sequential barycentric recentring of one small system.
 void recentre(int64_t n, double* pos, double* vel, const double* mass) {
   double M = 0.0, cx[3] = {0, 0, 0}, cv[3] = {0, 0, 0};
   for (int64_t i = 0; i < n; ++i) {
     M += mass[i];
     for (int c = 0; c < 3; ++c) {
       cx[c] += mass[i] * pos[3 * i + c];
-      if (vel) cv[c] += mass[i] * vel[3 * i + c];
+      cv[c] += mass[i] * vel[3 * i + c];
     }
   }
   for (int c = 0; c < 3; ++c) {
@@ -49,17 +37,8 @@ void recentre(int64_t n, double* pos, double* vel, const double* mass) {
   for (int64_t i = 0; i < n; ++i)
     for (int c = 0; c < 3; ++c) {
       pos[3 * i + c] -= cx[c];
-      if (vel) vel[3 * i + c] -= cv[c];
+      vel[3 * i + c] -= cv[c];
     }
-}
-
-void iso(Stream& s, double r, double* out) {
-  const double ct = 1.0 - 2.0 * s.u();
-  const double st = std::sqrt(std::fmax(0.0, 1.0 - ct * ct));
-  const double ph = 2.0 * kPi * s.u();
-  out[0] = r * st * std::cos(ph);
-  out[1] = r * st * std::sin(ph);
-  out[2] = r * ct;
 }
 
 }  // namespace
@@ -70,43 +49,46 @@ int aster_ic_plummer(int64_t n, uint64_t seed, double mtot, double b, double rma
                      double* vel, double* mass) {
   if (n < 1 || !pos || !mass || !(mtot > 0) || !(b > 0) || !(rmax > 0) || (mass_mode != 0 && mass_mode != 1))
     return ASTER_ERR_INVALID;
-  const double vscale = std::sqrt(mtot / b);
-  for (int64_t i = 0; i < n; ++i) {
-    Stream s(seed, (uint64_t)i);
-    double r = 0.0;
-    for (int tries = 0; tries < 1000; ++tries) {
-      const double x1 = s.u();
-      if (x1 <= 0.0) continue;
-      r = 1.0 / std::sqrt(std::pow(x1, -2.0 / 3.0) - 1.0);
-      if (r <= rmax) break;
-      r = rmax;  // only reached if all tries fail (probability ~ 0)
-    }
-    iso(s, r * b, pos + 3 * i);
-    double q = 0.0;
-    for (int tries = 0; tries < 1000; ++tries) {
-      const double x4 = s.u(), x5 = s.u();
-      const double g = x4 * x4 * std::pow(1.0 - x4 * x4, 3.5);
-      q = x4;
-      if (0.1 * x5 < g) break;
-    }
-    const double ve = std::sqrt(2.0) * std::pow(1.0 + r * r, -0.25) * vscale;
-    if (vel) iso(s, q * ve, vel + 3 * i);
-    mass[i] = (mass_mode == 0) ? 1.0 : std::pow(10.0, 2.0 * s.u());
+  std::vector<double> vtmp;
+  if (!vel) {
+    vtmp.resize((size_t)3 * n);
+    vel = vtmp.data();
   }
-  double msum = 0.0;
-  for (int64_t i = 0; i < n; ++i) msum += mass[i];
-  for (int64_t i = 0; i < n; ++i) mass[i] = (mass_mode == 0) ? mtot / (double)n : mass[i] * (mtot / msum);
-  recentre(n, pos, vel, mass);
+  const double vscale = std::sqrt(1.0 * mtot / b);
+  for (int64_t i = 0; i < n; ++i)
+    aster::icg::plummer_body(seed, (uint64_t)i, b, rmax, vscale, mass_mode, pos + 3 * i, vel + 3 * i, mass + i);
+  if (mass_mode == 1) {
+    double msum;
+    aster::icg::ordered_sums(n, 1, [&](int64_t i, double* v) { v[0] = mass[i]; }, &msum);
+    for (int64_t i = 0; i < n; ++i) mass[i] = mass[i] * (mtot / msum);
+  } else {
+    for (int64_t i = 0; i < n; ++i) mass[i] = mtot / (double)n;
+  }
+  // zero centre of mass and momentum
+  double S[7];
+  aster::icg::ordered_sums(
+      n, 7,
+      [&](int64_t i, double* v) {
+        v[0] = mass[i];
+        for (int c = 0; c < 3; ++c) {
+          v[1 + c] = mass[i] * pos[3 * i + c];
+          v[4 + c] = mass[i] * vel[3 * i + c];
+        }
+      },
+      S);
+  for (int64_t i = 0; i < n; ++i)
+    for (int c = 0; c < 3; ++c) {
+      pos[3 * i + c] = pos[3 * i + c] - S[1 + c] / S[0];
+      vel[3 * i + c] = vel[3 * i + c] - S[4 + c] / S[0];
+    }
   return ASTER_OK;
 }
 
 int aster_ic_cube(int64_t n, uint64_t seed, double h, double mtot, double* pos, double* vel, double* mass) {
   if (n < 1 || !pos || !mass || !(h > 0) || !(mtot > 0)) return ASTER_ERR_INVALID;
+  double vz[3];
   for (int64_t i = 0; i < n; ++i) {
-    Stream s(seed, (uint64_t)i);
-    for (int c = 0; c < 3; ++c) pos[3 * i + c] = -h + 2.0 * h * s.u();
-    if (vel)
-      for (int c = 0; c < 3; ++c) vel[3 * i + c] = 0.0;
+    aster::icg::cube_body(seed, (uint64_t)i, h, pos + 3 * i, vel ? vel + 3 * i : vz);
     mass[i] = mtot / (double)n;
   }
   return ASTER_OK;
@@ -135,28 +117,34 @@ int aster_ic_ensemble(int64_t nsys, uint64_t seed, int32_t nmin, int32_t nmax, i
     for (int c = 0; c < 3; ++c) pos[3 * o + c] = vel[3 * o + c] = 0.0;
     for (int64_t k = 1; k < ns; ++k) {
       const int64_t i = o + k;
-      const double m = std::pow(10.0, -7.0 + 4.0 * st.u());
-      const double a = 0.4 * std::pow(30.0 / 0.4, st.u());
+      const double m = aster::icm::exp((-7.0 + 4.0 * st.u()) * aster::icm::kLn10);
+      const double a = 0.4 * aster::icm::pow(30.0 / 0.4, st.u());
       const double e = 0.3 * st.u();
       const double inc = 5.0 * deg * st.u();
       const double Om = 2.0 * kPi * st.u(), om = 2.0 * kPi * st.u(), M = 2.0 * kPi * st.u();
       // Kepler's equation E - e sin E = M by Newton iteration.
       double E = M;
       for (int it = 0; it < 50; ++it) {
-        const double f = E - e * std::sin(E) - M;
-        const double dE = f / (1.0 - e * std::cos(E));
+        double sE, cE;
+        aster::icm::sincos(E, &sE, &cE);
+        const double f = E - e * sE - M;
+        const double dE = f / (1.0 - e * cE);
         E -= dE;
         if (std::fabs(dE) < 1e-15) break;
       }
       const double mu = 1.0 + m;
       const double nmo = std::sqrt(mu / (a * a * a));
-      const double cE = std::cos(E), sE = std::sin(E), q = std::sqrt(1.0 - e * e);
+      double sE, cE;
+      aster::icm::sincos(E, &sE, &cE);
+      const double q = std::sqrt(1.0 - e * e);
       const double xo = a * (cE - e), yo = a * q * sE;
       const double den = 1.0 - e * cE;
       const double vxo = -a * nmo * sE / den, vyo = a * nmo * q * cE / den;
       // Rotation R_z(Om) R_x(inc) R_z(om).
-      const double cO = std::cos(Om), sO = std::sin(Om), ci = std::cos(inc), si = std::sin(inc);
-      const double co = std::cos(om), so = std::sin(om);
+      double sO, cO, si, ci, so, co;
+      aster::icm::sincos(Om, &sO, &cO);
+      aster::icm::sincos(inc, &si, &ci);
+      aster::icm::sincos(om, &so, &co);
       const double r11 = cO * co - sO * so * ci, r12 = -cO * so - sO * co * ci;
       const double r21 = sO * co + cO * so * ci, r22 = -sO * so + cO * co * ci;
       const double r31 = so * si, r32 = co * si;

@@ -115,6 +115,54 @@ int import_state(aster_ctx* c, int64_t n, const double* d_pos, const double* d_v
   return ASTER_OK;
 }
 
+// Grows the state by `add` bodies after the current ones (whose planes are
+// copied into the wider layout) and returns the first new index in *base.
+// The new bodies are left for the caller to write; derived state
+// (accelerations, tree, energy reference, adjoint) is invalidated.
+int grow_state(aster_ctx* c, int64_t add, int64_t* base) {
+  const int64_t n0 = c->have_state ? c->n : 0;
+  const int64_t n1 = n0 + add;
+  DBuf<double> nx, nv, nm;
+  hipError_t e = nx.reserve(3 * n1);
+  if (e == hipSuccess) e = nv.reserve(3 * n1);
+  if (e == hipSuccess) e = nm.reserve(n1);
+  if (e != hipSuccess) {
+    nx.release();
+    nv.release();
+    nm.release();
+    return hip_fail(c, e, "grow_state allocation");
+  }
+  if (n0 > 0) {
+    for (int q = 0; q < 3; ++q) {
+      ASTER_HIP(c, hipMemcpyAsync(nx.p + q * n1, c->x.p + q * n0, n0 * sizeof(double), hipMemcpyDeviceToDevice,
+                                  c->stream));
+      ASTER_HIP(c, hipMemcpyAsync(nv.p + q * n1, c->v.p + q * n0, n0 * sizeof(double), hipMemcpyDeviceToDevice,
+                                  c->stream));
+    }
+    ASTER_HIP(c, hipMemcpyAsync(nm.p, c->m.p, n0 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  }
+  ASTER_HIP(c, hipStreamSynchronize(c->stream));
+  c->x.release();
+  c->v.release();
+  c->m.release();
+  c->x = nx;
+  c->v = nv;
+  c->m = nm;
+  ASTER_HIP(c, c->a.reserve(3 * n1));
+  ASTER_HIP(c, c->sh.reserve(round_up(n1, kApBlock)));
+  ASTER_HIP(c, hipMemsetAsync(c->a.p, 0, 3 * n1 * sizeof(double), c->stream));
+  c->n = n1;
+  c->npad = round_up(n1, kApBlock);
+  c->have_state = true;
+  c->acc_valid = false;
+  c->acc_stale = false;
+  c->tree_valid = false;
+  c->adj.ready = false;
+  c->e0_valid = false;
+  *base = n0;
+  return ASTER_OK;
+}
+
 // SoA plane -> AoS-d3 device array.
 int export_plane(aster_ctx* c, const double* soa, double* d_aos) {
   const int64_t n = c->n;

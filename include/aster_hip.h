@@ -324,6 +324,48 @@ int aster_ic_ensemble(int64_t nsys, uint64_t seed, int32_t nmin, int32_t nmax, i
  * U[-0.5,0.5]^3 shifted to zero total momentum. Arrays of 3*nsys bodies. */
 int aster_ic_threebody(int64_t nsys, uint64_t seed, double* pos, double* vel, double* mass);
 
+/* ---- cluster queue, generated on the device ---------------------------- */
+/* Cluster (include/Aster/building-api/clusters.h:17-116) and the presets
+ * that fill it (add_disk, src/clusters/presets_impl.cpp:207-255). Bodies are
+ * drawn from counter-based seeded streams on the device, so a queue of
+ * clusters loads 10^8 bodies without a host round trip; a Plummer or cube
+ * cluster holds the same bits as aster_ic_plummer / aster_ic_cube with the
+ * same seed (plus its translation). */
+enum aster_cluster_kind {
+  ASTER_CLUSTER_PLUMMER = 1, /* aster_ic_plummer body law; size = {b, rmax / b, -}; mass = total mass */
+  ASTER_CLUSTER_CUBE = 2,    /* uniform cube at rest; size = {half-extent, -, -}; mass = total mass */
+  ASTER_CLUSTER_DISK = 3     /* add_disk; size = {radius, thickness, inner radius}; mass = avr_mass */
+};
+typedef struct aster_cluster {
+  int32_t kind;        /* aster_cluster_kind */
+  int32_t mass_mode;   /* Plummer: 0 equal masses, 1 log-uniform over two decades (normalised) */
+  int64_t number;      /* bodies drawn (Cluster::number); a disk also has its central body */
+  uint64_t seed;       /* stream seed; body k of the cluster uses stream (seed, k) */
+  double position[3];  /* translation (Cluster::position; add_disk's center) */
+  double velocity[3];  /* bulk velocity (add_disk's v) */
+  double rotation[3];  /* degrees, disks only: rotate_point(p, rotation[0], rotation[2]) as add_disk */
+  double size[3];      /* per kind, see aster_cluster_kind */
+  double mass;         /* per kind, see aster_cluster_kind */
+  double center_mass;  /* disk: central body mass (add_disk's center_mass) */
+  double scale;        /* disk: Simulation::get_scale(), multiplies the cylinder point */
+} aster_cluster;
+/* ClusterQueue::add_cluster (clusters_endpoint.cpp:150-156) via a preset.
+ * Rejects what the presets reject with a warning (no bodies, non-positive
+ * mass, radius or thickness; presets_impl.cpp:208-221) and an inner radius
+ * above the radius (rng_point_in_cylinder, :131). A disk adds its central
+ * body to the state at once, as add_disk does (:227-231). */
+int aster_cluster_add(aster_ctx* ctx, const aster_cluster* cluster);
+/* ClusterQueue::pop_cluster (clusters_endpoint.cpp:163-174): drops the last
+ * queued cluster; on an empty queue it does nothing. */
+int aster_cluster_pop(aster_ctx* ctx);
+/* Queued clusters and the bodies they will add. */
+int aster_cluster_queued(const aster_ctx* ctx, int64_t* nclusters, int64_t* nbodies);
+/* ClusterQueue::load (clusters_endpoint.cpp:182-207): generates every queued
+ * cluster on the device, last added first, appends the bodies after the
+ * current state and empties the queue. Disk velocities use the G set by
+ * aster_set_params at this call, as the reference's builder reads get_G(). */
+int aster_cluster_load(aster_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

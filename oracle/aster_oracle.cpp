@@ -1036,4 +1036,164 @@ int orc_traj_loss(int64_t n, const double* pos0, const double* vel0, const doubl
                        nullptr, nullptr);
 }
 
+// ---- seeded initial conditions ------------------------------------------
+// The reference draws its presets from std::rand (presets_impl.cpp:70-108),
+// which no test can reproduce; what is restated here is the seeded stream
+// the library defines (SplitMix64, Steele, Lea & Flood 2014) and the body
+// laws: the Plummer sphere of Aarseth, Henon & Wielen (1974), the uniform
+// cube, and add_disk's per-body transform (presets_impl.cpp:47-64, 130-147,
+// 207-255). Elementary functions come from the C library, and sums are
+// sequential, so the library's generators (own exp/log/sincos, fixed
+// parallel reduction order) agree with these to rounding, not bit for bit.
+// Pinned by: the published SplitMix64 output sequence for state 1234567;
+// the Plummer cumulative mass profile M(<r) = r^3 (1 + r^2)^(-3/2) (KS test),
+// the virial ratio, zero barycentre and momentum; the disk's radius range,
+// circular speed and rotation norms.
+
+static uint64_t orc_mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+namespace {
+struct OrcStream {
+  uint64_t state;
+  OrcStream(uint64_t seed, uint64_t i) : state(orc_mix64(seed ^ orc_mix64(i + 0x632BE59BD9B4E019ull))) {}
+  uint64_t next() {
+    state += 0x9E3779B97F4A7C15ull;
+    return orc_mix64(state);
+  }
+  double u() { return (double)(next() >> 11) / 9007199254740992.0; }
+  double u_open0() { return (double)((next() >> 11) + 1) / 9007199254740992.0; }
+};
+
+void orc_iso(OrcStream& s, double r, double* out) {
+  const double ct = 1.0 - 2.0 * s.u();
+  const double st = std::sqrt(std::max(0.0, 1.0 - ct * ct));
+  const double ph = 2.0 * M_PI * s.u();
+  out[0] = r * st * std::cos(ph);
+  out[1] = r * st * std::sin(ph);
+  out[2] = r * ct;
+}
+
+void orc_recentre(int64_t n, double* pos, double* vel, const double* mass) {
+  double M = 0.0, cx[3] = {0, 0, 0}, cv[3] = {0, 0, 0};
+  for (int64_t i = 0; i < n; ++i) {
+    M += mass[i];
+    for (int c = 0; c < 3; ++c) {
+      cx[c] += mass[i] * pos[3 * i + c];
+      cv[c] += mass[i] * vel[3 * i + c];
+    }
+  }
+  for (int64_t i = 0; i < n; ++i)
+    for (int c = 0; c < 3; ++c) {
+      pos[3 * i + c] -= cx[c] / M;
+      vel[3 * i + c] -= cv[c] / M;
+    }
+}
+
+// rotate_point (presets_impl.cpp:47-64): single-precision intermediates.
+void orc_rotate_point(const double* v, double phi, double theta, double* out) {
+  float x1, z1, y1, z2;
+  theta *= M_PI / 180;
+  phi *= M_PI / 180;
+  x1 = v[0] * std::cos(phi) + v[2] * std::sin(phi);
+  z1 = -v[0] * std::sin(phi) + v[2] * std::cos(phi);
+  y1 = v[1] * std::cos(theta) + z1 * std::sin(theta);
+  z2 = -v[1] * std::sin(theta) + z1 * std::cos(theta);
+  out[0] = x1;
+  out[1] = y1;
+  out[2] = z2;
+}
+}  // namespace
+
+// Raw SplitMix64 outputs 1..k from `state`.
+int orc_splitmix64(uint64_t state, int k, uint64_t* out) {
+  for (int j = 0; j < k; ++j) {
+    state += 0x9E3779B97F4A7C15ull;
+    out[j] = orc_mix64(state);
+  }
+  return 0;
+}
+
+// First k uniforms [0, 1) of the stream of item i under `seed`.
+int orc_stream_uniforms(uint64_t seed, uint64_t i, int k, double* out) {
+  OrcStream s(seed, i);
+  for (int j = 0; j < k; ++j) out[j] = s.u();
+  return 0;
+}
+
+int orc_ic_plummer(int64_t n, uint64_t seed, double G, double mtot, double b, double rmax, int mass_mode,
+                   double* pos, double* vel, double* mass) {
+  const double vscale = std::sqrt(G * mtot / b);
+  for (int64_t i = 0; i < n; ++i) {
+    OrcStream s(seed, (uint64_t)i);
+    double r = rmax;
+    for (int tries = 0; tries < 1000; ++tries) {
+      const double x1 = s.u();
+      if (x1 <= 0.0) continue;
+      const double rt = 1.0 / std::sqrt(std::pow(x1, -2.0 / 3.0) - 1.0);
+      if (rt <= rmax) {
+        r = rt;
+        break;
+      }
+    }
+    orc_iso(s, r * b, pos + 3 * i);
+    double q = 0.0;
+    for (int tries = 0; tries < 1000; ++tries) {
+      const double x4 = s.u(), x5 = s.u();
+      q = x4;
+      if (0.1 * x5 < x4 * x4 * std::pow(1.0 - x4 * x4, 3.5)) break;
+    }
+    orc_iso(s, q * std::sqrt(2.0) * std::pow(1.0 + r * r, -0.25) * vscale, vel + 3 * i);
+    mass[i] = mass_mode == 1 ? std::pow(10.0, 2.0 * s.u()) : 1.0;
+  }
+  double msum = 0.0;
+  for (int64_t i = 0; i < n; ++i) msum += mass[i];
+  for (int64_t i = 0; i < n; ++i) mass[i] = mass_mode == 1 ? mass[i] * mtot / msum : mtot / (double)n;
+  orc_recentre(n, pos, vel, mass);
+  return 0;
+}
+
+int orc_ic_cube(int64_t n, uint64_t seed, double h, double mtot, double* pos, double* vel, double* mass) {
+  for (int64_t i = 0; i < n; ++i) {
+    OrcStream s(seed, (uint64_t)i);
+    for (int c = 0; c < 3; ++c) {
+      pos[3 * i + c] = -h + 2.0 * h * s.u();
+      vel[3 * i + c] = 0.0;
+    }
+    mass[i] = mtot / (double)n;
+  }
+  return 0;
+}
+
+// add_disk (presets_impl.cpp:207-255) with seeded draws: body 0 is the
+// central body, bodies 1..n the disk (stream k - 1 for body k).
+int orc_cluster_disk(int64_t n, uint64_t seed, double G, double radius, double thickness, double inner, double scale,
+                     double avr_mass, double center_mass, const double* center, const double* v,
+                     const double* rotation, double* pos, double* vel, double* mass) {
+  mass[0] = center_mass;
+  orc_rotate_point(center, rotation[0], rotation[2], pos);
+  orc_rotate_point(v, rotation[0], rotation[2], vel);
+  for (int64_t k = 0; k < n; ++k) {
+    OrcStream s(seed, (uint64_t)k);
+    // rng_point_in_cylinder (:130-147); its thickness sign flip (:134-135) is kept
+    const double th = 2.0 * M_PI * s.u();
+    const double rr = inner + (radius - inner) * s.u();
+    const double t = -thickness;
+    const double cyl[3] = {rr * std::cos(th), rr * std::sin(th), (s.u() - 0.5) * t};
+    const double p[3] = {cyl[0] * scale, cyl[1] * scale, cyl[2] * scale};
+    const double r = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    const double mv = std::sqrt(G * center_mass / r);
+    const double u[3] = {p[1] / r * mv, -p[0] / r * mv, 0.0};
+    mass[1 + k] = s.u_open0() * avr_mass;
+    const double pc[3] = {p[0] + center[0], p[1] + center[1], p[2] + center[2]};
+    const double uc[3] = {u[0] + v[0], u[1] + v[1], u[2] + v[2]};
+    orc_rotate_point(pc, rotation[0], rotation[2], pos + 3 * (1 + k));
+    orc_rotate_point(uc, rotation[0], rotation[2], vel + 3 * (1 + k));
+  }
+  return 0;
+}
+
 }  // extern "C"

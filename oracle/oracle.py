@@ -45,6 +45,11 @@ _SIG = {
     "orc_allpairs_sample": [_I64, _D, _D, _D, _dbl, _dbl, _dbl, ctypes.c_int, _I64, _I64, ctypes.c_int, _D, _D],
     "orc_bh_sample": [_I64, _D, _D, _D, _D, _dbl, _dbl, _dbl, _dbl, ctypes.c_int, _I64, ctypes.c_int, _D, _D,
                       _PI64],
+    "orc_splitmix64": [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)],
+    "orc_stream_uniforms": [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, _D],
+    "orc_ic_plummer": [_I64, ctypes.c_uint64, _dbl, _dbl, _dbl, _dbl, ctypes.c_int, _D, _D, _D],
+    "orc_ic_cube": [_I64, ctypes.c_uint64, _dbl, _dbl, _D, _D, _D],
+    "orc_cluster_disk": [_I64, ctypes.c_uint64, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _D, _D, _D, _D, _D, _D],
 }
 
 
@@ -249,3 +254,37 @@ def traj_loss(pos, vel, mass, dt, nsteps, snap_steps, target, G=1.0, soft=0.0):
     lib().orc_traj_loss(len(mass), _dp(pos), _dp(vel), _dp(mass), G, soft, dt, nsteps, len(snaps),
                         snaps.ctypes.data_as(_PI64), _dp(target), ctypes.byref(loss))
     return loss.value
+
+
+def splitmix64(state, k):
+    """Raw SplitMix64 outputs 1..k from `state`."""
+    out = np.empty(k, np.uint64)
+    lib().orc_splitmix64(state, k, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+    return out
+
+
+def stream_uniforms(seed, i, k):
+    out = np.empty(k)
+    lib().orc_stream_uniforms(seed, i, k, _dp(out))
+    return out
+
+
+def ic_plummer(n, seed, mtot=1.0, b=1.0, rmax=10.0, mass_mode=0, G=1.0):
+    pos, vel, mass = np.empty((n, 3)), np.empty((n, 3)), np.empty(n)
+    lib().orc_ic_plummer(n, seed, G, mtot, b, rmax, mass_mode, _dp(pos), _dp(vel), _dp(mass))
+    return pos, vel, mass
+
+
+def ic_cube(n, seed, h=4.0, mtot=1.0):
+    pos, vel, mass = np.empty((n, 3)), np.empty((n, 3)), np.empty(n)
+    lib().orc_ic_cube(n, seed, h, mtot, _dp(pos), _dp(vel), _dp(mass))
+    return pos, vel, mass
+
+
+def cluster_disk(n, seed, center, radius, thickness, rotation, avr_mass, center_mass, v=(0, 0, 0), scale=1.0,
+                 inner=10.0, G=1.0):
+    """add_disk with seeded draws: body 0 central, bodies 1..n the disk."""
+    pos, vel, mass = np.empty((n + 1, 3)), np.empty((n + 1, 3)), np.empty(n + 1)
+    lib().orc_cluster_disk(n, seed, G, radius, thickness, inner, scale, avr_mass, center_mass,
+                           _dp(_c(center)), _dp(_c(v)), _dp(_c(rotation)), _dp(pos), _dp(vel), _dp(mass))
+    return pos, vel, mass
