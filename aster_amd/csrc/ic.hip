@@ -94,13 +94,23 @@ __global__ __launch_bounds__(icg::kSumThreads) void k_cluster_sums(int64_t count
   if ((int)threadIdx.x < NQ) part[NQ * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// Block totals added in block order (one lane).
-__global__ void k_cluster_sums_final(int nparts, int nq, const double* __restrict__ part, double* __restrict__ out) {
-  if (threadIdx.x != 0) return;
-  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int b = 0; b < nparts; ++b)
-    for (int q = 0; q < nq; ++q) s[q] += part[nq * b + q];
-  for (int q = 0; q < nq; ++q) out[q] = s[q];
+// Block totals, padded with zeros to kSumBlocksMax, added pairwise with
+// halving strides (one block of kSumBlocksMax threads).
+template <int NQ>
+__global__ __launch_bounds__(icg::kSumBlocksMax) void k_cluster_sums_final(int nparts, const double* __restrict__ part,
+                                                                         double* __restrict__ out) {
+  __shared__ double red[NQ][icg::kSumBlocksMax];
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) red[q][t] = t < nparts ? part[NQ * t + q] : 0.0;
+  __syncthreads();
+  for (int w = icg::kSumBlocksMax / 2; w > 0; w >>= 1) {
+    if (t < w)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) red[q][t] += red[q][t + w];
+    __syncthreads();
+  }
+  if (t < NQ) out[t] = red[t][0];
 }
 
 __global__ __launch_bounds__(kGenThreads) void k_scale_mass(int64_t count, Planes P, double mtot,
@@ -227,11 +237,15 @@ int cluster_sums(aster_ctx* c, const Planes& P, int64_t count, int nq, double** 
   double* part = c->ework.p;
   double* res = c->ework.p + 8 * icg::kSumBlocksMax;
   const int B = icg::sum_blocks(count);
-  if (nq == 1)
+  if (nq == 1) {
     hipLaunchKernelGGL(k_cluster_sums<1>, dim3(B), dim3(icg::kSumThreads), 0, c->stream, count, P, part);
-  else
+    hipLaunchKernelGGL(k_cluster_sums_final<1>, dim3(1), dim3(icg::kSumBlocksMax), 0, c->stream, B,
+                       (const double*)part, res);
+  } else {
     hipLaunchKernelGGL(k_cluster_sums<7>, dim3(B), dim3(icg::kSumThreads), 0, c->stream, count, P, part);
-  hipLaunchKernelGGL(k_cluster_sums_final, dim3(1), dim3(64), 0, c->stream, B, nq, part, res);
+    hipLaunchKernelGGL(k_cluster_sums_final<7>, dim3(1), dim3(icg::kSumBlocksMax), 0, c->stream, B,
+                       (const double*)part, res);
+  }
   ASTER_HIP(c, hipGetLastError());
   *out = res;
   return ASTER_OK;

@@ -121,13 +121,14 @@ inline int sum_blocks(int64_t n) {
 // Host evaluation of sums over i < n of get(i, vals) (nq values per body) in
 // the device reduction's order: thread g of B * 256 adds i = g, g + B * 256,
 // ... in turn; the 256 partials of a block are added pairwise with strides
-// 128, 64, ..., 1; the block totals are added in block order.
+// 128, 64, ..., 1; the block totals, padded with zeros to 512, are added
+// pairwise with strides 256, 128, ..., 1.
 template <class Get>
 void ordered_sums(int64_t n, int nq, Get get, double* out) {
   const int B = sum_blocks(n);
   const int64_t stride = (int64_t)B * kSumThreads;
   std::vector<double> red((size_t)kSumThreads * nq);
-  std::vector<double> tot((size_t)nq, 0.0);
+  std::vector<double> tot((size_t)kSumBlocksMax * nq, 0.0);
   double vals[8];
   for (int b = 0; b < B; ++b) {
     for (int t = 0; t < kSumThreads; ++t) {
@@ -141,8 +142,11 @@ void ordered_sums(int64_t n, int nq, Get get, double* out) {
     for (int w = kSumThreads / 2; w > 0; w >>= 1)
       for (int t = 0; t < w; ++t)
         for (int q = 0; q < nq; ++q) red[(size_t)t * nq + q] += red[(size_t)(t + w) * nq + q];
-    for (int q = 0; q < nq; ++q) tot[q] += red[q];
+    for (int q = 0; q < nq; ++q) tot[(size_t)b * nq + q] = red[q];
   }
+  for (int w = kSumBlocksMax / 2; w > 0; w >>= 1)
+    for (int t = 0; t < w; ++t)
+      for (int q = 0; q < nq; ++q) tot[(size_t)t * nq + q] += tot[(size_t)(t + w) * nq + q];
   for (int q = 0; q < nq; ++q) out[q] = tot[q];
 }
 

@@ -292,6 +292,29 @@ def bench_batch(A, ctx, nsys=8192, steps=10000, dt=1e-3):
             "newtonian_wh_vs_yoshida_dt0.02_500_steps": wh}
 
 
+def bench_ic(A, n=1 << 24, host_n=1 << 20, reps=3):
+    """Seeded Plummer sphere generated on the device by the cluster queue
+    (fresh context per repetition: allocation + generation + recentring),
+    against the host generator of the same law on a smaller sample."""
+    times = []
+    for r in range(reps + 1):
+        with A.Context(0) as c:
+            c.set_params(G=1.0)
+            c.sync()
+            t0 = time.perf_counter()
+            c.cluster_add(A.CLUSTER_PLUMMER, n, 1000 + r, size=(1.0, 10.0, 0.0), mass=1.0)
+            c.cluster_load()
+            times.append(time.perf_counter() - t0)
+    dev_s = min(times[1:])
+    t0 = time.perf_counter()
+    A.ic_plummer(host_n, 1000)
+    host_s = time.perf_counter() - t0
+    return {"n": n, "device_load_ms": dev_s * 1e3, "device_bodies_per_s": n / dev_s,
+            "host_n": host_n, "host_ms": host_s * 1e3, "host_bodies_per_s": host_n / host_s,
+            "note": "device: cluster_add + cluster_load of a Plummer sphere (allocation, per-body draws, mass "
+                    "and barycentre sums, recentring); host: aster_ic_plummer, one thread"}
+
+
 def cpu_baseline_bh(orc, A, threads, n=1 << 20, theta=0.5, stride=1):
     pos, vel, mass = A.ic_mixed(n, 20240)
     box = orc.box_auto(pos)
@@ -325,7 +348,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward ratio")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched ensemble")
     ap.add_argument("--no-integrators", action="store_true", help="skip the integrator section")
-    ap.add_argument("--only", choices=["bh", "allpairs", "adjoint", "batch", "integrators"],
+    ap.add_argument("--no-ic", action="store_true", help="skip the initial-condition generation section")
+    ap.add_argument("--only", choices=["bh", "allpairs", "adjoint", "batch", "integrators", "ic"],
                     help="run one section only (profiling); no contract line")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -345,6 +369,8 @@ def main():
             r = bench_adjoint(A, ctx)
         elif args.only == "integrators":
             r = bench_integrators(A, ctx)
+        elif args.only == "ic":
+            r = bench_ic(A)
         else:
             r = bench_batch(A, ctx)
         print(json.dumps({"section": args.only, "result": r}, default=float), flush=True)
@@ -354,6 +380,7 @@ def main():
     integ = None if args.no_integrators else bench_integrators(A, ctx)
     adj = None if args.no_adjoint else bench_adjoint(A, ctx)
     batch = None if args.no_batch else bench_batch(A, ctx)
+    ic = None if args.no_ic else bench_ic(A)
 
     # roofline of the dominant kernel of the headline step: the traversal
     ms_t, cnt_t = bh["phases"]["traverse"]
@@ -405,6 +432,7 @@ def main():
         "adjoint_forward_ratio": None if adj is None else adj["ratio"],
         "adjoint": adj,
         "batch": batch,
+        "ic_generation": ic,
     }
     print(json.dumps(out), flush=True)
 
