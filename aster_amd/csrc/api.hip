@@ -146,8 +146,9 @@ int aster_destroy(aster_ctx* c) {
   c->codes.release(); c->xm4.release(); c->idx.release(); c->codes2.release(); c->idx2.release(); c->hist.release();
   c->left.release(); c->right.release(); c->parent.release(); c->escape.release(); c->scanpart.release();
   c->depth.release(); c->nmc.release(); c->nvel.release(); c->nbox.release();
-  c->levels.release(); c->bucket.release(); c->dsorted.release(); c->grange.release(); c->gscan.release();
-  c->gstart.release(); c->gcount.release(); c->dmax.release();
+  c->levels.release(); c->bucket.release(); c->grange.release(); c->gscan.release();
+  c->gstart.release(); c->gcount.release(); c->splitnode.release(); c->groot.release(); c->climb.release();
+  c->dsorted.release(); c->dmax.release();
   c->nrec.release(); c->nlink.release(); c->bh_counters.release(); c->gsh.release();
   auto& Bt = c->batch;
   Bt.x.release(); Bt.v.release(); Bt.a.release(); Bt.m.release(); Bt.ebody.release(); Bt.esys.release();

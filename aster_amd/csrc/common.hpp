@@ -126,10 +126,13 @@ struct aster_ctx {
   aster::DBuf<int32_t> left, right, parent, escape;// 2n-1
   aster::DBuf<uint32_t> scanpart;                  // per-chunk totals of the device scan
   aster::DBuf<int32_t> depth;                      // 2n-1
-  aster::DBuf<uint32_t> dsorted;                   // internal-node depths in bucket order (n-1)
-  aster::DBuf<int32_t> levels;                     // per-depth bucket offsets, level count, max depth
+  aster::DBuf<int32_t> splitnode;                  // internal node whose split lies after sorted position k (n-1)
+  aster::DBuf<int32_t> groot;                      // at a group's first position: the group's root node (n)
+  aster::DBuf<int32_t> climb;                      // leaf k: levels from the leaf up to its group's root (n)
+  aster::DBuf<uint32_t> dsorted;                   // census keys (depths) in bucket order (n-1)
   aster::DBuf<int32_t> dmax;                       // per-block largest internal-node depth
-  aster::DBuf<int32_t> bucket;                     // internal nodes grouped by depth (n-1)
+  aster::DBuf<int32_t> levels;                     // per-depth bucket offsets, level count, max depth
+  aster::DBuf<int32_t> bucket;                     // internal nodes grouped by depth, in-group ones last (n-1)
   aster::DBuf<int2> grange;                        // sorted-position range of internal node n+i
   aster::DBuf<uint32_t> gscan;                     // group-start flags, then their exclusive scan (n)
   aster::DBuf<int32_t> gstart;                     // first sorted position of each group (+ n)

@@ -34,6 +34,9 @@ constexpr int kMaxLevels = 128;
 // Traversal groups (and the moment kernel's per-wavefront subtrees): subtrees
 // of at most kGroup leaves whose parent has more.
 constexpr int kGroup = 64;
+// Depth-census key of the internal nodes inside the traversal groups (sorted
+// after every depth).
+constexpr uint32_t kInGroup = 255u;
 
 // ------------------------------------------------------------ Morton -------
 
@@ -231,7 +234,9 @@ __global__ __launch_bounds__(256) void k_build(int n, const uint32_t* __restrict
                                                int32_t* __restrict__ parent, int2* __restrict__ grange,
                                                double4* __restrict__ nmc, double4* __restrict__ nvel,
                                                float4* __restrict__ nbox, float4* __restrict__ nrec,
-                                               float4* __restrict__ gsh, uint32_t* __restrict__ flags) {
+                                               float4* __restrict__ gsh, int32_t* __restrict__ splitnode,
+                                               uint32_t* __restrict__ dkey, uint32_t* __restrict__ dval,
+                                               uint32_t* __restrict__ flags) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   {
@@ -282,52 +287,87 @@ __global__ __launch_bounds__(256) void k_build(int n, const uint32_t* __restrict
   parent[L] = n + i;
   parent[R] = n + i;
   grange[i] = make_int2(lo, hi);
+  splitnode[gamma] = n + i;  // every split position belongs to exactly one internal node
+  // depth census entry: in-group until k_above_census says otherwise
+  dkey[i] = kInGroup;
+  dval[i] = (uint32_t)(n + i);
 }
 
 // Depth of every node and its escape link in the right-first preorder (the
 // next node after its subtree: the left sibling if it is a right child, else
-// its parent's escape). Internal node n+i also writes (key, n+i) for the
-// counting sort that buckets nodes by depth: key = depth for nodes above the
-// traversal groups (more than kGroup leaves), kInGroup for the others, whose
-// moments the group kernel computes. The block's largest internal-node depth
-// goes to dmax[block].
-constexpr uint32_t kInGroup = 255u;
-
+// its parent's escape), for the per-body traversals and the reference MAC.
 __global__ __launch_bounds__(256) void k_depth(int n, const int32_t* __restrict__ left,
                                                const int32_t* __restrict__ right, const int32_t* __restrict__ parent,
-                                               const int2* __restrict__ grange, int32_t* __restrict__ depth,
-                                               int32_t* __restrict__ escape, uint32_t* __restrict__ dkey,
-                                               uint32_t* __restrict__ dval, int32_t* __restrict__ dmax,
+                                               int32_t* __restrict__ depth, int32_t* __restrict__ escape,
                                                uint32_t* __restrict__ flags) {
-  __shared__ int wmax[4];
   const int node = blockIdx.x * blockDim.x + threadIdx.x;
   const int nn = 2 * n - 1;
-  int dep = 0;
-  if (node < nn) {
-    int esc = -1;
-    int cur = node;
-    for (;;) {
-      const int p = parent[cur];
-      if (p < 0) break;
-      if (dep >= kMaxLevels - 1) {
-        atomicOr(flags, (uint32_t)FLAG_TREE);
-        break;
-      }
-      if (esc < 0 && right[p] == cur) esc = left[p];
-      ++dep;
-      cur = p;
+  if (node >= nn) return;
+  int dep = 0, esc = -1, cur = node;
+  for (;;) {
+    const int p = parent[cur];
+    if (p < 0) break;
+    if (dep >= kMaxLevels - 1) {
+      atomicOr(flags, (uint32_t)FLAG_TREE);
+      break;
     }
-    depth[node] = dep;
-    escape[node] = esc;
-    if (node >= n) {
-      const int2 r = grange[node - n];
-      dkey[node - n] = (r.y - r.x + 1 > kGroup) ? (uint32_t)dep : kInGroup;
-      dval[node - n] = (uint32_t)node;
+    if (esc < 0 && right[p] == cur) esc = left[p];
+    ++dep;
+    cur = p;
+  }
+  depth[node] = dep;
+  escape[node] = esc;
+}
+
+// Depths of the nodes above the traversal groups, whose moments the band
+// kernels form level by level. They are exactly the internal nodes whose
+// split lies at a boundary between consecutive groups (a node above the
+// groups splits between two of them, and each boundary is one node's
+// split), so thread t < ngroups - 1 takes boundary t, walks up for the
+// node's depth and makes that depth the node's census key (k_build keyed
+// every internal node kInGroup). Thread ngroups - 1 + g walks up from group
+// g's root instead: with the deepest climb of the group's leaves
+// (k_group_flags) that gives the group's deepest internal node. The block's
+// largest internal-node depth goes to dmax[block].
+__device__ __forceinline__ int walk_depth(int node, const int32_t* __restrict__ parent, uint32_t* __restrict__ flags) {
+  int d = 0;
+  for (int p = parent[node]; p >= 0; p = parent[p]) {
+    if (++d >= kMaxLevels - 1) {
+      atomicOr(flags, (uint32_t)FLAG_TREE);
+      break;
     }
   }
-  int m = (node >= n && node < nn) ? dep : 0;
-  for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
-  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+  return d;
+}
+
+__global__ __launch_bounds__(256) void k_above_census(int n, const int32_t* __restrict__ gstart,
+                                                      const uint32_t* __restrict__ gcount,
+                                                      const int32_t* __restrict__ splitnode,
+                                                      const int32_t* __restrict__ groot,
+                                                      const int32_t* __restrict__ climb,
+                                                      const int32_t* __restrict__ parent, int32_t* __restrict__ depth,
+                                                      uint32_t* __restrict__ dkey, int32_t* __restrict__ dmax,
+                                                      uint32_t* __restrict__ flags) {
+  __shared__ int wmax[4];
+  const int ng = (int)*gcount;
+  int mx = 0;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < 2 * ng - 1; t += gridDim.x * blockDim.x) {
+    if (t < ng - 1) {
+      const int node = splitnode[gstart[t + 1] - 1];
+      const int d = walk_depth(node, parent, flags);
+      depth[node] = d;
+      dkey[node - n] = (uint32_t)d;
+      mx = max(mx, d);
+    } else {
+      const int g = t - (ng - 1);
+      const int lo = gstart[g], hi = gstart[g + 1] - 1;
+      int cm = 0;
+      for (int k = lo; k <= hi; ++k) cm = max(cm, climb[k]);
+      if (cm > 0) mx = max(mx, walk_depth(groot[lo], parent, flags) + cm - 1);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off, 64));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
   __syncthreads();
   if (threadIdx.x == 0) dmax[blockIdx.x] = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
 }
@@ -371,42 +411,120 @@ __global__ __launch_bounds__(256) void k_level_bounds(int count, const uint32_t*
   }
 }
 
-// Monopole moments of the internal nodes of one depth; their children are one
-// level deeper and complete (kernel boundaries order the levels). VEL also
-// forms the mass-weighted velocities.
+// Band of up to H levels above the groups, one thread per node at the band's
+// top depth. The thread gathers the band below its node level by level (all
+// child ids of one level in one round of loads, with the range sizes that
+// tell which children lie above the groups), then the records of every child
+// that is already complete (group roots, leaves, nodes of the band below) in
+// one round, and forms the band's nodes bottom-up in registers: H levels cost
+// H + 2 dependent load rounds and one launch. Bottom-up order is what unite
+// (barnes_hut.cpp:137-152) needs: kernel boundaries order the bands.
+struct MomRec {
+  double4 com, vel;
+  float4 lo, hi;
+};
+
 template <bool VEL>
-__global__ __launch_bounds__(256) void k_moments_level(int n, const int32_t* __restrict__ bucket, int count,
-                                                       const int32_t* __restrict__ left,
-                                                       const int32_t* __restrict__ right, double4* __restrict__ nmc,
-                                                       double4* __restrict__ nvel, float4* __restrict__ nbox,
-                                                       float4* __restrict__ nrec) {
+__device__ __forceinline__ MomRec load_moments(int node, const double4* __restrict__ nmc,
+                                               const double4* __restrict__ nvel, const float4* __restrict__ nbox) {
+  MomRec r;
+  r.com = nmc[node];
+  r.vel = VEL ? nvel[node] : make_double4(0.0, 0.0, 0.0, 0.0);
+  r.lo = nbox[2 * node];
+  r.hi = nbox[2 * node + 1];
+  return r;
+}
+
+template <bool VEL>
+__device__ __forceinline__ MomRec unite_moments(const MomRec& cl, const MomRec& cr) {
 #pragma clang fp contract(off)
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= count) return;
-  const int node = bucket[t];
-  const int L = left[node], R = right[node];
-  const double4 cR = nmc[R], cL = nmc[L];
-  const double mR = cR.w, mL = cL.w;
+  const double mR = cr.com.w, mL = cl.com.w;
   const double total = mR + mL;
-  double4 com = make_double4(0.0, 0.0, 0.0, 0.0), vel = make_double4(0.0, 0.0, 0.0, 0.0);
+  MomRec me;
+  me.com = make_double4(0.0, 0.0, 0.0, 0.0);
+  me.vel = make_double4(0.0, 0.0, 0.0, 0.0);
   if (total > 0) {
-    com.w = total;
-    com.x = (cR.x * mR + cL.x * mL) / total;
-    com.y = (cR.y * mR + cL.y * mL) / total;
-    com.z = (cR.z * mR + cL.z * mL) / total;
+    me.com.w = total;
+    me.com.x = (cr.com.x * mR + cl.com.x * mL) / total;
+    me.com.y = (cr.com.y * mR + cl.com.y * mL) / total;
+    me.com.z = (cr.com.z * mR + cl.com.z * mL) / total;
     if (VEL) {
-      const double4 vR = nvel[R], vL = nvel[L];
-      vel.x = (vR.x * mR + vL.x * mL) / total;
-      vel.y = (vR.y * mR + vL.y * mL) / total;
-      vel.z = (vR.z * mR + vL.z * mL) / total;
+      me.vel.x = (cr.vel.x * mR + cl.vel.x * mL) / total;
+      me.vel.y = (cr.vel.y * mR + cl.vel.y * mL) / total;
+      me.vel.z = (cr.vel.z * mR + cl.vel.z * mL) / total;
     }
   }
-  nmc[node] = com;
-  if (VEL) nvel[node] = vel;
-  const float4 loL = nbox[2 * L], hiL = nbox[2 * L + 1], loR = nbox[2 * R], hiR = nbox[2 * R + 1];
-  nbox[2 * node] = make_float4(fminf(loL.x, loR.x), fminf(loL.y, loR.y), fminf(loL.z, loR.z), 0.f);
-  nbox[2 * node + 1] = make_float4(fmaxf(hiL.x, hiR.x), fmaxf(hiL.y, hiR.y), fmaxf(hiL.z, hiR.z), 0.f);
-  nrec[node] = make_float4((float)com.x, (float)com.y, (float)com.z, (float)com.w);
+  me.lo = make_float4(fminf(cl.lo.x, cr.lo.x), fminf(cl.lo.y, cr.lo.y), fminf(cl.lo.z, cr.lo.z), 0.f);
+  me.hi = make_float4(fmaxf(cl.hi.x, cr.hi.x), fmaxf(cl.hi.y, cr.hi.y), fmaxf(cl.hi.z, cr.hi.z), 0.f);
+  return me;
+}
+
+template <bool VEL, int H>
+__global__ __launch_bounds__(256) void k_moments_band(int n, const int32_t* __restrict__ bucket, int count,
+                                                      const int32_t* __restrict__ left,
+                                                      const int32_t* __restrict__ right,
+                                                      const int2* __restrict__ grange, double4* __restrict__ nmc,
+                                                      double4* __restrict__ nvel, float4* __restrict__ nbox,
+                                                      float4* __restrict__ nrec) {
+  constexpr int W = 1 << H;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  // id[l][k]: node at band level l (children of k at 2k, 2k + 1 of level
+  // l + 1), -1 when absent; in[l][k]: formed here (above the groups)
+  int id[H + 1][W];
+  bool in[H + 1][W];
+#pragma unroll
+  for (int l = 0; l <= H; ++l)
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      id[l][k] = -1;
+      in[l][k] = false;
+    }
+  id[0][0] = bucket[t];
+  in[0][0] = true;
+#pragma unroll
+  for (int l = 0; l < H; ++l) {
+#pragma unroll
+    for (int k = 0; k < (1 << l); ++k) {
+      if (in[l][k]) {
+        id[l + 1][2 * k] = left[id[l][k]];
+        id[l + 1][2 * k + 1] = right[id[l][k]];
+      }
+    }
+    if (l + 1 < H) {
+#pragma unroll
+      for (int k = 0; k < (2 << l); ++k) {
+        const int c = id[l + 1][k];
+        if (c >= n) {
+          const int2 r = grange[c - n];
+          in[l + 1][k] = r.y - r.x + 1 > kGroup;
+        }
+      }
+    }
+  }
+  // records of the complete children, all levels at once
+  MomRec rec[H + 1][W];
+#pragma unroll
+  for (int l = 1; l <= H; ++l)
+#pragma unroll
+    for (int k = 0; k < (1 << l); ++k)
+      if (id[l][k] >= 0 && !in[l][k]) rec[l][k] = load_moments<VEL>(id[l][k], nmc, nvel, nbox);
+  // bottom-up
+#pragma unroll
+  for (int l = H - 1; l >= 0; --l) {
+#pragma unroll
+    for (int k = 0; k < (1 << l); ++k) {
+      if (!in[l][k]) continue;
+      const int node = id[l][k];
+      const MomRec me = unite_moments<VEL>(rec[l + 1][2 * k], rec[l + 1][2 * k + 1]);
+      nmc[node] = me.com;
+      if (VEL) nvel[node] = me.vel;
+      nbox[2 * node] = me.lo;
+      nbox[2 * node + 1] = me.hi;
+      nrec[node] = make_float4((float)me.com.x, (float)me.com.y, (float)me.com.z, (float)me.com.w);
+      rec[l][k] = me;
+    }
+  }
 }
 
 // Moments of every internal node inside a traversal group, one wavefront per
@@ -416,7 +534,7 @@ __global__ __launch_bounds__(256) void k_moments_level(int n, const int32_t* __r
 // Lane t owns node n+base+t; the lanes sweep bottom-up, a node being ready
 // once both children are (leaves always are). Finished records stay in LDS
 // for the parents in the same wavefront; leaves come from k_build. Same
-// arithmetic as k_moments_level, so the results are the same bits.
+// arithmetic as unite_moments, so the results are the same bits.
 template <bool VEL>
 __global__ __launch_bounds__(256) void k_group_moments(int n, const int32_t* __restrict__ gstart,
                                                        const uint32_t* __restrict__ gcount,
@@ -715,11 +833,14 @@ __global__ __launch_bounds__(256) void k_bh_f64(int n, const double* __restrict_
 
 // Leaf k climbs while the parent still has at most kGroup leaves; k starts a
 // group when it is the first position of the subtree it stops at.
+// The climb's length (the leaf's depth below its group's root) and, at the
+// group's first position, the root itself are kept for the depth census.
 __global__ __launch_bounds__(256) void k_group_flags(int n, const int32_t* __restrict__ parent,
-                                                     const int2* __restrict__ grange, uint32_t* __restrict__ flag) {
+                                                     const int2* __restrict__ grange, uint32_t* __restrict__ flag,
+                                                     int32_t* __restrict__ climb, int32_t* __restrict__ groot) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  int v = k, lo = k;
+  int v = k, lo = k, steps = 0;
   for (;;) {
     const int p = parent[v];
     if (p < 0) break;
@@ -727,8 +848,11 @@ __global__ __launch_bounds__(256) void k_group_flags(int n, const int32_t* __res
     if (r.y - r.x + 1 > kGroup) break;
     v = p;
     lo = r.x;
+    ++steps;
   }
   flag[k] = (lo == k) ? 1u : 0u;
+  climb[k] = steps;
+  if (lo == k) groot[k] = v;
 }
 
 // From the exclusive scan of the flags: gstart[g] = first position of group g,
@@ -983,6 +1107,9 @@ static int reserve_tree(aster_ctx* c) {
   ASTER_HIP(c, c->depth.reserve(nn));
   ASTER_HIP(c, c->levels.reserve(kMaxLevels + 3));
   ASTER_HIP(c, c->bucket.reserve(n));
+  ASTER_HIP(c, c->splitnode.reserve(n));
+  ASTER_HIP(c, c->groot.reserve(n));
+  ASTER_HIP(c, c->climb.reserve(n));
   ASTER_HIP(c, c->dsorted.reserve(n));
   ASTER_HIP(c, c->grange.reserve(n));
   ASTER_HIP(c, c->nmc.reserve(nn));
@@ -1052,7 +1179,7 @@ static int build_groups(aster_ctx* c) {
   ASTER_HIP(c, c->gstart.reserve((size_t)n + 1));
   ASTER_HIP(c, c->gcount.reserve(1));
   hipLaunchKernelGGL(k_group_flags, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->parent.p, c->grange.p,
-                     c->gscan.p);
+                     c->gscan.p, c->climb.p, c->groot.p);
   int rc = device_scan(c, c->gscan.p, n, c->gcount.p);
   if (rc) return rc;
   hipLaunchKernelGGL(k_group_emit, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->gscan.p, c->gcount.p,
@@ -1061,7 +1188,32 @@ static int build_groups(aster_ctx* c) {
   return ASTER_OK;
 }
 
-static int build_tree(aster_ctx* c, int mac, bool vel) {
+constexpr int kBand = 3;  // levels per moments launch above the groups
+
+static int launch_moments_band(aster_ctx* c, bool vel, int h, int off, int cnt) {
+  const int n = (int)c->n;
+  const dim3 grid((cnt + 255) / 256), block(256);
+  const int32_t* b = c->bucket.p + off;
+#define ASTER_BAND(V, H)                                                                                          \
+  hipLaunchKernelGGL((k_moments_band<V, H>), grid, block, 0, c->stream, n, b, cnt, c->left.p, c->right.p,        \
+                     c->grange.p, c->nmc.p, c->nvel.p, c->nbox.p, c->nrec.p)
+  switch (h * 2 + (vel ? 1 : 0)) {
+    case 2: ASTER_BAND(false, 1); break;
+    case 3: ASTER_BAND(true, 1); break;
+    case 4: ASTER_BAND(false, 2); break;
+    case 5: ASTER_BAND(true, 2); break;
+    case 6: ASTER_BAND(false, 3); break;
+    case 7: ASTER_BAND(true, 3); break;
+    default: return fail(c, ASTER_ERR_STATE, "moments band height out of range");
+  }
+#undef ASTER_BAND
+  ASTER_HIP(c, hipGetLastError());
+  return ASTER_OK;
+}
+
+// full: also the depth and escape link of every node (the per-body
+// traversals and the reference MAC need them; the group traversal does not).
+static int build_tree(aster_ctx* c, int mac, bool vel, bool full) {
   const int n = (int)c->n;
   int rc = reserve_tree(c);
   if (rc) return rc;
@@ -1071,22 +1223,25 @@ static int build_tree(aster_ctx* c, int mac, bool vel) {
   auto build = vel ? k_build<true> : k_build<false>;
   hipLaunchKernelGGL(build, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->codes.p, c->idx.p, c->xm4.p,
                      c->v.p, c->left.p, c->right.p, c->parent.p, c->grange.p, c->nmc.p, c->nvel.p, c->nbox.p,
-                     c->nrec.p, c->gsh.p, c->flags.p);
+                     c->nrec.p, c->gsh.p, c->splitnode.p, c->codes2.p, c->idx2.p, c->flags.p);
   ASTER_HIP(c, hipGetLastError());
   // traversal groups (also the per-wavefront subtrees of the moment pass)
   if ((rc = build_groups(c))) return rc;
   phase_end(c, ASTER_PHASE_BUILD);
-  const int nn = 2 * n - 1, ni = n - 1;
+  const int nn = 2 * n - 1;
   phase_begin(c, ASTER_PHASE_MOMENTS);
-  // Depth census of the internal nodes above the groups: (depth, node) pairs,
-  // the in-group nodes keyed last, counting-sorted with one stable radix pass
-  // (depth < 128), so every level is a contiguous bucket. The sort scratch of
-  // the Morton pass is free here.
-  const int dblocks = (nn + 255) / 256;
-  ASTER_HIP(c, c->dmax.reserve((size_t)dblocks));
-  hipLaunchKernelGGL(k_depth, dim3(dblocks), dim3(256), 0, c->stream, n, c->left.p, c->right.p, c->parent.p,
-                     c->grange.p, c->depth.p, c->escape.p, c->codes2.p, c->idx2.p, c->dmax.p, c->flags.p);
-  hipLaunchKernelGGL(k_max_i32, dim3(1), dim3(1024), 0, c->stream, c->dmax.p, dblocks, c->levels.p + kMaxLevels + 2);
+  // Depth census of the nodes above the groups: their depths (walks from the
+  // group boundaries, and from the group roots for the largest depth) key
+  // (depth, node) pairs, the in-group nodes keyed last, counting-sorted with
+  // one stable radix pass (depth < 128), so every level is a contiguous
+  // bucket. The sort scratch of the Morton pass holds the pairs.
+  const int ni = n - 1;
+  const int cblocks = (int)std::min<int64_t>((2 * (int64_t)n / 32 + 2 + 255) / 256, 4096);
+  ASTER_HIP(c, c->dmax.reserve((size_t)cblocks));
+  hipLaunchKernelGGL(k_above_census, dim3(cblocks), dim3(256), 0, c->stream, n, c->gstart.p, c->gcount.p,
+                     c->splitnode.p, c->groot.p, c->climb.p, c->parent.p, c->depth.p, c->codes2.p, c->dmax.p,
+                     c->flags.p);
+  hipLaunchKernelGGL(k_max_i32, dim3(1), dim3(1024), 0, c->stream, c->dmax.p, cblocks, c->levels.p + kMaxLevels + 2);
   const int nblocks = (ni + kRsTile - 1) / kRsTile;
   hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(kRsThreads), 0, c->stream, c->codes2.p, ni, 0, c->hist.p,
                      nblocks);
@@ -1094,6 +1249,9 @@ static int build_tree(aster_ctx* c, int mac, bool vel) {
   hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(kRsThreads), 0, c->stream, c->codes2.p, c->idx2.p,
                      c->dsorted.p, reinterpret_cast<uint32_t*>(c->bucket.p), ni, 0, c->hist.p, nblocks);
   hipLaunchKernelGGL(k_level_bounds, dim3((ni + 255) / 256), dim3(256), 0, c->stream, ni, c->dsorted.p, c->levels.p);
+  if (full)
+    hipLaunchKernelGGL(k_depth, dim3((nn + 255) / 256), dim3(256), 0, c->stream, n, c->left.p, c->right.p,
+                       c->parent.p, c->depth.p, c->escape.p, c->flags.p);
   ASTER_HIP(c, hipGetLastError());
   // The level bounds and the largest depth go to pinned host memory
   // asynchronously; the host waits for them only after the group kernel below
@@ -1118,14 +1276,16 @@ static int build_tree(aster_ctx* c, int mac, bool vel) {
   if (nlev < 0 || nlev >= kMaxLevels || maxdep < nlev - 1 || maxdep >= kMaxLevels - 1 || (nlev > 0 && lv[0] != 0))
     return fail(c, ASTER_ERR_STATE, "radix tree depth census is inconsistent");
   c->tree_levels = maxdep + 1;
-  // the nodes above the groups, deepest level first (their children are
-  // group roots, leaves or nodes one level deeper)
-  auto moments = vel ? k_moments_level<true> : k_moments_level<false>;
-  for (int d = nlev - 1; d >= 0; --d) {
-    const int off = lv[d], cnt = lv[d + 1] - off;
+  // the nodes above the groups in bands of up to kBand levels, deepest band
+  // first (the children of a band's deepest nodes are group roots, leaves or
+  // nodes of the band below)
+  for (int d = nlev - 1; d >= 0;) {
+    const int top = d - kBand + 1 > 0 ? d - kBand + 1 : 0;
+    const int h = d - top + 1;
+    const int off = lv[top], cnt = lv[top + 1] - off;
     if (cnt <= 0) return fail(c, ASTER_ERR_STATE, "radix tree depth census has an empty level");
-    hipLaunchKernelGGL(moments, dim3((cnt + 255) / 256), dim3(256), 0, c->stream, n, c->bucket.p + off, cnt, c->left.p,
-                       c->right.p, c->nmc.p, c->nvel.p, c->nbox.p, c->nrec.p);
+    if ((rc = launch_moments_band(c, vel, h, off, cnt))) return rc;
+    d = top - 1;
   }
   ASTER_HIP(c, hipGetLastError());
   phase_end(c, ASTER_PHASE_MOMENTS);
@@ -1154,7 +1314,7 @@ int bh_forces(aster_ctx* c, int method) {
     return ASTER_OK;
   }
   const int mac = (method == ASTER_BH_PARITY) ? 0 : 1;
-  int rc = build_tree(c, mac, pn);
+  int rc = build_tree(c, mac, pn, method != ASTER_BH_GROUP);
   if (rc) return rc;
   phase_begin(c, ASTER_PHASE_TRAVERSE);
   const double ic = 1.0 / c->P.c;
