@@ -165,15 +165,20 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(uint32_t* __restric
 
 // Stable scatter: rank of a key among equal digits = earlier rounds (run[]),
 // earlier waves of this round (wc[][]), earlier lanes of its wave (ballot
-// match mask).
+// match mask). The tile is first reordered by digit in LDS, then written out
+// in that order, so consecutive lanes write consecutive addresses within
+// each digit's run.
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t* __restrict__ kin,
                                                            const uint32_t* __restrict__ vin,
                                                            uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                            int n, int shift, const uint32_t* __restrict__ hist,
                                                            int nblocks) {
+  constexpr int R = kRsTile / kRsThreads;
   __shared__ uint32_t run[256];
   __shared__ uint32_t gbase[256];
   __shared__ uint32_t wc[4][256];
+  __shared__ uint32_t lpre[256];
+  __shared__ uint32_t skey[kRsTile], sval[kRsTile];
   const int t = threadIdx.x;
   const int lane = t & 63, w = t >> 6;
   run[t] = 0;
@@ -183,12 +188,14 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t* __res
   __syncthreads();
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int base = blockIdx.x * kRsTile;
-  for (int r = 0; r < kRsTile / kRsThreads; ++r) {
+  uint32_t key[R], val[R], pos[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
     const int p = base + r * kRsThreads + t;
     const bool valid = p < n;
-    const uint32_t key = valid ? kin[p] : 0u;
-    const uint32_t val = valid ? vin[p] : 0u;
-    const uint32_t d = (key >> shift) & 255u;
+    key[r] = valid ? kin[p] : 0u;
+    val[r] = valid ? vin[p] : 0u;
+    const uint32_t d = (key[r] >> shift) & 255u;
     uint64_t m = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -199,18 +206,46 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t* __res
     const uint32_t rank = (uint32_t)__popcll(m & lt);
     if (valid && rank == 0) wc[w][d] = (uint32_t)__popcll(m);
     __syncthreads();
-    if (valid) {
-      uint32_t off = run[d];
-      for (int q = 0; q < w; ++q) off += wc[q][d];
-      const uint32_t pos = gbase[d] + off + rank;
-      kout[pos] = key;
-      vout[pos] = val;
-    }
+    uint32_t off = run[d];
+    for (int q = 0; q < w; ++q) off += wc[q][d];
+    pos[r] = valid ? off + rank : 0xFFFFFFFFu;  // rank among this tile's keys of digit d
     __syncthreads();
     run[t] += wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
 #pragma unroll
     for (int q = 0; q < 4; ++q) wc[q][t] = 0;
     __syncthreads();
+  }
+  // exclusive prefix of the tile's digit counts
+  {
+    const uint32_t c = run[t];
+    uint32_t inc = c;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(inc, off, 64);
+      if (lane >= off) inc += y;
+    }
+    if (lane == 63) wc[0][w] = inc;
+    __syncthreads();
+    uint32_t pre = inc - c;
+    for (int q = 0; q < w; ++q) pre += wc[0][q];
+    lpre[t] = pre;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (pos[r] != 0xFFFFFFFFu) {
+      const uint32_t slot = lpre[(key[r] >> shift) & 255u] + pos[r];
+      skey[slot] = key[r];
+      sval[slot] = val[r];
+    }
+  }
+  __syncthreads();
+  const int valid_here = min(kRsTile, n - base);
+  for (int j = t; j < valid_here; j += kRsThreads) {
+    const uint32_t k = skey[j];
+    const uint32_t d = (k >> shift) & 255u;
+    const uint32_t g = gbase[d] + ((uint32_t)j - lpre[d]);
+    kout[g] = k;
+    vout[g] = sval[j];
   }
 }
 
