@@ -53,6 +53,7 @@ NEWTON = 0x1
 PN1 = 0x10
 PN2 = 0x100
 PN25 = 0x1000
+EIH = 0x20
 # Profiling phases (enum aster_phase).
 PHASES = ["morton", "sort", "build", "moments", "links", "traverse", "allpairs", "combine", "kickdrift",
           "adjoint", "tiles", "phase11"]

@@ -205,6 +205,93 @@ __global__ __launch_bounds__(256) void k_allpairs_f64(int n, int lower, int uppe
   }
 }
 
+// ----------------------------------------------------------------- EIH ----
+//
+// Einstein-Infeld-Hoffmann 1PN accelerations (harmonic gauge) in the form of
+// Newhall, Standish & Williams (1983, A&A 125, 150, eq. 1) with beta = gamma
+// = 1, mu = G m, r from the softened r^2 = d.d + softening:
+//   a_i = sum_{j != i} mu_j d_ij / r^3 [1 - (4 phi_i + phi_j)/c^2
+//           + (v_i^2 + 2 v_j^2 - 4 v_i.v_j)/c^2 - 3/2 (d_ij.v_j / r)^2/c^2
+//           + d_ij.aN_j / (2 c^2)]
+//       + sum_{j != i} mu_j / (c^2 r^3) [-d_ij.(4 v_i - 3 v_j)] (v_i - v_j)
+//       + 7/(2 c^2) sum_{j != i} mu_j aN_j / r
+// with d_ij = x_j - x_i, phi_i = sum_{k != i} mu_k / r_ik and aN the
+// Newtonian accelerations. For two bodies it is the pairwise 1PN law.
+// Pass 1 forms phi and aN, pass 2 the accelerations; one thread per body,
+// sources in index order (the oracle's order, so the two agree bit for bit).
+__global__ __launch_bounds__(256) void k_eih_pass1(int n, const double* __restrict__ x, const double* __restrict__ m,
+                                                   double G, double soft, double* __restrict__ phi,
+                                                   double* __restrict__ an) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double p1x = x[i], p1y = x[n + i], p1z = x[2 * n + i];
+  double ph = 0.0, ax = 0.0, ay = 0.0, az = 0.0;
+  for (int j = 0; j < n; ++j) {
+    if (j == i) continue;
+    const double dx = x[j] - p1x, dy = x[n + j] - p1y, dz = x[2 * n + j] - p1z;
+    const double r2 = dx * dx + dy * dy + dz * dz + soft;
+    const double inv_r = 1.0 / sqrt(r2);
+    const double inv_r3 = inv_r * inv_r * inv_r;
+    const double mu = G * m[j];
+    ph += mu * inv_r;
+    ax += mu * dx * inv_r3;
+    ay += mu * dy * inv_r3;
+    az += mu * dz * inv_r3;
+  }
+  phi[i] = ph;
+  an[i] = ax;
+  an[n + i] = ay;
+  an[2 * n + i] = az;
+}
+
+__global__ __launch_bounds__(256) void k_eih_pass2(int n, const double* __restrict__ x, const double* __restrict__ v,
+                                                   const double* __restrict__ m, double G, double ic, double soft,
+                                                   const double* __restrict__ phi, const double* __restrict__ an,
+                                                   double* __restrict__ a) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double ic2 = ic * ic;
+  const double p1x = x[i], p1y = x[n + i], p1z = x[2 * n + i];
+  const double v1x = v[i], v1y = v[n + i], v1z = v[2 * n + i];
+  const double vi2 = v1x * v1x + v1y * v1y + v1z * v1z;
+  const double phi_i = phi[i];
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  for (int j = 0; j < n; ++j) {
+    if (j == i) continue;
+    const double dx = x[j] - p1x, dy = x[n + j] - p1y, dz = x[2 * n + j] - p1z;
+    const double r2 = dx * dx + dy * dy + dz * dz + soft;
+    const double inv_r = 1.0 / sqrt(r2);
+    const double inv_r3 = inv_r * inv_r * inv_r;
+    const double mu = G * m[j];
+    const double v2x = v[j], v2y = v[n + j], v2z = v[2 * n + j];
+    const double anx = an[j], any = an[n + j], anz = an[2 * n + j];
+    const double vj2 = v2x * v2x + v2y * v2y + v2z * v2z;
+    const double vij = v1x * v2x + v1y * v2y + v1z * v2z;
+    const double rv = (dx * v2x + dy * v2y + dz * v2z) * inv_r;
+    const double dan = dx * anx + dy * any + dz * anz;
+    const double B = 1.0 - (4.0 * phi_i + phi[j]) * ic2 + (vi2 + 2.0 * vj2 - 4.0 * vij) * ic2 -
+                     1.5 * (rv * rv) * ic2 + 0.5 * dan * ic2;
+    const double s1 = mu * inv_r3 * B;
+    const double w = -(dx * (4.0 * v1x - 3.0 * v2x) + dy * (4.0 * v1y - 3.0 * v2y) + dz * (4.0 * v1z - 3.0 * v2z));
+    const double s2 = mu * inv_r3 * ic2 * w;
+    const double s3 = 3.5 * mu * inv_r * ic2;
+    ax += dx * s1;
+    ax += (v1x - v2x) * s2;
+    ax += anx * s3;
+    ay += dy * s1;
+    ay += (v1y - v2y) * s2;
+    ay += any * s3;
+    az += dz * s1;
+    az += (v1z - v2z) * s2;
+    az += anz * s3;
+  }
+  a[i] = ax;
+  a[n + i] = ay;
+  a[2 * n + i] = az;
+}
+
 // --------------------------------------------------------------- launch ----
 
 int choose_split(aster_ctx* c) {
@@ -230,6 +317,21 @@ int launch_allpairs(aster_ctx* c, int method) {
   const int pnf = c->P.pn_flags & ASTER_PN_TERMS;
   const bool pn = pnf != 0;
   const double ic = 1.0 / c->P.c;
+  if (pnf & ASTER_EIH) {
+    if (method != ASTER_ALLPAIRS_FP64) return fail(c, ASTER_ERR_INVALID, "ASTER_EIH needs ASTER_ALLPAIRS_FP64");
+    if (lower != 0 || upper != n) return fail(c, ASTER_ERR_INVALID, "ASTER_EIH covers the full source range only");
+    ASTER_HIP(c, c->apn.reserve((size_t)4 * n));
+    double* an = c->apn.p;
+    double* phi = c->apn.p + 3 * (size_t)n;
+    const dim3 grid((n + 255) / 256);
+    phase_begin(c, ASTER_PHASE_ALLPAIRS);
+    hipLaunchKernelGGL(k_eih_pass1, grid, dim3(256), 0, c->stream, n, c->x.p, c->m.p, c->P.G, c->P.softening, phi, an);
+    hipLaunchKernelGGL(k_eih_pass2, grid, dim3(256), 0, c->stream, n, c->x.p, c->v.p, c->m.p, c->P.G, ic,
+                       c->P.softening, (const double*)phi, (const double*)an, c->a.p);
+    ASTER_HIP(c, hipGetLastError());
+    phase_end(c, ASTER_PHASE_ALLPAIRS);
+    return ASTER_OK;
+  }
   if (method == ASTER_ALLPAIRS_FP64) {
     dim3 grid((n + 255) / 256);
     phase_begin(c, ASTER_PHASE_ALLPAIRS);

@@ -171,7 +171,9 @@ int aster_set_params(aster_ctx* c, const aster_params* p) {
   if (p->precision != ASTER_FP32 && p->precision != ASTER_FP64)
     return fail(c, ASTER_ERR_INVALID, "precision must be ASTER_FP32 or ASTER_FP64");
   if ((p->pn_flags & ~(ASTER_NEWTON | ASTER_PN_TERMS)) != 0)
-    return fail(c, ASTER_ERR_INVALID, "force terms are ASTER_NEWTON, ASTER_PN1, ASTER_PN2 and ASTER_PN25");
+    return fail(c, ASTER_ERR_INVALID, "force terms are ASTER_NEWTON, ASTER_PN1, ASTER_PN2, ASTER_PN25 and ASTER_EIH");
+  if ((p->pn_flags & ASTER_EIH) && (p->pn_flags & (ASTER_PN1 | ASTER_PN2 | ASTER_PN25 | ASTER_NEWTON)) != ASTER_NEWTON)
+    return fail(c, ASTER_ERR_INVALID, "ASTER_EIH goes with ASTER_NEWTON alone (it replaces the pairwise terms)");
   const bool any = p->box[0] != 0.0 || p->box[1] != 0.0 || p->box[2] != 0.0;
   if (any && !(p->box[0] > 0.0 && p->box[1] > 0.0 && p->box[2] > 0.0))
     return fail(c, ASTER_ERR_INVALID, "box half-extents must all be > 0 (or all 0 for the automatic box)");

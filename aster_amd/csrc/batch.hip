@@ -559,6 +559,7 @@ static BatchArgs make_args(aster_ctx* c, int64_t nsteps, int nsub, const double*
 }
 
 static int launch_batch(aster_ctx* c, const BatchArgs& P) {
+  if (c->P.pn_flags & ASTER_EIH) return fail(c, ASTER_ERR_INVALID, "ASTER_EIH needs ASTER_ALLPAIRS_FP64 (single system)");
   const int wpb = c->batch.waves_per_block;
   const dim3 grid((P.nwaves + wpb - 1) / wpb), block(64 * wpb);
   const bool pn = (c->P.pn_flags & ASTER_PN_TERMS) != 0;
@@ -612,6 +613,7 @@ int batch_step(aster_ctx* c, int integrator, double dt, int64_t nsteps, int spl,
     return fail(c, ASTER_ERR_INVALID, "integrator must be ASTER_LEAPFROG, ASTER_YOSHIDA4 or ASTER_WH_PLANETARY");
   if (!std::isfinite(dt) || nsteps < 0 || spl < 0) return fail(c, ASTER_ERR_INVALID, "invalid dt, nsteps or spl");
   const bool pn = (c->P.pn_flags & ASTER_PN_TERMS) != 0;
+  if (c->P.pn_flags & ASTER_EIH) return fail(c, ASTER_ERR_INVALID, "ASTER_EIH needs ASTER_ALLPAIRS_FP64 (single system)");
   if (wh && pn) return fail(c, ASTER_ERR_INVALID, "ASTER_WH_PLANETARY is Newtonian: post-Newtonian terms are not supported");
   if ((c->P.pn_flags & (ASTER_PN2 | ASTER_PN25)) && c->P.precision != ASTER_FP64)
     return fail(c, ASTER_ERR_INVALID, "ASTER_PN2 / ASTER_PN25 in ensembles need precision ASTER_FP64 (exact mode)");

@@ -1339,6 +1339,7 @@ int bh_forces(aster_ctx* c, int method) {
   const int pnf = c->P.pn_flags & ASTER_PN_TERMS;
   const bool pn = pnf != 0;
   const bool f64 = method == ASTER_BH_PARITY && c->P.precision == ASTER_FP64;
+  if (pnf & ASTER_EIH) return fail(c, ASTER_ERR_INVALID, "ASTER_EIH needs ASTER_ALLPAIRS_FP64");
   if (pn && !f64)
     return fail(c, ASTER_ERR_INVALID, "Barnes-Hut with post-Newtonian terms needs ASTER_BH_PARITY with precision ASTER_FP64");
   if (!(c->P.theta >= 0.0)) return fail(c, ASTER_ERR_INVALID, "theta must be >= 0");

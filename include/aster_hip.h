@@ -45,8 +45,14 @@ enum aster_status {
 #define ASTER_PN1 0x10
 #define ASTER_PN2 0x100
 #define ASTER_PN25 0x1000
+/* Full N-body 1PN (Einstein-Infeld-Hoffmann) accelerations, including the
+ * cross terms the pairwise ASTER_PN1 leaves out; no counterpart in the
+ * reference (its get_pn1 is the two-body closed form summed over pairs,
+ * tool_chain_impl.cpp:110-118). Replaces ASTER_PN1/PN2/PN25 (combining them
+ * is an error); ASTER_ALLPAIRS_FP64 only. */
+#define ASTER_EIH 0x20
 /* every post-Newtonian term (velocity-dependent forces) */
-#define ASTER_PN_TERMS (ASTER_PN1 | ASTER_PN2 | ASTER_PN25)
+#define ASTER_PN_TERMS (ASTER_PN1 | ASTER_PN2 | ASTER_PN25 | ASTER_EIH)
 
 /* Force methods. */
 enum aster_method {
@@ -95,7 +101,7 @@ typedef struct aster_params {
   double theta;     /* opening parameter (reference default 0.2) */
   double box[3];    /* Morton box half-extent (Barnes_Hut::bounding_box);
                        all zero = symmetric max |coordinate| of the state */
-  int32_t pn_flags; /* ASTER_NEWTON | ASTER_PN1 | ASTER_PN2 | ASTER_PN25 */
+  int32_t pn_flags; /* ASTER_NEWTON | ASTER_PN1 | ASTER_PN2 | ASTER_PN25, or ASTER_NEWTON | ASTER_EIH */
   int32_t precision;/* ASTER_FP32 / ASTER_FP64 for ASTER_BH_PARITY */
   int32_t lower;    /* source range [lower, upper) for all-pairs, as */
   int32_t upper;    /* Solver::set_bounds (sim_obj.h:45); upper < 0 = n */

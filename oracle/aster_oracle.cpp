@@ -50,6 +50,7 @@ constexpr int kNewton = 0x1;   // basic.h:39
 constexpr int kPN1 = 0x10;     // basic.h:40
 constexpr int kPN2 = 0x100;    // basic.h:41
 constexpr int kPN25 = 0x1000;  // basic.h:42
+constexpr int kEIH = 0x20;     // full N-body 1PN; no counterpart in the reference
 
 struct V3 {
   double x, y, z;
@@ -203,8 +204,15 @@ inline V3 gravitational(double m1, double m2, V3 v1, V3 v2, V3 p1, V3 p2, const 
 // Parallelized::compute_forces, builder_endpoint.cpp:729-757: every target i in
 // [0,n) sums F(i,j)/m_i over sources j in [lower, upper), j != i, in j order,
 // then assigns accs[i].
+void eih(int64_t n, const double* pos, const double* vel, const double* mass, const Law& L, double* acc,
+         int nthreads);
+
 void allpairs(int64_t n, const double* pos, const double* vel, const double* mass, const Law& L,
               int64_t lower, int64_t upper, double* acc, int nthreads) {
+  if (L.pn & kEIH) {  // the full source range (orc_allpairs rejects bounds)
+    eih(n, pos, vel, mass, L, acc, nthreads);
+    return;
+  }
   if (upper < 0) upper = n;
   parallel_for(n, nthreads, [&](int64_t i) {
     V3 a{0.0, 0.0, 0.0};
@@ -218,6 +226,83 @@ void allpairs(int64_t n, const double* pos, const double* vel, const double* mas
       a.z += f.z / m1;
     }
     store3(acc, i, a);
+  });
+}
+
+// Einstein-Infeld-Hoffmann 1PN accelerations (no counterpart in the
+// reference, whose get_pn1 is the two-body law summed over pairs), in the
+// form of Newhall, Standish & Williams (1983, A&A 125, 150, eq. 1) with
+// beta = gamma = 1, mu = G m, r from the softened r^2 = d.d + softening:
+//   a_i = sum_{j != i} mu_j d_ij / r^3 [1 - (4 phi_i + phi_j)/c^2
+//           + (v_i^2 + 2 v_j^2 - 4 v_i.v_j)/c^2 - 3/2 (d_ij.v_j / r)^2/c^2
+//           + d_ij.aN_j / (2 c^2)]
+//       + sum_{j != i} mu_j / (c^2 r^3) [-d_ij.(4 v_i - 3 v_j)] (v_i - v_j)
+//       + 7/(2 c^2) sum_{j != i} mu_j aN_j / r
+// with d_ij = x_j - x_i, phi_i = sum_{k != i} mu_k / r_ik and aN the Newtonian
+// accelerations; sources in index order. Pinned by: for two bodies it equals
+// the pairwise 1PN law (Blanchet); an independent numpy evaluation of the
+// same published equation; conservation of the EIH (Landau-Lifshitz sec. 106)
+// 1PN energy along trajectories to O(1/c^4).
+void eih(int64_t n, const double* pos, const double* vel, const double* mass, const Law& L, double* acc,
+         int nthreads) {
+  std::vector<double> phi(n), an(3 * n);
+  parallel_for(n, nthreads, [&](int64_t i) {
+    const V3 p1 = load3(pos, i);
+    double ph = 0.0, ax = 0.0, ay = 0.0, az = 0.0;
+    for (int64_t j = 0; j < n; ++j) {
+      if (j == i) continue;
+      const V3 p2 = load3(pos, j);
+      const double dx = p2.x - p1.x, dy = p2.y - p1.y, dz = p2.z - p1.z;
+      const double r2 = dx * dx + dy * dy + dz * dz + L.soft;
+      const double inv_r = 1.0 / std::sqrt(r2);
+      const double inv_r3 = inv_r * inv_r * inv_r;
+      const double mu = L.G * mass[j];
+      ph += mu * inv_r;
+      ax += mu * dx * inv_r3;
+      ay += mu * dy * inv_r3;
+      az += mu * dz * inv_r3;
+    }
+    phi[i] = ph;
+    an[3 * i] = ax;
+    an[3 * i + 1] = ay;
+    an[3 * i + 2] = az;
+  });
+  const double ic = 1.0 / L.c;
+  const double ic2 = ic * ic;
+  parallel_for(n, nthreads, [&](int64_t i) {
+    const V3 p1 = load3(pos, i), v1 = load3(vel, i);
+    const double vi2 = v1.x * v1.x + v1.y * v1.y + v1.z * v1.z;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int64_t j = 0; j < n; ++j) {
+      if (j == i) continue;
+      const V3 p2 = load3(pos, j), v2 = load3(vel, j);
+      const double dx = p2.x - p1.x, dy = p2.y - p1.y, dz = p2.z - p1.z;
+      const double r2 = dx * dx + dy * dy + dz * dz + L.soft;
+      const double inv_r = 1.0 / std::sqrt(r2);
+      const double inv_r3 = inv_r * inv_r * inv_r;
+      const double mu = L.G * mass[j];
+      const double anx = an[3 * j], any = an[3 * j + 1], anz = an[3 * j + 2];
+      const double vj2 = v2.x * v2.x + v2.y * v2.y + v2.z * v2.z;
+      const double vij = v1.x * v2.x + v1.y * v2.y + v1.z * v2.z;
+      const double rv = (dx * v2.x + dy * v2.y + dz * v2.z) * inv_r;
+      const double dan = dx * anx + dy * any + dz * anz;
+      const double B = 1.0 - (4.0 * phi[i] + phi[j]) * ic2 + (vi2 + 2.0 * vj2 - 4.0 * vij) * ic2 -
+                       1.5 * (rv * rv) * ic2 + 0.5 * dan * ic2;
+      const double s1 = mu * inv_r3 * B;
+      const double w = -(dx * (4.0 * v1.x - 3.0 * v2.x) + dy * (4.0 * v1.y - 3.0 * v2.y) + dz * (4.0 * v1.z - 3.0 * v2.z));
+      const double s2 = mu * inv_r3 * ic2 * w;
+      const double s3 = 3.5 * mu * inv_r * ic2;
+      ax += dx * s1;
+      ax += (v1.x - v2.x) * s2;
+      ax += anx * s3;
+      ay += dy * s1;
+      ay += (v1.y - v2.y) * s2;
+      ay += any * s3;
+      az += dz * s1;
+      az += (v1.z - v2.z) * s2;
+      az += anz * s3;
+    }
+    store3(acc, i, V3{ax, ay, az});
   });
 }
 
@@ -427,6 +512,7 @@ int orc_version() { return 1; }
 int orc_allpairs(int64_t n, const double* pos, const double* vel, const double* mass, double G, double c,
                  double soft, int pn, int64_t lower, int64_t upper, double* acc, int nthreads) {
   if (n < 0 || !pos || !vel || !mass || !acc) return -1;
+  if ((pn & kEIH) && (lower != 0 || (upper >= 0 && upper != n))) return -1;
   allpairs(n, pos, vel, mass, Law{G, c, soft, pn}, lower, upper, acc, nthreads);
   return 0;
 }

@@ -1,0 +1,68 @@
+"""Full N-body 1PN (EIH) force law on the device: bit-identical to the
+oracle's restatement (the same sums in the same order, FP64), forces and
+trajectories; the methods that do not support it reject it."""
+import numpy as np
+import pytest
+
+import aster_amd as A
+
+pytestmark = pytest.mark.gpu
+
+EIH = A.NEWTON | A.EIH
+
+
+def cluster(n, seed):
+    r = np.random.default_rng(seed)
+    return r.normal(size=(n, 3)), r.normal(scale=0.5, size=(n, 3)), r.uniform(0.5, 1.5, n) / n
+
+
+@pytest.mark.parametrize("n,soft", [(2, 0.0), (3, 0.0), (100, 1e-4), (1000, 0.0), (4099, 1e-4)])
+def test_eih_forces_bit_identical(ctx, orc, n, soft):
+    pos, vel, mass = cluster(n, n)
+    ctx.set_params(G=1.0, c=8.0, softening=soft, pn_flags=EIH)
+    ctx.set_state(pos, vel, mass)
+    acc = ctx.forces(A.ALLPAIRS_FP64)
+    ref = orc.allpairs(pos, vel, mass, G=1.0, c=8.0, soft=soft, pn=0x21)
+    assert np.array_equal(acc, ref)
+
+
+@pytest.mark.parametrize("integrator,oi", [(A.LEAPFROG, 2), (A.YOSHIDA4, 5)])
+def test_eih_trajectory_bit_identical(ctx, orc, integrator, oi):
+    pos, vel, mass = cluster(64, 5)
+    ctx.set_params(G=1.0, c=8.0, softening=1e-4, pn_flags=EIH)
+    ctx.set_state(pos, vel, mass)
+    ctx.step(A.ALLPAIRS_FP64, integrator, 1e-3, 6)
+    p, v = ctx.get_state()
+    rp, rv, _ = orc.integrate(pos, vel, mass, 1e-3, 6, integrator=oi, c=8.0, soft=1e-4, pn=0x21)
+    assert np.array_equal(p, rp) and np.array_equal(v, rv)
+
+
+def test_eih_saba_bit_identical(ctx, orc):
+    pos, vel, mass = cluster(48, 6)
+    ctx.set_params(G=1.0, c=8.0, softening=1e-4, pn_flags=EIH)
+    ctx.set_state(pos, vel, mass)
+    ctx.integrate(A.ALLPAIRS_FP64, A.SABA, 4, 1e-3, 3)
+    p, v = ctx.get_state()
+    rp, rv, _ = orc.integrate(pos, vel, mass, 1e-3, 3, integrator=1, order=4, c=8.0, soft=1e-4, pn=0x21)
+    assert np.array_equal(p, rp) and np.array_equal(v, rv)
+
+
+def test_eih_rejections(ctx):
+    pos, vel, mass = cluster(300, 7)
+    with pytest.raises(A.AsterError):
+        ctx.set_params(pn_flags=A.NEWTON | A.EIH | A.PN1)
+    ctx.set_params(G=1.0, c=8.0, softening=1e-4, pn_flags=EIH, precision=A.FP64)
+    ctx.set_state(pos, vel, mass)
+    for method in (A.ALLPAIRS_FP32, A.ALLPAIRS_FP32_KAHAN, A.BH_PARITY, A.BH_GROUP):
+        with pytest.raises(A.AsterError):
+            ctx.forces(method)
+    ctx.set_params(G=1.0, c=8.0, softening=1e-4, pn_flags=EIH, lower=10, upper=200)
+    with pytest.raises(A.AsterError):
+        ctx.forces(A.ALLPAIRS_FP64)
+    ctx.set_params(G=1.0, c=8.0, softening=1e-4, pn_flags=EIH, lower=0, upper=-1)
+    off = np.array([0, 50, 100])
+    ctx.batch_set(off, pos[:100], vel[:100], mass[:100])
+    with pytest.raises(A.AsterError):
+        ctx.batch_step(A.LEAPFROG, 1e-3, 2)
+    with pytest.raises(A.AsterError):
+        ctx.traj_forward(1e-3, 4, 2, [4])
