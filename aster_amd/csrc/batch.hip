@@ -102,6 +102,79 @@ __device__ __forceinline__ void batch_accel(int lane, int base, int cnt, int max
   A[2] = az;
 }
 
+// Full N-body 1PN (EIH) accelerations of a lane's body within its system
+// (exact mode): pass 1 forms each body's potential and Newtonian
+// acceleration and publishes them in LDS; pass 2 is the EIH sum, in the
+// order and arithmetic of the oracle's EIH (sources in system order), so a
+// system's trajectory equals the oracle's bit for bit.
+__device__ __forceinline__ void batch_eih(int lane, int base, int cnt, int maxcnt, bool live, const double* lx,
+                                          const double* ly, const double* lz, const double* lm, const double* lvx,
+                                          const double* lvy, const double* lvz, const double* X, const double* V,
+                                          double G, double ic, double soft, double* lphi, double* lanx, double* lany,
+                                          double* lanz, double* A) {
+#pragma clang fp contract(off)
+  double ph = 0.0, nx = 0.0, ny = 0.0, nz = 0.0;
+  for (int j = 0; j < maxcnt; ++j) {
+    const int src = min(base + j, 63);
+    const bool valid = live && j < cnt && src != lane;
+    const double dx = lx[src] - X[0], dy = ly[src] - X[1], dz = lz[src] - X[2];
+    const double r2 = dx * dx + dy * dy + dz * dz + soft;
+    const double inv_r = 1.0 / sqrt(r2);
+    const double inv_r3 = inv_r * inv_r * inv_r;
+    const double mu = G * lm[src];
+    if (valid) {
+      ph += mu * inv_r;
+      nx += mu * dx * inv_r3;
+      ny += mu * dy * inv_r3;
+      nz += mu * dz * inv_r3;
+    }
+  }
+  lphi[lane] = ph;
+  lanx[lane] = nx;
+  lany[lane] = ny;
+  lanz[lane] = nz;
+  wave_sync();
+  const double ic2 = ic * ic;
+  const double vi2 = V[0] * V[0] + V[1] * V[1] + V[2] * V[2];
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  for (int j = 0; j < maxcnt; ++j) {
+    const int src = min(base + j, 63);
+    const bool valid = live && j < cnt && src != lane;
+    const double dx = lx[src] - X[0], dy = ly[src] - X[1], dz = lz[src] - X[2];
+    const double r2 = dx * dx + dy * dy + dz * dz + soft;
+    const double inv_r = 1.0 / sqrt(r2);
+    const double inv_r3 = inv_r * inv_r * inv_r;
+    const double mu = G * lm[src];
+    const double v2x = lvx[src], v2y = lvy[src], v2z = lvz[src];
+    const double anx = lanx[src], any = lany[src], anz = lanz[src];
+    const double vj2 = v2x * v2x + v2y * v2y + v2z * v2z;
+    const double vij = V[0] * v2x + V[1] * v2y + V[2] * v2z;
+    const double rv = (dx * v2x + dy * v2y + dz * v2z) * inv_r;
+    const double dan = dx * anx + dy * any + dz * anz;
+    const double B = 1.0 - (4.0 * ph + lphi[src]) * ic2 + (vi2 + 2.0 * vj2 - 4.0 * vij) * ic2 - 1.5 * (rv * rv) * ic2 +
+                     0.5 * dan * ic2;
+    const double s1 = mu * inv_r3 * B;
+    const double w = -(dx * (4.0 * V[0] - 3.0 * v2x) + dy * (4.0 * V[1] - 3.0 * v2y) + dz * (4.0 * V[2] - 3.0 * v2z));
+    const double s2 = mu * inv_r3 * ic2 * w;
+    const double s3 = 3.5 * mu * inv_r * ic2;
+    if (valid) {
+      ax += dx * s1;
+      ax += (V[0] - v2x) * s2;
+      ax += anx * s3;
+      ay += dy * s1;
+      ay += (V[1] - v2y) * s2;
+      ay += any * s3;
+      az += dz * s1;
+      az += (V[2] - v2z) * s2;
+      az += anz * s3;
+    }
+  }
+  wave_sync();  // the next evaluation rewrites the LDS potentials
+  A[0] = ax;
+  A[1] = ay;
+  A[2] = az;
+}
+
 __device__ __forceinline__ void kick_drift_reg(double* X, double* V, const double* A, double dt_prev, double dt) {
 #pragma clang fp contract(off)
 #pragma unroll
@@ -123,6 +196,8 @@ template <bool PN, bool EXACT>
 __global__ __launch_bounds__(256) void k_batch(BatchArgs P) {
   __shared__ double sx[kBatchMaxWpb][64], sy[kBatchMaxWpb][64], sz[kBatchMaxWpb][64], sm[kBatchMaxWpb][64];
   __shared__ double svx[PN ? kBatchMaxWpb : 1][64], svy[PN ? kBatchMaxWpb : 1][64], svz[PN ? kBatchMaxWpb : 1][64];
+  constexpr int kE = (PN && EXACT) ? kBatchMaxWpb : 1;  // EIH potentials and Newtonian accelerations
+  __shared__ double sphi[kE][64], sanx[kE][64], sany[kE][64], sanz[kE][64];
   const int wl = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int wave = blockIdx.x * (blockDim.x >> 6) + wl;
   if (wave >= P.nwaves) return;
@@ -160,10 +235,20 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs P) {
     }
     wave_sync();
   };
+  const bool eih = PN && EXACT && (P.pnf & ASTER_EIH);
+  auto accel = [&]() {
+    if (eih) {
+      const int e = (PN && EXACT) ? wl : 0;
+      batch_eih(lane, base, cnt, maxcnt, live, lx, ly, lz, sm[wl], lvx, lvy, lvz, X, V, P.G, P.ic, P.soft, sphi[e],
+                sanx[e], sany[e], sanz[e], A);
+    } else {
+      batch_accel<PN, EXACT>(lane, base, cnt, maxcnt, live, lx, ly, lz, sm[wl], lvx, lvy, lvz, X, V, mi, P.G, P.ic2,
+                             P.ic, P.pnf, P.soft, A);
+    }
+  };
   if (P.init) {
     publish();
-    batch_accel<PN, EXACT>(lane, base, cnt, maxcnt, live, lx, ly, lz, sm[wl], lvx, lvy, lvz, X, V, mi, P.G, P.ic2,
-                           P.ic, P.pnf, P.soft, A);
+    accel();
     wave_sync();
   }
   double pending = 0.0;
@@ -176,14 +261,12 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs P) {
         if (pending != 0.0) kick_reg(V, A, pending);
         pending = 0.0;
         publish();
-        batch_accel<PN, EXACT>(lane, base, cnt, maxcnt, live, lx, ly, lz, sm[wl], lvx, lvy, lvz, X, V, mi, P.G,
-                               P.ic2, P.ic, P.pnf, P.soft, A);
+        accel();
         wave_sync();
       }
       kick_drift_reg(X, V, A, pending, h);
       publish();
-      batch_accel<PN, EXACT>(lane, base, cnt, maxcnt, live, lx, ly, lz, sm[wl], lvx, lvy, lvz, X, V, mi, P.G,
-                             P.ic2, P.ic, P.pnf, P.soft, A);
+      accel();
       wave_sync();
       pending = h;
     }
@@ -559,7 +642,8 @@ static BatchArgs make_args(aster_ctx* c, int64_t nsteps, int nsub, const double*
 }
 
 static int launch_batch(aster_ctx* c, const BatchArgs& P) {
-  if (c->P.pn_flags & ASTER_EIH) return fail(c, ASTER_ERR_INVALID, "ASTER_EIH needs ASTER_ALLPAIRS_FP64 (single system)");
+  if ((c->P.pn_flags & ASTER_EIH) && c->P.precision != ASTER_FP64)
+    return fail(c, ASTER_ERR_INVALID, "ASTER_EIH in ensembles needs precision ASTER_FP64 (exact mode)");
   const int wpb = c->batch.waves_per_block;
   const dim3 grid((P.nwaves + wpb - 1) / wpb), block(64 * wpb);
   const bool pn = (c->P.pn_flags & ASTER_PN_TERMS) != 0;
@@ -613,7 +697,8 @@ int batch_step(aster_ctx* c, int integrator, double dt, int64_t nsteps, int spl,
     return fail(c, ASTER_ERR_INVALID, "integrator must be ASTER_LEAPFROG, ASTER_YOSHIDA4 or ASTER_WH_PLANETARY");
   if (!std::isfinite(dt) || nsteps < 0 || spl < 0) return fail(c, ASTER_ERR_INVALID, "invalid dt, nsteps or spl");
   const bool pn = (c->P.pn_flags & ASTER_PN_TERMS) != 0;
-  if (c->P.pn_flags & ASTER_EIH) return fail(c, ASTER_ERR_INVALID, "ASTER_EIH needs ASTER_ALLPAIRS_FP64 (single system)");
+  if ((c->P.pn_flags & ASTER_EIH) && c->P.precision != ASTER_FP64)
+    return fail(c, ASTER_ERR_INVALID, "ASTER_EIH in ensembles needs precision ASTER_FP64 (exact mode)");
   if (wh && pn) return fail(c, ASTER_ERR_INVALID, "ASTER_WH_PLANETARY is Newtonian: post-Newtonian terms are not supported");
   if ((c->P.pn_flags & (ASTER_PN2 | ASTER_PN25)) && c->P.precision != ASTER_FP64)
     return fail(c, ASTER_ERR_INVALID, "ASTER_PN2 / ASTER_PN25 in ensembles need precision ASTER_FP64 (exact mode)");

@@ -59,10 +59,32 @@ def test_eih_rejections(ctx):
     ctx.set_params(G=1.0, c=8.0, softening=1e-4, pn_flags=EIH, lower=10, upper=200)
     with pytest.raises(A.AsterError):
         ctx.forces(A.ALLPAIRS_FP64)
-    ctx.set_params(G=1.0, c=8.0, softening=1e-4, pn_flags=EIH, lower=0, upper=-1)
+    ctx.set_params(G=1.0, c=8.0, softening=1e-4, pn_flags=EIH, lower=0, upper=-1, precision=A.FP32)
     off = np.array([0, 50, 100])
     ctx.batch_set(off, pos[:100], vel[:100], mass[:100])
-    with pytest.raises(A.AsterError):
+    with pytest.raises(A.AsterError):  # ensembles take EIH in exact mode only
         ctx.batch_step(A.LEAPFROG, 1e-3, 2)
     with pytest.raises(A.AsterError):
+        ctx.batch_step(A.WH_PLANETARY, 1e-3, 2)
+    with pytest.raises(A.AsterError):
         ctx.traj_forward(1e-3, 4, 2, [4])
+
+
+def test_eih_ensemble_exact_bit_identical(ctx, orc):
+    # planetary ensembles (central mass 1) with EIH in exact mode: every
+    # system's trajectory equals the oracle's EIH trajectory bit for bit
+    off, pos, vel, mass = A.ic_ensemble(40, 12, 3, 64)
+    ctx.set_params(G=1.0, c=30.0, softening=0.0, pn_flags=EIH, precision=A.FP64)
+    ctx.batch_set(off, pos, vel, mass)
+    ctx.batch_step(A.YOSHIDA4, 1e-3, 7, steps_per_launch=3, use_graph=False)
+    p, v = ctx.batch_get()
+    for s in range(40):
+        a, b = off[s], off[s + 1]
+        rp, rv, _ = orc.integrate(pos[a:b], vel[a:b], mass[a:b], 1e-3, 7, integrator=5, G=1.0, c=30.0, soft=0.0,
+                                  pn=0x21)
+        assert np.array_equal(p[a:b], rp) and np.array_equal(v[a:b], rv), s
+    # graph replay gives the same bits
+    ctx.batch_set(off, pos, vel, mass)
+    ctx.batch_step(A.YOSHIDA4, 1e-3, 7, steps_per_launch=7, use_graph=True)
+    p2, v2 = ctx.batch_get()
+    assert np.array_equal(p, p2) and np.array_equal(v, v2)
