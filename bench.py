@@ -288,13 +288,41 @@ def bench_batch(A, ctx, nsys=8192, steps=10000, dt=1e-3):
         wh[name] = {"ms_per_step": tw / nc * 1e3, "dE_rel_median": float(np.median(d)),
                     "dE_rel_p90": float(np.quantile(d, 0.9))}
     log(f"[bench] Newtonian ensemble, {nc} steps dt={dtc}: {wh}")
+    # The same ensemble with the full N-body 1PN (EIH) law in exact FP64
+    # mode, 1000 steps, against the pairwise 1PN law in exact mode: cost, and
+    # how far the cross terms move the bodies at c = 1e4.
+    ne = 1000
+    eih = {}
+    for name, flags in (("pairwise_1pn_exact", A.NEWTON | A.PN1), ("eih_exact", A.NEWTON | A.EIH)):
+        ctx.set_params(G=1.0, softening=0.0, c=1e4, pn_flags=flags, precision=A.FP64)
+        ctx.batch_set(off, pos, vel, mass)
+        en0, _ = ctx.batch_energy()
+        ctx.batch_step(A.YOSHIDA4, dt, 0)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.batch_step(A.YOSHIDA4, dt, ne, steps_per_launch=spl, use_graph=True)
+        ctx.sync()
+        tw = time.perf_counter() - t0
+        en1, _ = ctx.batch_energy()
+        pe, _ = ctx.batch_get()
+        d = np.abs(en1 - en0) / np.abs(en0)
+        eih[name] = {"ms_per_step": tw / ne * 1e3, "pair_interactions_per_s": pairs * ne * 3 * 2 / tw,
+                     "dE_rel_median": float(np.median(d)), "positions": pe}
+    dx = np.abs(eih["eih_exact"].pop("positions") - eih["pairwise_1pn_exact"].pop("positions")).max(1)
+    per_sys = np.maximum.reduceat(dx, off[:-1])  # largest |dx| of each system
+    # an unsoftened close encounter (the maximum) amplifies any difference
+    eih["per_system_max_abs_dx_eih_vs_pairwise"] = {"median": float(np.median(per_sys)),
+                                                    "p99": float(np.quantile(per_sys, 0.99)),
+                                                    "max": float(per_sys.max())}
+    log(f"[bench] ensemble EIH vs pairwise 1PN, exact mode, {ne} steps: {eih}")
     return {"systems": nsys, "bodies": int(off[-1]), "steps": steps, "wall_s": wall,
             "system_steps_per_s": nsys * steps / wall, "pair_interactions_per_s": inter / wall,
             "waves_per_block": wpb, "steps_per_launch": spl,
             "energy_drift": {"median": float(np.median(de)), "p90": float(np.quantile(de, 0.9)),
                              "max": float(de.max())},
             "L_drift_max": float((np.abs(L1 - L0).max(1) / np.abs(L0).max(1)).max()),
-            "newtonian_wh_vs_yoshida_dt0.02_500_steps": wh}
+            "newtonian_wh_vs_yoshida_dt0.02_500_steps": wh,
+            "eih_vs_pairwise_1pn_exact_1000_steps": eih}
 
 
 def bench_ic(A, n=1 << 24, host_n=1 << 20, reps=3):
