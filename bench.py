@@ -27,6 +27,20 @@ sys.path.insert(0, ROOT)
 
 METRIC = "pair interactions/s (all-pairs N=64K, BH N=1M); adjoint/forward step time ratio"
 FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
+HBM_PEAK_GBPS = 8000.0    # MI355X published HBM3E bandwidth
+
+# Algorithmic HBM bytes per body of the bandwidth-bound phases of a BH step
+# (what each phase must read and write once):
+#  morton: box pass 24 B (FP64 x) + Morton pass 24 B x + 8 B m read, 4 B code
+#          + 4 B index + 32 B {x, m} record written;
+#  sort: 4 passes x (4 B histogram read + 8 B key/value read + 8 B written);
+#  build: 4 B code + 4 B permutation + 32 B record read, 96 B of leaf records
+#         (FP64 {x, m}, FP32 box and record, FP32 sorted body) and 24 B of
+#         internal-node links (left, right, parent, range, split) written,
+#         plus the group pass's 12 B (start flag, climb, root);
+#  kickdrift: FP64 x, v, a read (72 B), x, v written (48 B).
+PHASE_BYTES_PER_BODY = {"morton": 24 + 24 + 8 + 4 + 4 + 32, "sort": 4 * (4 + 8 + 8), "build": 4 + 4 + 32 + 96 + 24 + 12,
+                        "kickdrift": 72 + 48}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -459,6 +473,10 @@ def main():
         "node_visits_per_step": visits_per_eval,
         "force_error_vs_fp64_allpairs": bh["errors"],
         "phases_ms": phases,
+        "hbm_phases": {k: {"bytes_per_body": b,
+                           "achieved_GBps": b * bh["n"] / (phases[k]["ms_per_launch"] * 1e-3) / 1e9,
+                           "frac_of_8TBps": b * bh["n"] / (phases[k]["ms_per_launch"] * 1e-3) / 1e9 / HBM_PEAK_GBPS}
+                       for k, b in PHASE_BYTES_PER_BODY.items() if k in phases},
         "event_ms_per_step": bh["event_ms"] / args.steps,
         "allpairs": ap_res,
         "integrators": integ,
