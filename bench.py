@@ -212,20 +212,59 @@ def bench_integrators(A, ctx, n=65536, dt=2.5e-4, steps=1000, steps64=20):
     dx = float(np.abs(p32 - p64).max())
     log(f"[bench] Yoshida-4 n={n}: mixed {steps} steps {mixed}; {steps64} steps mixed {mixed_short} "
         f"fp64 {fp64_short}, max |dx| {dx:.2e}")
-    fam = {}
+    fam, fam_drift = {}, {}
+    nfam = 40
     for name, integ, order in (("leapfrog", A.LEAPFROG, 0), ("yoshida4", A.YOSHIDA4, 0), ("saba4", A.SABA, 4),
                                ("saba10", A.SABA, 10)):
         ctx.set_state(pos, vel, mass)
-        ctx.integrate(A.ALLPAIRS_FP32_KAHAN, integ, order, dt, 1)  # warm
+        _, e0, L0 = ctx.energy()
         ctx.sync()
         t0 = time.perf_counter()
-        ctx.integrate(A.ALLPAIRS_FP32_KAHAN, integ, order, dt, 5)
+        ctx.integrate(A.ALLPAIRS_FP32_KAHAN, integ, order, dt, nfam)
         ctx.sync()
-        fam[name] = (time.perf_counter() - t0) / 5 * 1e3
-    log(f"[bench] integrator family ms/step (mixed forces, n={n}): {fam}")
+        fam[name] = (time.perf_counter() - t0) / nfam * 1e3
+        _, e1, L1 = ctx.energy()
+        fam_drift[name] = {"dE_rel": abs(e1 - e0) / abs(e0),
+                           "dL_rel": float(np.linalg.norm(L1 - L0) / np.linalg.norm(L0))}
+    log(f"[bench] integrator family ms/step (mixed forces, n={n}): {fam}; drift over {nfam} steps {fam_drift}")
     return {"n": n, "dt": dt, "integrator": "YOSHIDA4", "forces": "ALLPAIRS_FP32_KAHAN",
             f"mixed_{steps}_steps": mixed, f"mixed_{steps64}_steps": mixed_short, f"fp64_{steps64}_steps": fp64_short,
-            f"max_abs_dx_mixed_vs_fp64_{steps64}_steps": dx, "ms_per_step_by_integrator": fam}
+            f"max_abs_dx_mixed_vs_fp64_{steps64}_steps": dx, "ms_per_step_by_integrator": fam,
+            f"drift_by_integrator_{nfam}_steps": fam_drift}
+
+
+def bench_reference_case(A, ctx, cpu=True, n=4096, steps=100):
+    """The reference's CPU-runnable case: Plummer N=4096 (equal masses, b = 1),
+    eps = 0.01, FP64 all-pairs, leapfrog dt = (double)(float)1e-3, 100 steps.
+    The device path is bit-identical to the reference's CPU path restated in
+    the oracle; both are timed (the oracle on the host's cores) and the final
+    states compared."""
+    dt = float(np.float32(1e-3))
+    pos, vel, mass = A.ic_plummer(n, 1)
+    ctx.set_params(G=1.0, softening=1e-4)
+    ctx.set_state(pos, vel, mass)
+    e_ref0, e_pl0, L0 = ctx.energy()
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(A.ALLPAIRS_FP64, A.LEAPFROG, dt, steps)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    p, v = ctx.get_state()
+    e_ref1, e_pl1, L1 = ctx.energy()
+    out = {"n": n, "steps": steps, "dt": dt, "gpu_ms_per_step": wall / steps * 1e3,
+           "dE_rel_reference_definition": abs(e_ref1 - e_ref0) / abs(e_ref0),
+           "dE_rel_plummer": abs(e_pl1 - e_pl0) / abs(e_pl0),
+           "dL_rel": float(np.linalg.norm(L1 - L0) / np.linalg.norm(L0))}
+    if cpu:
+        from oracle import oracle as orc
+        th = host_threads()
+        t0 = time.perf_counter()
+        rp, rv, _ = orc.integrate(pos, vel, mass, dt, steps, integrator=2, soft=1e-4, threads=th)
+        cw = time.perf_counter() - t0
+        out.update({"cpu_ms_per_step": cw / steps * 1e3, "cpu_threads": th, "cpu_kind": "port",
+                    "bit_identical": bool(np.array_equal(p, rp) and np.array_equal(v, rv))})
+    log(f"[bench] reference case (Plummer {n}, FP64, leapfrog, {steps} steps): {out}")
+    return out
 
 
 def bench_adjoint(A, ctx, n=65536, T=512, ckpt=64, dt=1e-3):
@@ -396,7 +435,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched ensemble")
     ap.add_argument("--no-integrators", action="store_true", help="skip the integrator section")
     ap.add_argument("--no-ic", action="store_true", help="skip the initial-condition generation section")
-    ap.add_argument("--only", choices=["bh", "allpairs", "adjoint", "batch", "integrators", "ic"],
+    ap.add_argument("--only", choices=["bh", "allpairs", "adjoint", "batch", "integrators", "ic", "reference"],
                     help="run one section only (profiling); no contract line")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -418,6 +457,8 @@ def main():
             r = bench_integrators(A, ctx)
         elif args.only == "ic":
             r = bench_ic(A)
+        elif args.only == "reference":
+            r = bench_reference_case(A, ctx, cpu=not args.no_cpu)
         else:
             r = bench_batch(A, ctx)
         print(json.dumps({"section": args.only, "result": r}, default=float), flush=True)
@@ -428,6 +469,7 @@ def main():
     adj = None if args.no_adjoint else bench_adjoint(A, ctx)
     batch = None if args.no_batch else bench_batch(A, ctx)
     ic = None if args.no_ic else bench_ic(A)
+    refcase = bench_reference_case(A, ctx, cpu=not args.no_cpu)
 
     # roofline of the dominant kernel of the headline step: the traversal
     ms_t, cnt_t = bh["phases"]["traverse"]
@@ -484,6 +526,7 @@ def main():
         "adjoint": adj,
         "batch": batch,
         "ic_generation": ic,
+        "reference_case": refcase,
     }
     print(json.dumps(out), flush=True)
 
