@@ -90,6 +90,17 @@ class Cluster(ctypes.Structure):
     ]
 
 
+class IntegrationReport(ctypes.Structure):
+    """aster_integration_report (include/aster_hip.h)."""
+    _fields_ = [
+        ("total_time_ms", ctypes.c_double), ("time_per_step_ms", ctypes.c_double),
+        ("initial_energy", ctypes.c_double), ("final_energy", ctypes.c_double), ("precision", ctypes.c_double),
+        ("tested_precision", ctypes.c_int32), ("steps", ctypes.c_int32),
+    ]
+
+
+HALTING_CONDITION = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p)
+
 _P = ctypes.c_void_p
 _D = ctypes.POINTER(ctypes.c_double)
 _I64 = ctypes.c_int64
@@ -144,6 +155,11 @@ SIGNATURES = {
     "aster_ic_cube": (ctypes.c_int, [_I64, _U64, ctypes.c_double, ctypes.c_double, _D, _D, _D]),
     "aster_ic_ensemble": (ctypes.c_int, [_I64, _U64, _I32, _I32, _PI64, _PI64, _D, _D, _D]),
     "aster_ic_threebody": (ctypes.c_int, [_I64, _U64, _D, _D, _D]),
+    "aster_integrate_report": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _I64,
+                                              ctypes.c_int, ctypes.POINTER(IntegrationReport)]),
+    "aster_integrate_until": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                             HALTING_CONDITION, ctypes.c_void_p, _I64, ctypes.c_int,
+                                             ctypes.POINTER(IntegrationReport)]),
     "aster_cluster_add": (ctypes.c_int, [_P, ctypes.POINTER(Cluster)]),
     "aster_cluster_pop": (ctypes.c_int, [_P]),
     "aster_cluster_queued": (ctypes.c_int, [_P, _PI64, _PI64]),
@@ -315,6 +331,21 @@ class Context:
     def integrate(self, method, integrator, order, dt, nsteps):
         """aster_integrate: SABA of `order` (1..10), Euler, or the step integrators."""
         self._ck(lib().aster_integrate(self.h, method, integrator, int(order), float(dt), int(nsteps)))
+
+    def integrate_report(self, method, integrator, dt, nsteps, order=0, precision_test=True):
+        """Simulation::integrate(times, precision_test): IntegrationReport fields as a dict."""
+        r = IntegrationReport()
+        self._ck(lib().aster_integrate_report(self.h, method, integrator, int(order), float(dt), int(nsteps),
+                                              1 if precision_test else 0, ctypes.byref(r)))
+        return {k: getattr(r, k) for k, _ in IntegrationReport._fields_}
+
+    def integrate_until(self, method, integrator, dt, halt, max_steps=-1, order=0, precision_test=True):
+        """Simulation::integrate(halting_condition, ...): halt(ctx) -> bool is checked before each step."""
+        cb = HALTING_CONDITION(lambda _c, _u: 1 if halt(self) else 0)
+        r = IntegrationReport()
+        self._ck(lib().aster_integrate_until(self.h, method, integrator, int(order), float(dt), cb, None,
+                                             int(max_steps), 1 if precision_test else 0, ctypes.byref(r)))
+        return {k: getattr(r, k) for k, _ in IntegrationReport._fields_}
 
     def energy(self):
         e = ctypes.c_double()

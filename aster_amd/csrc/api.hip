@@ -5,6 +5,8 @@
 // src/utils/kernel_uploaders.cpp) and the global device state of
 // include/Aster/building-api/GPU_endpoint.h:25-31 with one context object per
 // stream; errors are returned, never exit()ed.
+#include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -408,6 +410,68 @@ int aster_saba_coeffs(int order, double* cs, double* ds) {
   for (int i = 0; i <= order; ++i) cs[i] = kSabaC[order][i];
   for (int i = 0; i < order; ++i) ds[i] = kSabaD[order][i];
   return ASTER_OK;
+}
+
+static int energy_ref(aster_ctx* c, double* e) {
+  double s[6];
+  int rc = launch_energy(c, s);
+  if (rc) return rc;
+  *e = s[0] - 0.5 * s[1];
+  return check_flags(c);
+}
+
+// Shared by the two IntegrationReport entry points: halt == nullptr runs
+// nsteps in one aster_integrate call; otherwise one step per check.
+static int integrate_report(aster_ctx* c, int method, int integrator, int order, double dt, int64_t nsteps,
+                            aster_halting_condition halt, void* user, int precision_test,
+                            aster_integration_report* rep) {
+  if (!c) return ASTER_ERR_INVALID;
+  if (!rep) return fail(c, ASTER_ERR_INVALID, "report is NULL");
+  if (!c->have_state) return fail(c, ASTER_ERR_STATE, "no state loaded");
+  *rep = aster_integration_report{};
+  ASTER_HIP(c, hipSetDevice(c->device));
+  int rc;
+  double e0 = 0.0, e1 = 0.0;
+  if (precision_test && (rc = energy_ref(c, &e0))) return rc;
+  ASTER_HIP(c, hipStreamSynchronize(c->stream));
+  const auto t0 = std::chrono::steady_clock::now();
+  int64_t done = 0;
+  if (!halt) {
+    if (nsteps > 0) {
+      if ((rc = aster_integrate(c, method, integrator, order, dt, nsteps))) return rc;
+      done = nsteps;
+    }
+  } else {
+    while ((nsteps < 0 || done < nsteps) && !halt(c, user)) {
+      if ((rc = aster_integrate(c, method, integrator, order, dt, 1))) return rc;
+      ASTER_HIP(c, hipStreamSynchronize(c->stream));
+      ++done;
+    }
+  }
+  ASTER_HIP(c, hipStreamSynchronize(c->stream));
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (precision_test && (rc = energy_ref(c, &e1))) return rc;
+  rep->total_time_ms = ms;
+  rep->time_per_step_ms = done > 0 ? ms / (double)done : 0.0;
+  rep->initial_energy = e0;
+  rep->final_energy = e1;
+  rep->precision = (precision_test && e0 != 0.0) ? std::fabs(e0 - e1) / std::fabs(e0) : 0.0;
+  rep->tested_precision = precision_test ? 1 : 0;
+  rep->steps = (int32_t)std::min<int64_t>(done, INT32_MAX);
+  return check_flags(c);
+}
+
+int aster_integrate_report(aster_ctx* c, int method, int integrator, int order, double dt, int64_t nsteps,
+                           int precision_test, aster_integration_report* report) {
+  if (c && nsteps < 0) return fail(c, ASTER_ERR_INVALID, "nsteps must be >= 0");
+  return integrate_report(c, method, integrator, order, dt, nsteps, nullptr, nullptr, precision_test, report);
+}
+
+int aster_integrate_until(aster_ctx* c, int method, int integrator, int order, double dt,
+                          aster_halting_condition halt, void* user, int64_t max_steps, int precision_test,
+                          aster_integration_report* report) {
+  if (c && !halt) return fail(c, ASTER_ERR_INVALID, "halting condition is NULL");
+  return integrate_report(c, method, integrator, order, dt, max_steps, halt, user, precision_test, report);
 }
 
 int aster_energy(aster_ctx* c, double* e_ref, double* e_plummer, double* L) {

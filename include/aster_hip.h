@@ -194,6 +194,35 @@ int aster_collect(aster_ctx* ctx, int kind, int64_t body, double* out);
  * (Simulation::get_center_of_mass, builder_endpoint.cpp:262-269). */
 int aster_center_of_mass(aster_ctx* ctx, double* com);
 
+/* IntegrationReport (include/Aster/simulations/sim_obj.h:80-87): wall time
+ * of the steps, and with tested_precision the energy (the reference's
+ * definition, Graphs::get_total_energy) before and after and
+ * precision = |E_after - E_before| / |E_before| (0 when E_before is 0). */
+typedef struct aster_integration_report {
+  double total_time_ms;
+  double time_per_step_ms;
+  double initial_energy;
+  double final_energy;
+  double precision;
+  int32_t tested_precision;
+  int32_t steps; /* steps taken (aster_integrate_until may stop early) */
+} aster_integration_report;
+/* Simulation::integrate(times, precision_test) (src/utils/builder_endpoint.cpp:
+ * 287-352): aster_integrate for nsteps (order as there), timed on the host
+ * around the device work (synchronised before and after), energies only
+ * when precision_test. */
+int aster_integrate_report(aster_ctx* ctx, int method, int integrator, int order, double dt, int64_t nsteps,
+                           int precision_test, aster_integration_report* report);
+/* Simulation::integrate(halting_condition, precision_test, max_steps)
+ * (builder_endpoint.cpp:354-410): one step at a time until halt(ctx, user)
+ * returns nonzero (checked before each step, as the reference's while loop)
+ * or max_steps steps (max_steps < 0: no limit) have run. The state is
+ * synchronised before each check, so halt may read it (aster_get_state). */
+typedef int (*aster_halting_condition)(aster_ctx* ctx, void* user);
+int aster_integrate_until(aster_ctx* ctx, int method, int integrator, int order, double dt,
+                          aster_halting_condition halt, void* user, int64_t max_steps, int precision_test,
+                          aster_integration_report* report);
+
 /* SABAn coefficients used by ASTER_SABA (order 2..10): drifts c[0..order]
  * and kicks d[0..order-1] (GPU::saba_coeffs, include/Aster/physics/
  * coeffs.h:8-161, stores them interleaved). They are the Gauss-Legendre

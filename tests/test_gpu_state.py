@@ -114,3 +114,47 @@ def test_collectors(ctx, orc):
         ctx.collect(A.COLLECT_DISTANCE, 2048)
     ctx.set_state(pos, vel, mass)  # a new state resets E0
     assert ctx.collect(A.COLLECT_ERROR) == 0.0
+
+
+def test_integration_report(ctx, orc):
+    # Simulation::integrate(times, precision_test): energies before/after in
+    # the reference's definition, |dE|/|E0|, step count; the state equals a
+    # plain aster_integrate of the same steps.
+    pos, vel, mass = A.ic_plummer(512, 3)
+    ctx.set_params(G=1.0, softening=1e-4)
+    ctx.set_state(pos, vel, mass)
+    e0, _, _ = ctx.energy()
+    rep = ctx.integrate_report(A.ALLPAIRS_FP64, A.LEAPFROG, 1e-3, 20)
+    e1, _, _ = ctx.energy()
+    p, v = ctx.get_state()
+    assert rep["steps"] == 20 and rep["tested_precision"] == 1
+    assert rep["initial_energy"] == e0 and rep["final_energy"] == e1
+    assert rep["precision"] == abs(e0 - e1) / abs(e0)
+    assert rep["total_time_ms"] > 0 and abs(rep["time_per_step_ms"] * 20 - rep["total_time_ms"]) < 1e-9
+    rp, rv, _ = orc.integrate(pos, vel, mass, 1e-3, 20, integrator=2, soft=1e-4)
+    assert np.array_equal(p, rp) and np.array_equal(v, rv)
+    rep = ctx.integrate_report(A.ALLPAIRS_FP64, A.SABA, 1e-3, 3, order=4, precision_test=False)
+    assert rep["tested_precision"] == 0 and rep["precision"] == 0.0 and rep["steps"] == 3
+
+
+def test_integrate_until_halting_condition(ctx, orc):
+    # Simulation::integrate(halting_condition, ...): the condition is checked
+    # before every step and may read the state; max_steps caps the run.
+    pos, vel, mass = A.ic_plummer(256, 4)
+    ctx.set_params(G=1.0, softening=1e-4)
+    ctx.set_state(pos, vel, mass)
+    seen = []
+
+    def halt(c):
+        p, _ = c.get_state()
+        seen.append(p[0, 0])
+        return len(seen) > 7
+
+    rep = ctx.integrate_until(A.ALLPAIRS_FP64, A.YOSHIDA4, 1e-3, halt)
+    assert rep["steps"] == 7 and len(seen) == 8
+    p, v = ctx.get_state()
+    rp, rv, _ = orc.integrate(pos, vel, mass, 1e-3, 7, integrator=5, soft=1e-4)
+    assert np.array_equal(p, rp) and np.array_equal(v, rv)
+    assert seen[0] == pos[0, 0] and seen[-1] == rp[0, 0]
+    rep = ctx.integrate_until(A.ALLPAIRS_FP64, A.LEAPFROG, 1e-3, lambda c: False, max_steps=3, precision_test=False)
+    assert rep["steps"] == 3

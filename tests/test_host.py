@@ -130,6 +130,13 @@ def test_invalid_arguments_rejected_without_device():
     assert L.aster_get_state_device(None, None, None) == -1
     assert L.aster_collect(None, aster_amd.COLLECT_HAMILTONIAN, 0, None) == -1
     assert L.aster_center_of_mass(None, None) == -1
+    assert L.aster_integrate_report(None, 0, aster_amd.LEAPFROG, 0, 1e-3, 1, 1, None) == -1
+    assert L.aster_integrate_until(None, 0, aster_amd.LEAPFROG, 0, 1e-3, aster_amd.HALTING_CONDITION(), None, 1, 0,
+                                   None) == -1
+    assert L.aster_cluster_add(None, None) == -1
+    assert L.aster_cluster_pop(None) == -1
+    assert L.aster_cluster_queued(None, None, None) == -1
+    assert L.aster_cluster_load(None) == -1
     assert L.aster_destroy(None) == 0
     assert L.aster_last_error(None)  # message for failed creation
 
