@@ -921,26 +921,65 @@ __global__ __launch_bounds__(256) void k_group_emit(int n, const uint32_t* __res
 // Scheduling: the 8 XCDs take workgroups round-robin, so workgroup b works on
 // the groups of chunk (b mod 8), consecutive in Morton order, and neighbouring
 // groups (which open the same nodes) share an L2.
-// Small groups: a group of g <= 32 bodies gives each body KB lanes (the
-// largest power of two with g KB <= 64); lane sub = lane % KB of a body takes
-// list entries sub, sub + KB, ..., and the KB partial sums of a body are then
-// added across its adjacent lanes. The list is padded to a multiple of KB
-// with zero-mass entries (they add exactly zero force).
+// List evaluation in pieces: the g bodies of a group would leave 64 - g lanes
+// idle if each body took one lane through the whole list (groups average ~42
+// bodies). Instead each body's list is cut into q pieces of equal length
+// (lam = ceil(m / q), the list padded with zero-mass entries, which add exactly
+// zero force), and the g q pieces are spread over P passes of the 64 lanes:
+// slot j = 64 p + lane of pass p takes piece j mod q of body j / q (slots past
+// g q repeat piece 0 of the last body and are discarded). Every lane of a pass
+// runs the same lam iterations, so the loop stays uniform. After the passes
+// the partial sums go to LDS and body b adds its pieces b q .. b q + q - 1 in
+// that order (deterministic). (P, q) depends only on g and is chosen, for a
+// list of ~96 entries, to minimise P (lam + 1) with P <= 3 and q <= 64; g = 64
+// gives P = q = 1, g = 42 gives P = 2, q = 3 (98% of the lanes busy instead
+// of 66%). Lane use measured over a 2^20-body tree: 73% with power-of-two
+// lanes per body, ~88% with pieces (simulated on the tree's group sizes).
 // The list holds up to kGrpList entries (flushed once it has more than
-// kGrpList - 64, since one round adds at most 64) plus the padding. LDS per
-// wavefront bounds the occupancy of this kernel: 128 entries measured 5-8%
-// faster than 256 plus padding.
+// kGrpList - 64, since one round adds at most 64) plus the padding (< q <= 64
+// entries); the same LDS holds the P 64 partial sums after the passes. LDS
+// per wavefront bounds the occupancy of this kernel: 128 entries measured
+// 5-8% faster than 256 plus padding.
 constexpr int kGrpList = 128;
 constexpr int kGrpListPad = 64;  // room for that padding
+constexpr int kPieceMaxPasses = 3;
+static_assert(kPieceMaxPasses * 64 <= kGrpList + kGrpListPad, "partial sums must fit the list LDS");
 
 __host__ __device__ constexpr int grp_stack_cap(int levels) { return 32 * levels + 64; }
 
-template <int KB, bool MASK>
-__device__ __forceinline__ void group_list_sum(const float4* __restrict__ lst, int mp, int sub, const float4& me,
-                                               float eps2, float& tx, float& ty, float& tz) {
-  const float4* __restrict__ p = lst + sub;
+struct PieceTable {
+  uint8_t passes[65];
+  uint8_t pieces[65];
+};
+
+constexpr PieceTable make_piece_table() {
+  PieceTable t{};
+  for (int g = 1; g <= 64; ++g) {
+    int best = 1 << 30, bp = 1, bq = 1;
+    for (int p = 1; p <= kPieceMaxPasses; ++p) {
+      int q = (64 * p) / g;
+      if (q > 64) q = 64;
+      const int cost = p * ((96 + q - 1) / q + 1);
+      if (cost < best) {
+        best = cost;
+        bp = p;
+        bq = q;
+      }
+    }
+    t.passes[g] = (uint8_t)bp;
+    t.pieces[g] = (uint8_t)bq;
+  }
+  return t;
+}
+
+__constant__ PieceTable kPieces = make_piece_table();
+
+// One piece: lam list entries from `p` against the body at `me`.
+template <bool MASK>
+__device__ __forceinline__ void group_piece_sum(const float4* __restrict__ p, int lam, const float4& me,
+                                                float eps2, float& tx, float& ty, float& tz) {
 #pragma unroll 4
-  for (int e = 0; e < mp; e += KB) {
+  for (int e = 0; e < lam; ++e) {
     const float4 q = p[e];
     const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
     float r2 = fmaf(dx, dx, eps2);
@@ -952,12 +991,6 @@ __device__ __forceinline__ void group_list_sum(const float4* __restrict__ lst, i
     tx = fmaf(f, dx, tx);
     ty = fmaf(f, dy, ty);
     tz = fmaf(f, dz, tz);
-  }
-#pragma unroll
-  for (int off = 1; off < KB; off <<= 1) {
-    tx += __shfl_xor(tx, off, 64);
-    ty += __shfl_xor(ty, off, 64);
-    tz += __shfl_xor(tz, off, 64);
   }
 }
 
@@ -982,14 +1015,26 @@ __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restric
     const int g = xcd * per_xcd + i;
     if (g >= ngroups) break;
     const int lo = gstart[g], hi = gstart[g + 1] - 1;
-    const int gsz = hi - lo + 1;
-    int kb = 1;  // lanes per body
-    while (2 * kb * gsz <= 64) kb *= 2;
-    kb = __builtin_amdgcn_readfirstlane(kb);
-    const int body = lane / kb, sub = lane & (kb - 1);
-    const int k = lo + body;
+    const int gsz = __builtin_amdgcn_readfirstlane(hi - lo + 1);
+    const int k = lo + lane;
     const bool live = k <= hi;
     const float4 me = gsh[live ? k : hi];
+    // this lane's piece in each pass
+    const int npass = __builtin_amdgcn_readfirstlane((int)kPieces.passes[gsz]);
+    const int npc = __builtin_amdgcn_readfirstlane((int)kPieces.pieces[gsz]);
+    float4 ppos[kPieceMaxPasses];
+    int pidx[kPieceMaxPasses];
+#pragma unroll
+    for (int p = 0; p < kPieceMaxPasses; ++p) {
+      const int j = 64 * p + lane;
+      int b = j / npc, pc = j - b * npc;
+      if (b >= gsz) {
+        b = gsz - 1;
+        pc = 0;
+      }
+      ppos[p] = make_float4(__shfl(me.x, b, 64), __shfl(me.y, b, 64), __shfl(me.z, b, 64), 0.f);
+      pidx[p] = pc;
+    }
     float blo[3] = {me.x, me.y, me.z}, bhi[3] = {me.x, me.y, me.z};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -1012,23 +1057,36 @@ __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restric
       return fmaf(ex, ex, fmaf(ey, ey, ez * ez));
     };
     auto flush = [&]() {
-      float tx = 0.f, ty = 0.f, tz = 0.f;
       const int m = __builtin_amdgcn_readfirstlane(cnt);
-      const int mp = (m + kb - 1) & ~(kb - 1);
+      const int lam = __builtin_amdgcn_readfirstlane((m + npc - 1) / npc);
+      const int mp = npc * lam;  // m <= mp < m + npc <= m + 64
       if (lane < mp - m) lst[m + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
       wave_sync();
-      switch (kb) {
-        case 1: group_list_sum<1, MASK>(lst, mp, sub, me, eps2, tx, ty, tz); break;
-        case 2: group_list_sum<2, MASK>(lst, mp, sub, me, eps2, tx, ty, tz); break;
-        case 4: group_list_sum<4, MASK>(lst, mp, sub, me, eps2, tx, ty, tz); break;
-        case 8: group_list_sum<8, MASK>(lst, mp, sub, me, eps2, tx, ty, tz); break;
-        case 16: group_list_sum<16, MASK>(lst, mp, sub, me, eps2, tx, ty, tz); break;
-        case 32: group_list_sum<32, MASK>(lst, mp, sub, me, eps2, tx, ty, tz); break;
-        default: group_list_sum<64, MASK>(lst, mp, sub, me, eps2, tx, ty, tz); break;
+      float px[kPieceMaxPasses], py[kPieceMaxPasses], pz[kPieceMaxPasses];
+#pragma unroll
+      for (int p = 0; p < kPieceMaxPasses; ++p) {
+        px[p] = py[p] = pz[p] = 0.f;
+        if (p < npass) group_piece_sum<MASK>(lst + pidx[p] * lam, lam, ppos[p], eps2, px[p], py[p], pz[p]);
       }
-      kahan_add32(sx, cx, tx);
-      kahan_add32(sy, cy, ty);
-      kahan_add32(sz, cz, tz);
+      wave_sync();  // every lane is done with the list: its LDS takes the partial sums
+#pragma unroll
+      for (int p = 0; p < kPieceMaxPasses; ++p)
+        if (p < npass) lst[64 * p + lane] = make_float4(px[p], py[p], pz[p], 0.f);
+      wave_sync();
+      if (live) {
+        float tx = 0.f, ty = 0.f, tz = 0.f;
+        const float4* __restrict__ mine = lst + lane * npc;
+        for (int c = 0; c < npc; ++c) {
+          const float4 v = mine[c];
+          tx += v.x;
+          ty += v.y;
+          tz += v.z;
+        }
+        kahan_add32(sx, cx, tx);
+        kahan_add32(sy, cy, ty);
+        kahan_add32(sz, cz, tz);
+      }
+      wave_sync();  // the partial sums are read before the list is refilled
       listed += (unsigned long long)cnt;
       cnt = 0;
     };
@@ -1081,7 +1139,7 @@ __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restric
     }
     if (cnt > 0) flush();
     wave_sync();
-    if (live && sub == 0) {
+    if (live) {
       const int b = nlink[k].w;
       a[b] = G * ((double)sx - (double)cx);
       a[n + b] = G * ((double)sy - (double)cy);

@@ -88,9 +88,18 @@ def cpu_model():
     return "unknown"
 
 
+def params(A, ctx, **kw):
+    """Set every context parameter: the library's defaults overridden by kw.
+    Parameters persist on a context, so each section states all of them."""
+    p = dict(G=6e-11, c=299792458.0, softening=1e-14, theta=0.2, box=(0.0, 0.0, 0.0), pn_flags=A.NEWTON,
+             precision=A.FP32, lower=0, upper=-1, split=0)
+    p.update(kw)
+    return ctx.set_params(**p)
+
+
 def bench_bh(A, ctx, steps, warmup, method, n=1 << 20, theta=0.5, dt=5e-4, reference=True):
     pos, vel, mass = A.ic_mixed(n, 20240)
-    ctx.set_params(G=1.0, softening=0.005 ** 2, theta=theta, precision=A.FP32)
+    params(A, ctx, G=1.0, softening=0.005 ** 2, theta=theta, precision=A.FP32)
     ctx.set_state(pos, vel, mass)
     errs = None
     if reference:
@@ -136,11 +145,29 @@ def bench_bh(A, ctx, steps, warmup, method, n=1 << 20, theta=0.5, dt=5e-4, refer
                 n=n, theta=theta, dt=dt)
 
 
+def bench_scaling(A, ctx, method, sizes=(18, 20, 22, 24, 26)):
+    """Barnes-Hut leapfrog step against N on the same workload family as the
+    headline step (theta = 0.5); fewer timed steps at the larger sizes."""
+    rows = []
+    for lg in sizes:
+        n = 1 << lg
+        steps = 10 if lg <= 22 else (5 if lg <= 24 else 3)
+        r = bench_bh(A, ctx, steps, 1, method, n=n, reference=False)
+        ms_t, cnt_t = r["phases"]["traverse"]
+        step_ms = r["wall_s"] / steps * 1e3
+        inter = r["interactions"] / steps
+        rows.append({"n": n, "steps": steps, "ms_per_step": step_ms, "traverse_ms": ms_t / cnt_t,
+                     "interactions_per_body": inter / n, "interactions_per_s": inter / (step_ms * 1e-3),
+                     "bodies_per_s": n / (step_ms * 1e-3)})
+        log(f"[bench] scaling {rows[-1]}")
+    return rows
+
+
 def bench_allpairs(A, ctx, evals, n=65536):
     """All-pairs at the mixed-precision configuration's shape (Plummer N=65536,
     log-uniform masses, eps=0.005): both FP32 methods, timed per kernel."""
     pos, vel, mass = A.ic_plummer(n, 3, mass_mode=1)
-    ctx.set_params(G=1.0, softening=0.005 ** 2, theta=0.5)
+    params(A, ctx, G=1.0, softening=0.005 ** 2, theta=0.5)
     ctx.set_state(pos, vel, mass)
     ref = ctx.forces(A.ALLPAIRS_FP64)
     out = {}
@@ -192,7 +219,7 @@ def bench_integrators(A, ctx, n=65536, dt=2.5e-4, steps=1000, steps64=20):
     of each integrator of the family, timed with the mixed forces."""
     pos, vel, mass = A.ic_plummer(n, 3, mass_mode=1)
     soft = 0.005 ** 2
-    ctx.set_params(G=1.0, softening=soft)
+    params(A, ctx, G=1.0, softening=soft)
 
     def drift(method, nsteps):
         ctx.set_state(pos, vel, mass)
@@ -241,7 +268,7 @@ def bench_reference_case(A, ctx, cpu=True, n=4096, steps=100):
     states compared."""
     dt = float(np.float32(1e-3))
     pos, vel, mass = A.ic_plummer(n, 1)
-    ctx.set_params(G=1.0, softening=1e-4)
+    params(A, ctx, G=1.0, softening=1e-4)
     ctx.set_state(pos, vel, mass)
     e_ref0, e_pl0, L0 = ctx.energy()
     ctx.sync()
@@ -274,7 +301,7 @@ def bench_adjoint(A, ctx, n=65536, T=512, ckpt=64, dt=1e-3):
     pos, vel, mass = A.ic_plummer(n, 5)
     rng = np.random.default_rng(5)
     snaps = np.arange(1, 9) * (T // 8)
-    ctx.set_params(G=1.0, softening=0.005 ** 2)
+    params(A, ctx, G=1.0, softening=0.005 ** 2)
     ctx.set_state(pos + rng.normal(scale=1e-3, size=pos.shape), vel + rng.normal(scale=1e-3, size=vel.shape), mass)
     target = ctx.traj_forward(dt, T, ckpt, snaps)
     ctx.set_state(pos, vel, mass)
@@ -300,7 +327,7 @@ def bench_batch(A, ctx, nsys=8192, steps=10000, dt=1e-3):
     """Planetary ensembles: 8192 systems of N in [3, 64], 1PN (c=1e4) in FP64,
     Yoshida-4 dt=1e-3 for 10^4 steps in captured launches."""
     off, pos, vel, mass = A.ic_ensemble(nsys, 11, 3, 64)
-    ctx.set_params(G=1.0, softening=0.0, c=1e4, pn_flags=A.NEWTON | A.PN1, precision=A.FP32)
+    params(A, ctx, G=1.0, softening=0.0, c=1e4, pn_flags=A.NEWTON | A.PN1, precision=A.FP32)
     ctx.batch_set(off, pos, vel, mass)
     e0, L0 = ctx.batch_energy()
     wpb, spl = ctx.batch_autotune(A.YOSHIDA4, dt)
@@ -326,7 +353,7 @@ def bench_batch(A, ctx, nsys=8192, steps=10000, dt=1e-3):
     # splitting's O(m/M dt^2) error is meant to pay off.
     wh = {}
     dtc, nc = 0.02, 500
-    ctx.set_params(G=1.0, softening=0.0, pn_flags=A.NEWTON, precision=A.FP32)
+    params(A, ctx, G=1.0, softening=0.0, pn_flags=A.NEWTON, precision=A.FP32)
     for name, integ in (("yoshida4", A.YOSHIDA4), ("wisdom_holman", A.WH_PLANETARY)):
         ctx.batch_set(off, pos, vel, mass)
         en0, _ = ctx.batch_energy()
@@ -347,7 +374,7 @@ def bench_batch(A, ctx, nsys=8192, steps=10000, dt=1e-3):
     ne = 1000
     eih = {}
     for name, flags in (("pairwise_1pn_exact", A.NEWTON | A.PN1), ("eih_exact", A.NEWTON | A.EIH)):
-        ctx.set_params(G=1.0, softening=0.0, c=1e4, pn_flags=flags, precision=A.FP64)
+        params(A, ctx, G=1.0, softening=0.0, c=1e4, pn_flags=flags, precision=A.FP64)
         ctx.batch_set(off, pos, vel, mass)
         en0, _ = ctx.batch_energy()
         ctx.batch_step(A.YOSHIDA4, dt, 0)
@@ -435,7 +462,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched ensemble")
     ap.add_argument("--no-integrators", action="store_true", help="skip the integrator section")
     ap.add_argument("--no-ic", action="store_true", help="skip the initial-condition generation section")
-    ap.add_argument("--only", choices=["bh", "allpairs", "adjoint", "batch", "integrators", "ic", "reference"],
+    ap.add_argument("--only", choices=["bh", "allpairs", "adjoint", "batch", "integrators", "ic", "reference",
+                                       "scaling"],
                     help="run one section only (profiling); no contract line")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -459,6 +487,8 @@ def main():
             r = bench_ic(A)
         elif args.only == "reference":
             r = bench_reference_case(A, ctx, cpu=not args.no_cpu)
+        elif args.only == "scaling":
+            r = bench_scaling(A, ctx, method)
         else:
             r = bench_batch(A, ctx)
         print(json.dumps({"section": args.only, "result": r}, default=float), flush=True)
