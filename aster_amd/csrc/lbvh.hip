@@ -685,18 +685,25 @@ __global__ __launch_bounds__(256) void k_group_moments(int n, const int32_t* __r
 // mac 0 (parity): crit2 = (s / theta)^2 with s = |box| 2^-depth.
 // mac 1 (geometric): crit2 = (l / theta + delta)^2, l = longest AABB side,
 //                    delta = |COM - AABB centre|.
+// With `kids` set (the group traversal) the node's record and threshold go
+// instead to its slot under its parent, kid_slot(parent, side): opening a node
+// then costs one load of 64 contiguous bytes for both children. The root has
+// the extra slot 2 (n - 1).
+__host__ __device__ inline int kid_slot(int n, int parent, int side) { return 2 * (parent - n) + side; }
+
 __global__ __launch_bounds__(256) void k_links(int n, const int32_t* __restrict__ right,
                                                const int32_t* __restrict__ escape, const int32_t* __restrict__ depth,
                                                const float4* __restrict__ nbox, const float4* __restrict__ nrec,
                                                const double* __restrict__ boxd, double theta, int mac,
-                                               int4* __restrict__ nlink, const uint32_t* __restrict__ perm) {
+                                               int4* __restrict__ nlink, const uint32_t* __restrict__ perm,
+                                               const int32_t* __restrict__ parent, const int32_t* __restrict__ left,
+                                               float4* __restrict__ kids) {
   const int node = blockIdx.x * blockDim.x + threadIdx.x;
   const int nn = 2 * n - 1;
   if (node >= nn) return;
-  const int dep = depth[node];
-  const int esc = escape[node];
   float crit2;
   if (mac == 0) {
+    const int dep = depth[node];
     const double s = ldexp(boxd[3], -dep);
     const double q = s / theta;
     crit2 = (float)(q * q);
@@ -712,8 +719,18 @@ __global__ __launch_bounds__(256) void k_links(int n, const int32_t* __restrict_
     const float q = l / (float)theta + delta;
     crit2 = q * q;
   }
+  if (kids) {
+    int slot = 2 * (n - 1);
+    if (node != n) {
+      const int p = parent[node];
+      slot = kid_slot(n, p, left[p] == node ? 0 : 1);
+    }
+    kids[2 * slot] = nrec[node];
+    kids[2 * slot + 1] = make_float4(__int_as_float(node), crit2, 0.f, 0.f);
+    return;
+  }
   const int body = node < n ? (int)perm[node] : -1;
-  nlink[node] = make_int4(node < n ? -1 : right[node], esc, __float_as_int(crit2), body);
+  nlink[node] = make_int4(node < n ? -1 : right[node], escape[node], __float_as_int(crit2), body);
 }
 
 // --------------------------------------------------------- traversals ------
@@ -997,8 +1014,8 @@ __device__ __forceinline__ void group_piece_sum(const float4* __restrict__ p, in
 template <bool MASK>
 __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restrict__ gstart,
                                                  const uint32_t* __restrict__ gcount, const float4* __restrict__ gsh,
-                                                 const float4* __restrict__ nrec, const int4* __restrict__ nlink,
-                                                 const int32_t* __restrict__ left, float eps2, double G,
+                                                 const float4* __restrict__ kids, const uint32_t* __restrict__ perm,
+                                                 float eps2, double G,
                                                  double* __restrict__ a, unsigned long long* __restrict__ counters,
                                                  uint32_t* __restrict__ flags, int stack_cap) {
   extern __shared__ float4 grp_lds[];
@@ -1091,9 +1108,9 @@ __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restric
       cnt = 0;
     };
     {
-      const int root = n;
-      const float4 r = nrec[root];
-      if (dmin2(r) > __int_as_float(nlink[root].z)) {
+      const int root = n, rs = 2 * (n - 1);
+      const float4 r = kids[2 * rs];
+      if (dmin2(r) > kids[2 * rs + 1].y) {
         if (lane == 0) lst[0] = r;
         cnt = 1;
       } else {
@@ -1102,36 +1119,52 @@ __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restric
       }
     }
     wave_sync();
+    // The children of the next batch (the top min(sp, 32) stack entries) are
+    // loaded before a pending flush evaluates the list, so the load's latency
+    // overlaps that arithmetic. Lane 2 i + s takes child s of entry i: the two
+    // children of a node are 64 contiguous bytes. Lanes past the batch load a
+    // copy of its last entry and are masked off; every lane loads, so the
+    // loop carries one definition of the record and needs no branches.
+    int take = 0;
+    float4 nr;
+    float2 na;
+    // With an empty stack (the walk is over) it loads the root's children.
+    auto fetch = [&]() {
+      take = min(sp, 32);
+      const int e = max(min(lane >> 1, take - 1), 0);
+      const int opened = sp > 0 ? stk[sp - take + e] : n;
+      const int slot = kid_slot(n, opened, lane & 1);
+      nr = kids[2 * slot];
+      na = *reinterpret_cast<const float2*>(kids + 2 * slot + 1);
+    };
+    fetch();
     bool overflow = false;
     while (sp > 0) {
-      const int take = min(sp, 32);
-      int node = -1;
-      if (lane < 2 * take) {
-        const int opened = stk[sp - take + (lane >> 1)];
-        node = (lane & 1) ? nlink[opened].x : left[opened];
-      }
+      const float4 r = nr;
+      const int node = __float_as_int(na.x);
+      // both terms evaluated (no short circuit): the id and the threshold
+      // arrive with the record in one round trip
+      const bool take_it = (node < n) | (dmin2(r) > na.y);
+      const bool mine = lane < 2 * take;
+      const bool acc = mine & take_it, opn = mine & !take_it;
       sp -= take;
       visits += 2 * take;
       wave_sync();
-      bool acc = false, opn = false;
-      float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (node >= 0) {
-        r = nrec[node];
-        acc = node < n || dmin2(r) > __int_as_float(nlink[node].z);
-        opn = !acc;
-      }
       const uint64_t am = __ballot(acc), om = __ballot(opn);
       const int nopen = (int)__popcll(om);
-      if (sp + nopen > stack_cap) {  // cannot happen for a valid tree (see bound above)
+      // cannot happen for a valid tree (see bound above); the walk then ends
+      // here (no early exit from the loop, which keeps its control flow simple)
+      const bool over = sp + nopen > stack_cap;
+      if (over) {
         if (lane == 0) atomicOr(flags, (uint32_t)FLAG_TREE);
         overflow = true;
-        break;
       }
       if (acc) lst[cnt + __popcll(am & lt)] = r;
-      if (opn) stk[sp + __popcll(om & lt)] = node;
+      if (opn && !over) stk[sp + __popcll(om & lt)] = node;
       cnt += (int)__popcll(am);
-      sp += nopen;
+      sp = over ? 0 : sp + nopen;
       wave_sync();
+      fetch();
       if (cnt > kGrpList - 64) {
         flush();
         wave_sync();
@@ -1140,7 +1173,7 @@ __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restric
     if (cnt > 0) flush();
     wave_sync();
     if (live) {
-      const int b = nlink[k].w;
+      const int b = (int)perm[k];
       a[b] = G * ((double)sx - (double)cx);
       a[n + b] = G * ((double)sy - (double)cy);
       a[2 * n + b] = G * ((double)sz - (double)cz);
@@ -1189,7 +1222,7 @@ static int device_scan(aster_ctx* c, uint32_t* data, int count, uint32_t* total)
   return ASTER_OK;
 }
 
-static int reserve_tree(aster_ctx* c) {
+static int reserve_tree(aster_ctx* c, bool full) {
   const size_t n = (size_t)c->n, nn = 2 * n - 1;
   int rc = reserve_sort(c);
   if (rc) return rc;
@@ -1208,7 +1241,8 @@ static int reserve_tree(aster_ctx* c) {
   ASTER_HIP(c, c->nmc.reserve(nn));
   ASTER_HIP(c, c->nbox.reserve(2 * nn));
   ASTER_HIP(c, c->nrec.reserve(nn));
-  ASTER_HIP(c, c->nlink.reserve(nn));
+  if (full) ASTER_HIP(c, c->nlink.reserve(nn));
+  else ASTER_HIP(c, c->kids.reserve(2 * nn));  // 2 float4 per child slot; 2 (n - 1) + 1 slots
   ASTER_HIP(c, c->gsh.reserve(n));
   if (!c->bh_counters.p) {
     // Counters accumulate over evaluations until aster_bh_stats reads them.
@@ -1308,7 +1342,7 @@ static int launch_moments_band(aster_ctx* c, bool vel, int h, int off, int cnt) 
 // traversals and the reference MAC need them; the group traversal does not).
 static int build_tree(aster_ctx* c, int mac, bool vel, bool full) {
   const int n = (int)c->n;
-  int rc = reserve_tree(c);
+  int rc = reserve_tree(c, full);
   if (rc) return rc;
   if ((rc = morton_sort(c, true))) return rc;
   phase_begin(c, ASTER_PHASE_BUILD);
@@ -1384,7 +1418,8 @@ static int build_tree(aster_ctx* c, int mac, bool vel, bool full) {
   phase_end(c, ASTER_PHASE_MOMENTS);
   phase_begin(c, ASTER_PHASE_LINKS);
   hipLaunchKernelGGL(k_links, dim3((nn + 255) / 256), dim3(256), 0, c->stream, n, c->right.p, c->escape.p,
-                     c->depth.p, c->nbox.p, c->nrec.p, c->boxd.p, c->P.theta, mac, c->nlink.p, c->idx.p);
+                     c->depth.p, c->nbox.p, c->nrec.p, c->boxd.p, c->P.theta, mac, c->nlink.p, c->idx.p,
+                     c->parent.p, c->left.p, full ? nullptr : c->kids.p);
   ASTER_HIP(c, hipGetLastError());
   phase_end(c, ASTER_PHASE_LINKS);
   c->tree_valid = true;
@@ -1433,8 +1468,8 @@ int bh_forces(aster_ctx* c, int method) {
       const int cap = grp_stack_cap(c->tree_levels);
       const size_t lds = (kGrpList + kGrpListPad) * sizeof(float4) + (size_t)cap * sizeof(int);
       auto kern = mask ? k_bh_group<true> : k_bh_group<false>;
-      hipLaunchKernelGGL(kern, grid, block, lds, c->stream, n, c->gstart.p, c->gcount.p, c->gsh.p, c->nrec.p,
-                         c->nlink.p, c->left.p, eps2, c->P.G, c->a.p, c->bh_counters.p, c->flags.p, cap);
+      hipLaunchKernelGGL(kern, grid, block, lds, c->stream, n, c->gstart.p, c->gcount.p, c->gsh.p, c->kids.p,
+                         c->idx.p, eps2, c->P.G, c->a.p, c->bh_counters.p, c->flags.p, cap);
     } else if (mask) {
       hipLaunchKernelGGL((k_bh_f32<true>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->gsh.p, c->nrec.p,
                          c->nlink.p, eps2, c->P.G, c->a.p, c->bh_counters.p, c->flags.p);
