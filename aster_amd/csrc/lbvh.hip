@@ -459,6 +459,39 @@ struct MomRec {
   float4 lo, hi;
 };
 
+// The group traversal reads, for each internal node, its two children's
+// records, node ids and thresholds from slots kid_slot(node, 0/1): opening a
+// node costs one load of 64 contiguous bytes. The node that forms its moments
+// writes its children's slots; the root has the extra slot 2 (n - 1).
+__host__ __device__ inline int kid_slot(int n, int parent, int side) { return 2 * (parent - n) + side; }
+
+// Geometric opening threshold of a node from its FP32 box and record: accept
+// when d^2 > crit2 = (l / theta + delta)^2, l = longest AABB side, delta =
+// |COM - AABB centre| (a leaf's degenerate box gives 0).
+__device__ __forceinline__ float geo_crit2(const float4& lo, const float4& hi, const float4& r, double theta) {
+#pragma clang fp contract(off)
+  const float l = fmaxf(hi.x - lo.x, fmaxf(hi.y - lo.y, hi.z - lo.z));
+  const float cx = 0.5f * (hi.x + lo.x);
+  const float cy = 0.5f * (hi.y + lo.y);
+  const float cz = 0.5f * (hi.z + lo.z);
+  const float ex = r.x - cx, ey = r.y - cy, ez = r.z - cz;
+  const float delta = sqrtf(ex * ex + ey * ey + ez * ez);
+  const float q = l / (float)theta + delta;
+  return q * q;
+}
+
+__device__ __forceinline__ float4 node_rec(const double4& com) {
+  return make_float4((float)com.x, (float)com.y, (float)com.z, (float)com.w);
+}
+
+// The group traversal's child slot (see k_links): record, node id, threshold.
+__device__ __forceinline__ void put_kid(float4* __restrict__ kids, int slot, int node, const double4& com,
+                                        const float4& lo, const float4& hi, double theta) {
+  const float4 r = node_rec(com);
+  kids[2 * slot] = r;
+  kids[2 * slot + 1] = make_float4(__int_as_float(node), geo_crit2(lo, hi, r, theta), 0.f, 0.f);
+}
+
 template <bool VEL>
 __device__ __forceinline__ MomRec load_moments(int node, const double4* __restrict__ nmc,
                                                const double4* __restrict__ nvel, const float4* __restrict__ nbox) {
@@ -500,7 +533,8 @@ __global__ __launch_bounds__(256) void k_moments_band(int n, const int32_t* __re
                                                       const int32_t* __restrict__ right,
                                                       const int2* __restrict__ grange, double4* __restrict__ nmc,
                                                       double4* __restrict__ nvel, float4* __restrict__ nbox,
-                                                      float4* __restrict__ nrec) {
+                                                      float4* __restrict__ nrec, double theta,
+                                                      float4* __restrict__ kids) {
   constexpr int W = 1 << H;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= count) return;
@@ -556,7 +590,14 @@ __global__ __launch_bounds__(256) void k_moments_band(int n, const int32_t* __re
       if (VEL) nvel[node] = me.vel;
       nbox[2 * node] = me.lo;
       nbox[2 * node + 1] = me.hi;
-      nrec[node] = make_float4((float)me.com.x, (float)me.com.y, (float)me.com.z, (float)me.com.w);
+      if (!kids) nrec[node] = node_rec(me.com);
+      if (kids) {  // this node's child slots (and the root's own slot)
+        const MomRec& a = rec[l + 1][2 * k];
+        const MomRec& b = rec[l + 1][2 * k + 1];
+        put_kid(kids, kid_slot(n, node, 0), id[l + 1][2 * k], a.com, a.lo, a.hi, theta);
+        put_kid(kids, kid_slot(n, node, 1), id[l + 1][2 * k + 1], b.com, b.lo, b.hi, theta);
+        if (node == n) put_kid(kids, 2 * (n - 1), n, me.com, me.lo, me.hi, theta);
+      }
       rec[l][k] = me;
     }
   }
@@ -577,7 +618,8 @@ __global__ __launch_bounds__(256) void k_group_moments(int n, const int32_t* __r
                                                        const int32_t* __restrict__ left,
                                                        const int32_t* __restrict__ right, double4* __restrict__ nmc,
                                                        double4* __restrict__ nvel, float4* __restrict__ nbox,
-                                                       float4* __restrict__ nrec) {
+                                                       float4* __restrict__ nrec, double theta,
+                                                       float4* __restrict__ kids) {
 #pragma clang fp contract(off)
   __shared__ double4 smc[4][64];
   __shared__ double4 svel[VEL ? 4 : 1][64];
@@ -673,9 +715,18 @@ __global__ __launch_bounds__(256) void k_group_moments(int n, const int32_t* __r
     if (mine) {
       nmc[node] = com;
       if (VEL) nvel[node] = vel;
-      nbox[2 * node] = blo;
-      nbox[2 * node + 1] = bhi;
-      nrec[node] = make_float4((float)com.x, (float)com.y, (float)com.z, (float)com.w);
+      // With child slots (the group traversal) nobody reads nrec, nor the
+      // box of a node below its group's root.
+      if (!kids || node == n + (root_lo ? lo : hi)) {
+        nbox[2 * node] = blo;
+        nbox[2 * node + 1] = bhi;
+      }
+      if (!kids) nrec[node] = node_rec(com);
+      if (kids) {  // this node's child slots (and the root's own slot)
+        put_kid(kids, kid_slot(n, node, 0), L, cL, loL, hiL, theta);
+        put_kid(kids, kid_slot(n, node, 1), R, cR, loR, hiR, theta);
+        if (node == n) put_kid(kids, 2 * (n - 1), n, com, blo, bhi, theta);
+      }
     }
   }
 }
@@ -685,19 +736,11 @@ __global__ __launch_bounds__(256) void k_group_moments(int n, const int32_t* __r
 // mac 0 (parity): crit2 = (s / theta)^2 with s = |box| 2^-depth.
 // mac 1 (geometric): crit2 = (l / theta + delta)^2, l = longest AABB side,
 //                    delta = |COM - AABB centre|.
-// With `kids` set (the group traversal) the node's record and threshold go
-// instead to its slot under its parent, kid_slot(parent, side): opening a node
-// then costs one load of 64 contiguous bytes for both children. The root has
-// the extra slot 2 (n - 1).
-__host__ __device__ inline int kid_slot(int n, int parent, int side) { return 2 * (parent - n) + side; }
-
 __global__ __launch_bounds__(256) void k_links(int n, const int32_t* __restrict__ right,
                                                const int32_t* __restrict__ escape, const int32_t* __restrict__ depth,
                                                const float4* __restrict__ nbox, const float4* __restrict__ nrec,
                                                const double* __restrict__ boxd, double theta, int mac,
-                                               int4* __restrict__ nlink, const uint32_t* __restrict__ perm,
-                                               const int32_t* __restrict__ parent, const int32_t* __restrict__ left,
-                                               float4* __restrict__ kids) {
+                                               int4* __restrict__ nlink, const uint32_t* __restrict__ perm) {
   const int node = blockIdx.x * blockDim.x + threadIdx.x;
   const int nn = 2 * n - 1;
   if (node >= nn) return;
@@ -708,26 +751,7 @@ __global__ __launch_bounds__(256) void k_links(int n, const int32_t* __restrict_
     const double q = s / theta;
     crit2 = (float)(q * q);
   } else {
-    const float4 lo = nbox[2 * node], hi = nbox[2 * node + 1];
-    const float l = fmaxf(hi.x - lo.x, fmaxf(hi.y - lo.y, hi.z - lo.z));
-    const float4 r = nrec[node];
-    const float cx = 0.5f * (hi.x + lo.x);
-    const float cy = 0.5f * (hi.y + lo.y);
-    const float cz = 0.5f * (hi.z + lo.z);
-    const float ex = r.x - cx, ey = r.y - cy, ez = r.z - cz;
-    const float delta = sqrtf(ex * ex + ey * ey + ez * ez);
-    const float q = l / (float)theta + delta;
-    crit2 = q * q;
-  }
-  if (kids) {
-    int slot = 2 * (n - 1);
-    if (node != n) {
-      const int p = parent[node];
-      slot = kid_slot(n, p, left[p] == node ? 0 : 1);
-    }
-    kids[2 * slot] = nrec[node];
-    kids[2 * slot + 1] = make_float4(__int_as_float(node), crit2, 0.f, 0.f);
-    return;
+    crit2 = geo_crit2(nbox[2 * node], nbox[2 * node + 1], nrec[node], theta);
   }
   const int body = node < n ? (int)perm[node] : -1;
   nlink[node] = make_int4(node < n ? -1 : right[node], escape[node], __float_as_int(crit2), body);
@@ -1317,13 +1341,13 @@ static int build_groups(aster_ctx* c) {
 
 constexpr int kBand = 3;  // levels per moments launch above the groups
 
-static int launch_moments_band(aster_ctx* c, bool vel, int h, int off, int cnt) {
+static int launch_moments_band(aster_ctx* c, bool vel, int h, int off, int cnt, float4* kids) {
   const int n = (int)c->n;
   const dim3 grid((cnt + 255) / 256), block(256);
   const int32_t* b = c->bucket.p + off;
 #define ASTER_BAND(V, H)                                                                                          \
   hipLaunchKernelGGL((k_moments_band<V, H>), grid, block, 0, c->stream, n, b, cnt, c->left.p, c->right.p,        \
-                     c->grange.p, c->nmc.p, c->nvel.p, c->nbox.p, c->nrec.p)
+                     c->grange.p, c->nmc.p, c->nvel.p, c->nbox.p, c->nrec.p, c->P.theta, kids)
   switch (h * 2 + (vel ? 1 : 0)) {
     case 2: ASTER_BAND(false, 1); break;
     case 3: ASTER_BAND(true, 1); break;
@@ -1344,6 +1368,7 @@ static int build_tree(aster_ctx* c, int mac, bool vel, bool full) {
   const int n = (int)c->n;
   int rc = reserve_tree(c, full);
   if (rc) return rc;
+  float4* kids = full ? nullptr : c->kids.p;  // the group traversal's child slots
   if ((rc = morton_sort(c, true))) return rc;
   phase_begin(c, ASTER_PHASE_BUILD);
   if (vel) ASTER_HIP(c, c->nvel.reserve((size_t)2 * n - 1));
@@ -1394,7 +1419,7 @@ static int build_tree(aster_ctx* c, int mac, bool vel, bool full) {
   const int gblocks = ((n / 32 + 1) + 3) / 4;
   auto gmom = vel ? k_group_moments<true> : k_group_moments<false>;
   hipLaunchKernelGGL(gmom, dim3(gblocks), dim3(256), 0, c->stream, n, c->gstart.p, c->gcount.p, c->grange.p, c->left.p,
-                     c->right.p, c->nmc.p, c->nvel.p, c->nbox.p, c->nrec.p);
+                     c->right.p, c->nmc.p, c->nvel.p, c->nbox.p, c->nrec.p, c->P.theta, kids);
   ASTER_HIP(c, hipGetLastError());
   ASTER_HIP(c, hipEventSynchronize(c->tree_ev));
   const int32_t* lv = c->host_words;
@@ -1411,16 +1436,17 @@ static int build_tree(aster_ctx* c, int mac, bool vel, bool full) {
     const int h = d - top + 1;
     const int off = lv[top], cnt = lv[top + 1] - off;
     if (cnt <= 0) return fail(c, ASTER_ERR_STATE, "radix tree depth census has an empty level");
-    if ((rc = launch_moments_band(c, vel, h, off, cnt))) return rc;
+    if ((rc = launch_moments_band(c, vel, h, off, cnt, kids))) return rc;
     d = top - 1;
   }
   ASTER_HIP(c, hipGetLastError());
   phase_end(c, ASTER_PHASE_MOMENTS);
   phase_begin(c, ASTER_PHASE_LINKS);
-  hipLaunchKernelGGL(k_links, dim3((nn + 255) / 256), dim3(256), 0, c->stream, n, c->right.p, c->escape.p,
-                     c->depth.p, c->nbox.p, c->nrec.p, c->boxd.p, c->P.theta, mac, c->nlink.p, c->idx.p,
-                     c->parent.p, c->left.p, full ? nullptr : c->kids.p);
-  ASTER_HIP(c, hipGetLastError());
+  if (full) {  // the group traversal's child slots were written with the moments
+    hipLaunchKernelGGL(k_links, dim3((nn + 255) / 256), dim3(256), 0, c->stream, n, c->right.p, c->escape.p,
+                       c->depth.p, c->nbox.p, c->nrec.p, c->boxd.p, c->P.theta, mac, c->nlink.p, c->idx.p);
+    ASTER_HIP(c, hipGetLastError());
+  }
   phase_end(c, ASTER_PHASE_LINKS);
   c->tree_valid = true;
   c->tree_mac = mac;
