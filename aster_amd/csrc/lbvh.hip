@@ -271,7 +271,7 @@ __global__ __launch_bounds__(256) void k_build(int n, const uint32_t* __restrict
                                                float4* __restrict__ nbox, float4* __restrict__ nrec,
                                                float4* __restrict__ gsh, int32_t* __restrict__ splitnode,
                                                uint32_t* __restrict__ dkey, uint32_t* __restrict__ dval,
-                                               uint32_t* __restrict__ flags) {
+                                               uint32_t* __restrict__ flags, int leaf_records) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   {
@@ -281,11 +281,13 @@ __global__ __launch_bounds__(256) void k_build(int n, const uint32_t* __restrict
     nmc[k] = pm;
     if (VEL) nvel[k] = make_double4(v[b], v[n + b], v[2 * n + b], 0.0);
     const float fx = (float)px, fy = (float)py, fz = (float)pz;
-    const float4 p = make_float4(fx, fy, fz, 0.f);
-    nbox[2 * k] = p;
-    nbox[2 * k + 1] = p;
     const float4 r = make_float4(fx, fy, fz, (float)mb);
-    nrec[k] = r;
+    if (leaf_records) {  // the moment passes form a leaf's box from nmc themselves
+      const float4 p = make_float4(fx, fy, fz, 0.f);
+      nbox[2 * k] = p;
+      nbox[2 * k + 1] = p;
+      nrec[k] = r;
+    }
     gsh[k] = r;
     left[k] = -1;
     right[k] = -1;
@@ -492,14 +494,23 @@ __device__ __forceinline__ void put_kid(float4* __restrict__ kids, int slot, int
   kids[2 * slot + 1] = make_float4(__int_as_float(node), geo_crit2(lo, hi, r, theta), 0.f, 0.f);
 }
 
+// A leaf's box is its position rounded to FP32 (what k_build would store).
+__device__ __forceinline__ float4 leaf_box(const double4& com) {
+  return make_float4((float)com.x, (float)com.y, (float)com.z, 0.f);
+}
+
 template <bool VEL>
-__device__ __forceinline__ MomRec load_moments(int node, const double4* __restrict__ nmc,
+__device__ __forceinline__ MomRec load_moments(int n, int node, const double4* __restrict__ nmc,
                                                const double4* __restrict__ nvel, const float4* __restrict__ nbox) {
   MomRec r;
   r.com = nmc[node];
   r.vel = VEL ? nvel[node] : make_double4(0.0, 0.0, 0.0, 0.0);
-  r.lo = nbox[2 * node];
-  r.hi = nbox[2 * node + 1];
+  if (node < n) {
+    r.lo = r.hi = leaf_box(r.com);
+  } else {
+    r.lo = nbox[2 * node];
+    r.hi = nbox[2 * node + 1];
+  }
   return r;
 }
 
@@ -577,7 +588,7 @@ __global__ __launch_bounds__(256) void k_moments_band(int n, const int32_t* __re
   for (int l = 1; l <= H; ++l)
 #pragma unroll
     for (int k = 0; k < (1 << l); ++k)
-      if (id[l][k] >= 0 && !in[l][k]) rec[l][k] = load_moments<VEL>(id[l][k], nmc, nvel, nbox);
+      if (id[l][k] >= 0 && !in[l][k]) rec[l][k] = load_moments<VEL>(n, id[l][k], nmc, nvel, nbox);
   // bottom-up
 #pragma unroll
   for (int l = H - 1; l >= 0; --l) {
@@ -651,14 +662,12 @@ __global__ __launch_bounds__(256) void k_group_moments(int n, const int32_t* __r
       if (L < n) {
         cL = nmc[L];
         if (VEL) vL = nvel[L];
-        loL = nbox[2 * L];
-        hiL = nbox[2 * L + 1];
+        loL = hiL = leaf_box(cL);
       }
       if (R < n) {
         cR = nmc[R];
         if (VEL) vR = nvel[R];
-        loR = nbox[2 * R];
-        hiR = nbox[2 * R + 1];
+        loR = hiR = leaf_box(cR);
       }
     }
     bool done = !mine;
@@ -1375,7 +1384,7 @@ static int build_tree(aster_ctx* c, int mac, bool vel, bool full) {
   auto build = vel ? k_build<true> : k_build<false>;
   hipLaunchKernelGGL(build, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->codes.p, c->idx.p, c->xm4.p,
                      c->v.p, c->left.p, c->right.p, c->parent.p, c->grange.p, c->nmc.p, c->nvel.p, c->nbox.p,
-                     c->nrec.p, c->gsh.p, c->splitnode.p, c->codes2.p, c->idx2.p, c->flags.p);
+                     c->nrec.p, c->gsh.p, c->splitnode.p, c->codes2.p, c->idx2.p, c->flags.p, full ? 1 : 0);
   ASTER_HIP(c, hipGetLastError());
   // traversal groups (also the per-wavefront subtrees of the moment pass)
   if ((rc = build_groups(c))) return rc;

@@ -34,12 +34,12 @@ HBM_PEAK_GBPS = 8000.0    # MI355X published HBM3E bandwidth
 #  morton: box pass 24 B (FP64 x) + Morton pass 24 B x + 8 B m read, 4 B code
 #          + 4 B index + 32 B {x, m} record written;
 #  sort: 4 passes x (4 B histogram read + 8 B key/value read + 8 B written);
-#  build: 4 B code + 4 B permutation + 32 B record read, 96 B of leaf records
-#         (FP64 {x, m}, FP32 box and record, FP32 sorted body) and 24 B of
-#         internal-node links (left, right, parent, range, split) written,
-#         plus the group pass's 12 B (start flag, climb, root);
+#  build: 4 B code + 4 B permutation + 32 B record read, 48 B of leaf records
+#         (FP64 {x, m}, FP32 sorted body) and 24 B of internal-node links
+#         (left, right, parent, range, split) written, plus the group pass's
+#         12 B (start flag, climb, root);
 #  kickdrift: FP64 x, v, a read (72 B), x, v written (48 B).
-PHASE_BYTES_PER_BODY = {"morton": 24 + 24 + 8 + 4 + 4 + 32, "sort": 4 * (4 + 8 + 8), "build": 4 + 4 + 32 + 96 + 24 + 12,
+PHASE_BYTES_PER_BODY = {"morton": 24 + 24 + 8 + 4 + 4 + 32, "sort": 4 * (4 + 8 + 8), "build": 4 + 4 + 32 + 48 + 24 + 12,
                         "kickdrift": 72 + 48}
 HBM_PEAK_GBS = 8000.0
 
