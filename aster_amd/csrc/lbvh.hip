@@ -562,22 +562,23 @@ __global__ __launch_bounds__(256) void k_moments_band(int n, const int32_t* __re
     }
   id[0][0] = bucket[t];
   in[0][0] = true;
+  // one round of loads per level: a node's children and its own leaf range,
+  // which gives the children's ranges (the left child covers [lo, gamma], the
+  // right one [gamma + 1, hi], gamma being the split)
 #pragma unroll
   for (int l = 0; l < H; ++l) {
 #pragma unroll
     for (int k = 0; k < (1 << l); ++k) {
       if (in[l][k]) {
-        id[l + 1][2 * k] = left[id[l][k]];
-        id[l + 1][2 * k + 1] = right[id[l][k]];
-      }
-    }
-    if (l + 1 < H) {
-#pragma unroll
-      for (int k = 0; k < (2 << l); ++k) {
-        const int c = id[l + 1][k];
-        if (c >= n) {
-          const int2 r = grange[c - n];
-          in[l + 1][k] = r.y - r.x + 1 > kGroup;
+        const int node = id[l][k];
+        const int L = left[node], R = right[node];
+        const int2 r = grange[node - n];
+        id[l + 1][2 * k] = L;
+        id[l + 1][2 * k + 1] = R;
+        if (l + 1 < H) {
+          const int gamma = L < n ? L : L - n;
+          in[l + 1][2 * k] = L >= n && gamma - r.x + 1 > kGroup;
+          in[l + 1][2 * k + 1] = R >= n && r.y - gamma > kGroup;
         }
       }
     }
