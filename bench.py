@@ -152,7 +152,8 @@ def bench_scaling(A, ctx, method, sizes=(18, 20, 22, 24, 26)):
     for lg in sizes:
         n = 1 << lg
         steps = 10 if lg <= 22 else (5 if lg <= 24 else 3)
-        r = bench_bh(A, ctx, steps, 1, method, n=n, reference=False)
+        warmup = 20 if lg <= 22 else 3  # at least ~30 ms of steps before timing
+        r = bench_bh(A, ctx, steps, warmup, method, n=n, reference=False)
         ms_t, cnt_t = r["phases"]["traverse"]
         step_ms = r["wall_s"] / steps * 1e3
         inter = r["interactions"] / steps
@@ -455,7 +456,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    # the GPU takes ~30 ms of sustained work (about 20 steps at N = 2^20) to reach its steady clock
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--method", default="BH_GROUP")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward ratio")
