@@ -604,6 +604,186 @@ int orc_bh(int64_t n, const double* pos, const double* vel, const double* mass, 
   return 0;
 }
 
+// Group traversal (no counterpart in the reference; the library's
+// ASTER_BH_GROUP): the tree of make_tree, groups = subtrees of at most 64
+// leaves whose parent has more. For each group, starting at the root, a node
+// is accepted when it is a leaf or when the squared distance from its FP32
+// COM to the group's FP32 box exceeds crit2 = (l / theta + delta)^2 (l the
+// longest side of the node's FP32 box, delta = |COM - box centre|), and opened
+// otherwise. The decisions use the same FP32 operations as the library (boxes
+// from FP32-rounded positions, fmaf in the distance), so the accepted sets are
+// the same; every body of the group then sums G m (r - x) / (d^2 + eps)^(3/2)
+// over them in FP64 from the FP32 records and the FP32 body position. Counts:
+// interactions = sum over groups of accepted x bodies - bodies (self pairs),
+// visits = sum over groups of 1 + 2 x opened. Pinned by: theta -> 0 equals
+// all-pairs on the FP32-rounded input with n (n - 1) interactions; the accepted
+// sets agree with an ancestor-chain brute force (a node is accepted iff it
+// passes and every ancestor was opened).
+namespace {
+struct GroupTree {
+  std::vector<int32_t> parent;
+  std::vector<int64_t> size;                 // leaves below each node
+  std::vector<float> lo, hi, rec, crit2;     // 3, 3, 4 per node and 1 per node
+};
+
+float geo_crit2_f(const float* lo, const float* hi, const float* r, double theta) {
+  const float l = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+  const float cx = 0.5f * (hi[0] + lo[0]);
+  const float cy = 0.5f * (hi[1] + lo[1]);
+  const float cz = 0.5f * (hi[2] + lo[2]);
+  const float ex = r[0] - cx, ey = r[1] - cy, ez = r[2] - cz;
+  const float delta = std::sqrt(ex * ex + ey * ey + ez * ez);
+  const float q = l / (float)theta + delta;
+  return q * q;
+}
+
+void group_tree(int64_t n, const Tree& T, double theta, GroupTree& G) {
+  const int64_t nn = 2 * n - 1;
+  G.parent.assign(nn, -1);
+  G.size.assign(nn, 1);
+  G.lo.assign(3 * nn, 0.f);
+  G.hi.assign(3 * nn, 0.f);
+  G.rec.assign(4 * nn, 0.f);
+  G.crit2.assign(nn, 0.f);
+  for (int64_t k = n; k < nn; ++k) {
+    G.parent[T.left[k]] = (int32_t)k;
+    G.parent[T.right[k]] = (int32_t)k;
+  }
+  for (int64_t k = 0; k < nn; ++k) {
+    for (int c = 0; c < 3; ++c) G.rec[4 * k + c] = (float)T.com[3 * k + c];
+    G.rec[4 * k + 3] = (float)T.mass[k];
+  }
+  for (int64_t k = 0; k < n; ++k)
+    for (int c = 0; c < 3; ++c) G.lo[3 * k + c] = G.hi[3 * k + c] = (float)T.com[3 * k + c];
+  // internal nodes bottom-up: post-order from the root
+  std::vector<std::pair<int64_t, int>> st{{n, 0}};
+  while (!st.empty()) {
+    auto& top = st.back();
+    const int64_t k = top.first;
+    if (k < n) {
+      st.pop_back();
+      continue;
+    }
+    if (top.second < 2) {
+      const int side = top.second++;
+      st.push_back({side == 0 ? T.left[k] : T.right[k], 0});
+      continue;
+    }
+    st.pop_back();
+    const int64_t L = T.left[k], R = T.right[k];
+    G.size[k] = G.size[L] + G.size[R];
+    for (int c = 0; c < 3; ++c) {
+      G.lo[3 * k + c] = std::min(G.lo[3 * L + c], G.lo[3 * R + c]);
+      G.hi[3 * k + c] = std::max(G.hi[3 * L + c], G.hi[3 * R + c]);
+    }
+  }
+  for (int64_t k = 0; k < nn; ++k) G.crit2[k] = geo_crit2_f(&G.lo[3 * k], &G.hi[3 * k], &G.rec[4 * k], theta);
+}
+
+// squared distance from the FP32 point r to the box [blo, bhi], as the library forms it
+float box_d2_f(const float* blo, const float* bhi, const float* r) {
+  float e[3];
+  for (int c = 0; c < 3; ++c) e[c] = std::max(std::max(blo[c] - r[c], r[c] - bhi[c]), 0.f);
+  return std::fmaf(e[0], e[0], std::fmaf(e[1], e[1], e[2] * e[2]));
+}
+}  // namespace
+
+extern "C" {
+int orc_bh_group(int64_t n, const double* pos, const double* mass, const double* box, double theta, double G,
+                 double soft, double* acc, int64_t* interactions, int64_t* visits, int nthreads) {
+  if (n < 2) {
+    for (int64_t i = 0; i < 3 * n; ++i) acc[i] = 0.0;
+    if (interactions) *interactions = 0;
+    if (visits) *visits = 0;
+    return 0;
+  }
+  std::vector<double> zero(3 * n, 0.0);
+  BuiltTree B;
+  make_tree(n, pos, zero.data(), mass, box, B, nthreads);
+  GroupTree GT;
+  group_tree(n, B.T, theta, GT);
+  std::vector<int64_t> roots;
+  for (int64_t k = 0; k < 2 * n - 1; ++k)
+    if (GT.size[k] <= 64 && (GT.parent[k] < 0 || GT.size[GT.parent[k]] > 64)) roots.push_back(k);
+  // the leaf range of a subtree: leftmost and rightmost descendants
+  auto first_leaf = [&](int64_t k) { while (k >= n) k = B.T.left[k]; return k; };
+  const double eps = (double)(float)soft;
+  std::vector<int64_t> g_inter(roots.size()), g_vis(roots.size());
+  parallel_for((int64_t)roots.size(), nthreads, [&](int64_t g) {
+    const int64_t root = roots[g], lo = first_leaf(root), cnt = GT.size[root];
+    const float* blo = &GT.lo[3 * root];
+    const float* bhi = &GT.hi[3 * root];
+    std::vector<int64_t> list, stack{n};
+    int64_t opened = 0;
+    while (!stack.empty()) {
+      const int64_t k = stack.back();
+      stack.pop_back();
+      if (k < n || box_d2_f(blo, bhi, &GT.rec[4 * k]) > GT.crit2[k]) {
+        list.push_back(k);
+      } else {
+        ++opened;
+        stack.push_back(B.T.left[k]);
+        stack.push_back(B.T.right[k]);
+      }
+    }
+    std::sort(list.begin(), list.end());  // a fixed summation order
+    for (int64_t s = lo; s < lo + cnt; ++s) {
+      const double xf = (double)(float)B.T.com[3 * s], yf = (double)(float)B.T.com[3 * s + 1],
+                   zf = (double)(float)B.T.com[3 * s + 2];
+      double ax = 0.0, ay = 0.0, az = 0.0;
+      for (int64_t k : list) {
+        const float* q = &GT.rec[4 * k];
+        const double dx = (double)q[0] - xf, dy = (double)q[1] - yf, dz = (double)q[2] - zf;
+        const double r2 = dx * dx + dy * dy + dz * dz + eps;
+        if (r2 == 0.0) continue;
+        const double f = (double)q[3] / (r2 * std::sqrt(r2));
+        ax += f * dx;
+        ay += f * dy;
+        az += f * dz;
+      }
+      const uint32_t b = B.perm[s];
+      acc[3 * b] = G * ax;
+      acc[3 * b + 1] = G * ay;
+      acc[3 * b + 2] = G * az;
+    }
+    g_inter[g] = (int64_t)list.size() * cnt - cnt;
+    g_vis[g] = 1 + 2 * opened;
+  });
+  if (interactions) *interactions = std::accumulate(g_inter.begin(), g_inter.end(), (int64_t)0);
+  if (visits) *visits = std::accumulate(g_vis.begin(), g_vis.end(), (int64_t)0);
+  return 0;
+}
+
+// The accepted sets of orc_bh_group counted without a walk: node k is accepted
+// for group g iff it passes (leaf or outside crit2) and every ancestor of k
+// fails. Returns the interaction count; test infrastructure for the walk.
+int64_t orc_bh_group_bruteforce_count(int64_t n, const double* pos, const double* mass, const double* box,
+                                      double theta) {
+  if (n < 2) return 0;
+  std::vector<double> zero(3 * n, 0.0);
+  BuiltTree B;
+  make_tree(n, pos, zero.data(), mass, box, B, 0);
+  GroupTree GT;
+  group_tree(n, B.T, theta, GT);
+  int64_t total = 0;
+  for (int64_t g = 0; g < 2 * n - 1; ++g) {
+    if (!(GT.size[g] <= 64 && (GT.parent[g] < 0 || GT.size[GT.parent[g]] > 64))) continue;
+    auto pass = [&](int64_t k) {
+      return k < n || box_d2_f(&GT.lo[3 * g], &GT.hi[3 * g], &GT.rec[4 * k]) > GT.crit2[k];
+    };
+    int64_t acc = 0;
+    for (int64_t k = 0; k < 2 * n - 1; ++k) {
+      if (!pass(k)) continue;
+      bool ok = true;
+      for (int64_t a = GT.parent[k]; a >= 0 && ok; a = GT.parent[a]) ok = a >= n && !pass(a);
+      acc += ok;
+    }
+    total += acc * GT.size[g] - GT.size[g];
+  }
+  return total;
+}
+}  // extern "C"
+
 // Graphs::get_total_energy, collectors.cpp:19-53 (serial order), plus the
 // Plummer-consistent potential -G m_i m_j / sqrt(r^2 + eps^2) and the total
 // angular momentum sum m x cross v (no reference counterpart).

@@ -31,6 +31,8 @@ _SIG = {
     "orc_karras": [_I64, _PU32, _PI32, _PI32],
     "orc_build_tree": [_I64, _D, _D, _D, _D, _PU32, _PU32, _PI32, _PI32, _D, _D, _D],
     "orc_bh": [_I64, _D, _D, _D, _D, _dbl, _dbl, _dbl, _dbl, ctypes.c_int, _D, _PI64, ctypes.c_int],
+    "orc_bh_group": [_I64, _D, _D, _D, _dbl, _dbl, _dbl, _D, _PI64, _PI64, ctypes.c_int],
+    "orc_bh_group_bruteforce_count": [_I64, _D, _D, _D, _dbl],
     "orc_energy": [_I64, _D, _D, _D, _dbl, _dbl, _D, _D, _D],
     "orc_yoshida_weights": [_D],
     "orc_integrate": [_I64, _D, _D, _D, _dbl, _dbl, _dbl, ctypes.c_int, ctypes.c_int, _D, _dbl, ctypes.c_int,
@@ -89,7 +91,7 @@ def lib():
         for k, args in _SIG.items():
             f = getattr(L, k)
             f.argtypes = args
-            f.restype = ctypes.c_int
+            f.restype = ctypes.c_int64 if k == "orc_bh_group_bruteforce_count" else ctypes.c_int
         _lib = L
     return _lib
 
@@ -173,6 +175,26 @@ def bh(pos, vel, mass, box, theta, G=1.0, c=1e4, soft=0.0, pn=1, threads=0, coun
                      None if cnt is None else cnt.ctypes.data_as(_PI64), threads)
     assert r == 0, "traversal stack overflow"
     return (acc, cnt) if counts else acc
+
+
+def bh_group(pos, mass, box, theta, G=1.0, soft=0.0, threads=0):
+    """Group traversal restated (the library's ASTER_BH_GROUP): accelerations
+    (n, 3) in FP64 from the FP32 records, plus the interaction and node-visit
+    counts, which equal the library's counters exactly."""
+    pos, mass, box = _c(pos), _c(mass), _c(box)
+    n = len(mass)
+    acc = np.empty((n, 3))
+    inter = ctypes.c_int64()
+    vis = ctypes.c_int64()
+    lib().orc_bh_group(n, _dp(pos), _dp(mass), _dp(box), theta, G, soft, _dp(acc), ctypes.byref(inter),
+                       ctypes.byref(vis), threads)
+    return acc, inter.value, vis.value
+
+
+def bh_group_bruteforce_count(pos, mass, box, theta):
+    """Interaction count of the group traversal from an ancestor-chain brute force (no walk)."""
+    pos, mass, box = _c(pos), _c(mass), _c(box)
+    return int(lib().orc_bh_group_bruteforce_count(len(mass), _dp(pos), _dp(mass), _dp(box), theta))
 
 
 def energy(pos, vel, mass, G=1.0, soft=0.0):

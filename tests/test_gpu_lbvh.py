@@ -142,6 +142,31 @@ def test_bh_group_handles_partial_wavefronts_and_tiny_n(ctx, orc):
         assert np.median(e) < 1e-2, n
 
 
+@pytest.mark.parametrize("n,seed", [(2, 1), (3, 2), (64, 3), (65, 4), (100, 5), (4097, 6), (20000, 7),
+                                    (1 << 18, 8)])
+def test_bh_group_matches_oracle_lists(ctx, orc, n, seed):
+    """Same accepted node sets as the oracle's restatement of the group
+    traversal (interaction and node-visit counts equal exactly, for every
+    group size the input produces), and the same forces within FP32 rounding
+    (the library sums FP32 terms, Kahan-compensated across list flushes; the
+    oracle sums the same terms in FP64)."""
+    pos, vel, mass = mixed(n, seed=seed)
+    box = orc.box_auto(pos)
+    for theta in (0.3, 0.5, 0.7):
+        load(ctx, pos, vel, mass, theta=theta)
+        try:
+            ctx.bh_stats()  # clears counts left by earlier evaluations
+        except A.AsterError:
+            pass  # none yet on this context
+        acc = ctx.forces(A.BH_GROUP)
+        inter, visits = ctx.bh_stats()
+        ref, ref_inter, ref_visits = orc.bh_group(pos, mass, box, theta, soft=2.5e-5)
+        assert (inter, visits) == (ref_inter, ref_visits), (n, theta)
+        scale = np.linalg.norm(ref, axis=1)
+        e = np.linalg.norm(acc - ref, axis=1) / np.maximum(scale, np.median(scale))
+        assert np.median(e) < 1e-6 and e.max() < 1e-4, (n, theta, np.median(e), e.max())
+
+
 def force_errors(ctx, pos, vel, mass, method, thetas):
     load(ctx, pos, vel, mass)
     ref = ctx.forces(A.ALLPAIRS_FP64)

@@ -410,6 +410,84 @@ def test_bh_recursive_restatement(orc):
         np.testing.assert_allclose(a[i], recursive_bh(T, 96, i, pos, mass, size, 0.6, 1e-4), rtol=1e-11)
 
 
+def test_bh_group_theta_zero_is_allpairs_on_fp32_input(orc):
+    import aster_amd
+    n = 500
+    pos, vel, mass = aster_amd.ic_mixed(n, 21)
+    box = orc.box_auto(pos)
+    a, inter, visits = orc.bh_group(pos, mass, box, theta=1e-6, soft=2.5e-5)
+    assert inter == n * (n - 1)             # every group lists every leaf
+    assert visits % (2 * n - 1) == 0        # and opens all n - 1 internal nodes
+    f = np.float32
+    pf = pos.astype(f).astype(np.float64)
+    mf = mass.astype(f).astype(np.float64)
+    ref = orc.allpairs(pf, np.zeros_like(pf), mf, soft=float(f(2.5e-5)))
+    np.testing.assert_allclose(a, ref, rtol=1e-11, atol=1e-13 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("theta", [0.3, 0.5, 0.7, 1.5])
+def test_bh_group_walk_matches_bruteforce(orc, theta):
+    import aster_amd
+    pos, vel, mass = aster_amd.ic_mixed(1500, 22)
+    box = orc.box_auto(pos)
+    _, inter, _ = orc.bh_group(pos, mass, box, theta=theta, soft=2.5e-5)
+    assert inter == orc.bh_group_bruteforce_count(pos, mass, box, theta)
+
+
+def group_mac_count(T, n, pos, theta):
+    """Interactions of the group traversal, restated with numpy in FP64 from
+    the FP32 boxes: groups are subtrees of <= 64 leaves whose parent has more;
+    a node is accepted when it is a leaf or its COM lies farther than
+    l / theta + delta from the group's box."""
+    L, R = T["left"], T["right"]
+    nn = 2 * n - 1
+    lo = np.zeros((nn, 3)); hi = np.zeros((nn, 3)); size = np.ones(nn, np.int64)
+    parent = np.full(nn, -1)
+    parent[L[n:]] = np.arange(n, nn)
+    parent[R[n:]] = np.arange(n, nn)
+    leafpos = T["com"][:n].astype(np.float32).astype(np.float64)
+    lo[:n] = leafpos; hi[:n] = leafpos
+    order, stack = [], [n]
+    while stack:
+        k = stack.pop(); order.append(k)
+        if k >= n:
+            stack += [L[k], R[k]]
+    for k in reversed(order):
+        if k >= n:
+            lo[k] = np.minimum(lo[L[k]], lo[R[k]]); hi[k] = np.maximum(hi[L[k]], hi[R[k]])
+            size[k] = size[L[k]] + size[R[k]]
+    com = T["com"].astype(np.float32).astype(np.float64)
+    l = (hi - lo).max(1)
+    delta = np.linalg.norm(com - 0.5 * (hi + lo), axis=1)
+    crit = l / theta + delta
+    total = 0
+    for g in range(nn):
+        if size[g] > 64 or (parent[g] >= 0 and size[parent[g]] <= 64):
+            continue
+        accepted, stack = 0, [n]
+        while stack:
+            k = stack.pop()
+            e = np.maximum(np.maximum(lo[g] - com[k], com[k] - hi[g]), 0.0)
+            if k < n or np.sqrt(e @ e) > crit[k]:
+                accepted += 1
+            else:
+                stack += [L[k], R[k]]
+        total += accepted * size[g] - size[g]
+    return total
+
+
+def test_bh_group_mac_restatement(orc):
+    import aster_amd
+    n = 1200
+    pos, vel, mass = aster_amd.ic_mixed(n, 23)
+    box = orc.box_auto(pos)
+    T = orc.build_tree(pos, vel, mass, box)
+    for theta in (0.4, 0.8):
+        _, inter, _ = orc.bh_group(pos, mass, box, theta=theta, soft=2.5e-5)
+        # FP64 here against FP32 there: only nodes within an ulp of the threshold may differ
+        assert abs(group_mac_count(T, n, pos, theta) - inter) <= 1e-4 * inter
+
+
 # ------------------------------------------------------------------ energy --
 
 def test_energy_closed_form(orc):
