@@ -142,6 +142,7 @@ struct aster_ctx {
   aster::DBuf<float4> nbox;                        // FP32 AABB: {lo, 0}, {hi, 0} per node (2 (2n-1))
   aster::DBuf<float4> nrec;                        // FP32 node record: com, mass (2n-1)
   aster::DBuf<int4> nlink;                         // right, escape, crit2 bits, leaf body (2n-1)
+  aster::DBuf<int32_t> topslot;                    // top-levels moments: node - n -> list position (n)
   aster::DBuf<float4> kids;                        // group traversal: per child slot {record}, {node, crit2} (2(2n-1))
   aster::DBuf<unsigned long long> bh_counters;     // BHC_COUNT
   aster::DBuf<float4> gsh;                         // bodies in sorted order (FP32), n

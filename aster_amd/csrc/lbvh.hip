@@ -615,6 +615,89 @@ __global__ __launch_bounds__(256) void k_moments_band(int n, const int32_t* __re
   }
 }
 
+// The top levels above the groups (at most kTopNodes nodes in all) in one
+// workgroup: thread p owns the p-th node of the depth-sorted list. Its
+// children's ids and leaf ranges, then the records of the children that are
+// already complete (formed by the bands below, group roots, leaves), are
+// loaded up front; the levels are then formed bottom-up through LDS, one
+// barrier per level, instead of one band launch per 3 levels.
+constexpr int kTopNodes = 512;
+constexpr int kTopMaxLevels = 32;
+struct TopLevels {
+  int nlev;                    // levels 0 .. nlev - 1
+  int off[kTopMaxLevels + 1];  // their offsets in the depth-sorted list
+};
+
+template <bool VEL>
+__global__ __launch_bounds__(kTopNodes) void k_moments_top(int n, const int32_t* __restrict__ bucket, TopLevels tl,
+                                                           const int32_t* __restrict__ left,
+                                                           const int32_t* __restrict__ right,
+                                                           const int2* __restrict__ grange, double4* __restrict__ nmc,
+                                                           double4* __restrict__ nvel, float4* __restrict__ nbox,
+                                                           float4* __restrict__ nrec, double theta,
+                                                           float4* __restrict__ kids, int32_t* __restrict__ topslot) {
+  __shared__ double4 scom[kTopNodes];
+  __shared__ double4 svel[VEL ? kTopNodes : 1];
+  __shared__ float4 slo[kTopNodes], shi[kTopNodes];
+  const int p = threadIdx.x;
+  const bool mine = p < tl.off[tl.nlev];
+  int node = n, L = 0, R = 0, lev = 0;
+  bool inL = false, inR = false;
+  if (mine) {
+    node = bucket[p];
+    while (tl.off[lev + 1] <= p) ++lev;
+    L = left[node];
+    R = right[node];
+    const int2 r = grange[node - n];  // children cover [r.x, gamma] and [gamma + 1, r.y]
+    const int gamma = L < n ? L : L - n;
+    const bool below = lev + 1 < tl.nlev;  // children at a level formed here
+    inL = below && L >= n && gamma - r.x + 1 > kGroup;
+    inR = below && R >= n && r.y - gamma > kGroup;
+    topslot[node - n] = p;
+  }
+  __syncthreads();
+  MomRec cl{}, cr{};
+  int sl = -1, sr = -1;
+  if (mine) {
+    if (inL) sl = topslot[L - n];
+    else cl = load_moments<VEL>(n, L, nmc, nvel, nbox);
+    if (inR) sr = topslot[R - n];
+    else cr = load_moments<VEL>(n, R, nmc, nvel, nbox);
+  }
+  for (int d = tl.nlev - 1; d >= 0; --d) {
+    if (mine && lev == d) {
+      if (sl >= 0) {
+        cl.com = scom[sl];
+        if (VEL) cl.vel = svel[sl];
+        cl.lo = slo[sl];
+        cl.hi = shi[sl];
+      }
+      if (sr >= 0) {
+        cr.com = scom[sr];
+        if (VEL) cr.vel = svel[sr];
+        cr.lo = slo[sr];
+        cr.hi = shi[sr];
+      }
+      const MomRec me = unite_moments<VEL>(cl, cr);
+      scom[p] = me.com;
+      if (VEL) svel[p] = me.vel;
+      slo[p] = me.lo;
+      shi[p] = me.hi;
+      nmc[node] = me.com;
+      if (VEL) nvel[node] = me.vel;
+      nbox[2 * node] = me.lo;
+      nbox[2 * node + 1] = me.hi;
+      if (!kids) nrec[node] = node_rec(me.com);
+      if (kids) {  // this node's child slots (and the root's own slot)
+        put_kid(kids, kid_slot(n, node, 0), L, cl.com, cl.lo, cl.hi, theta);
+        put_kid(kids, kid_slot(n, node, 1), R, cr.com, cr.lo, cr.hi, theta);
+        if (node == n) put_kid(kids, 2 * (n - 1), n, me.com, me.lo, me.hi, theta);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Moments of every internal node inside a traversal group, one wavefront per
 // group. Internal node n+i's range starts or ends at i (Karras), so the
 // hi - lo internal nodes of a group with leaf range [lo, hi] are n+lo ..
@@ -1271,6 +1354,7 @@ static int reserve_tree(aster_ctx* c, bool full) {
   ASTER_HIP(c, c->groot.reserve(n));
   ASTER_HIP(c, c->climb.reserve(n));
   ASTER_HIP(c, c->dsorted.reserve(n));
+  ASTER_HIP(c, c->topslot.reserve(n));
   ASTER_HIP(c, c->grange.reserve(n));
   ASTER_HIP(c, c->nmc.reserve(nn));
   ASTER_HIP(c, c->nbox.reserve(2 * nn));
@@ -1438,16 +1522,27 @@ static int build_tree(aster_ctx* c, int mac, bool vel, bool full) {
   if (nlev < 0 || nlev >= kMaxLevels || maxdep < nlev - 1 || maxdep >= kMaxLevels - 1 || (nlev > 0 && lv[0] != 0))
     return fail(c, ASTER_ERR_STATE, "radix tree depth census is inconsistent");
   c->tree_levels = maxdep + 1;
-  // the nodes above the groups in bands of up to kBand levels, deepest band
-  // first (the children of a band's deepest nodes are group roots, leaves or
-  // nodes of the band below)
-  for (int d = nlev - 1; d >= 0;) {
-    const int top = d - kBand + 1 > 0 ? d - kBand + 1 : 0;
+  // the nodes above the groups: the top levels holding at most kTopNodes
+  // nodes in one workgroup, the rest in bands of up to kBand levels, deepest
+  // band first (the children of a band's deepest nodes are group roots,
+  // leaves or nodes of the band below)
+  int ntop = 0;
+  while (ntop < nlev && ntop < kTopMaxLevels && lv[ntop + 1] <= kTopNodes) ++ntop;
+  for (int d = nlev - 1; d >= ntop;) {
+    const int top = d - kBand + 1 > ntop ? d - kBand + 1 : ntop;
     const int h = d - top + 1;
     const int off = lv[top], cnt = lv[top + 1] - off;
     if (cnt <= 0) return fail(c, ASTER_ERR_STATE, "radix tree depth census has an empty level");
     if ((rc = launch_moments_band(c, vel, h, off, cnt, kids))) return rc;
     d = top - 1;
+  }
+  if (ntop > 0) {
+    TopLevels tl;
+    tl.nlev = ntop;
+    for (int d = 0; d <= ntop; ++d) tl.off[d] = lv[d];
+    auto topk = vel ? k_moments_top<true> : k_moments_top<false>;
+    hipLaunchKernelGGL(topk, dim3(1), dim3(kTopNodes), 0, c->stream, n, c->bucket.p, tl, c->left.p, c->right.p,
+                       c->grange.p, c->nmc.p, c->nvel.p, c->nbox.p, c->nrec.p, c->P.theta, kids, c->topslot.p);
   }
   ASTER_HIP(c, hipGetLastError());
   phase_end(c, ASTER_PHASE_MOMENTS);
