@@ -1039,13 +1039,15 @@ __global__ __launch_bounds__(256) void k_group_emit(int n, const uint32_t* __res
 // Wavefront-cooperative traversal, one group per wavefront (lanes = the
 // group's bodies, at most 64). The wavefront walks the tree once for the
 // whole group with a shared LDS stack of opened nodes, testing up to 64
-// candidate nodes per iteration (one per lane, so 64 independent node loads
-// are in flight). A node is accepted when the distance from its centre of
+// candidate nodes per iteration: lane 2 i + s takes child s of the i-th popped
+// node from that node's child slots (record, id and threshold of both
+// children in 64 contiguous bytes), so a batch costs one load round trip.
+// A node is accepted when the distance from its centre of
 // mass to the group's bounding box exceeds its opening radius (the geometric
 // test, so it holds for every body of the group); leaves are always accepted.
-// Accepted nodes are appended to an LDS interaction list that all lanes then
-// evaluate coherently (LDS broadcast reads, 12 VALU + 1 rsq per interaction),
-// FP32 sums per list flush, Kahan-compensated across flushes.
+// Accepted nodes are appended to an LDS interaction list that the lanes then
+// evaluate in pieces (below; 12 VALU + 1 rsq per entry and lane), FP32 sums
+// per list flush, Kahan-compensated across flushes.
 // Stack bound: each iteration pops the top min(sp, 32) entries and pushes
 // their opened children (at most 64) in order, so the stack stays sorted by
 // depth from bottom to top and a pop only ever leaves behind part of one
@@ -1067,13 +1069,13 @@ __global__ __launch_bounds__(256) void k_group_emit(int n, const uint32_t* __res
 // that order (deterministic). (P, q) depends only on g and is chosen, for a
 // list of ~96 entries, to minimise P (lam + 1) with P <= 3 and q <= 64; g = 64
 // gives P = q = 1, g = 42 gives P = 2, q = 3 (98% of the lanes busy instead
-// of 66%). Lane use measured over a 2^20-body tree: 73% with power-of-two
-// lanes per body, ~88% with pieces (simulated on the tree's group sizes).
+// of 66%). Lane use over a 2^20-body tree, from the rsq count: 90.5% (73%
+// with a power-of-two number of lanes per body of a group of <= 32).
 // The list holds up to kGrpList entries (flushed once it has more than
 // kGrpList - 64, since one round adds at most 64) plus the padding (< q <= 64
-// entries); the same LDS holds the P 64 partial sums after the passes. LDS
-// per wavefront bounds the occupancy of this kernel: 128 entries measured
-// 5-8% faster than 256 plus padding.
+// entries); the same LDS holds the P 64 partial sums after the passes.
+// 256-entry lists, a 4th pass and a smaller stack (more waves per CU) each
+// measured within 1%: the kernel is VALU-issue bound.
 constexpr int kGrpList = 128;
 constexpr int kGrpListPad = 64;  // room for that padding
 constexpr int kPieceMaxPasses = 3;
