@@ -34,9 +34,6 @@ constexpr int kMaxLevels = 128;
 // Traversal groups (and the moment kernel's per-wavefront subtrees): subtrees
 // of at most kGroup leaves whose parent has more.
 constexpr int kGroup = 64;
-// Depth-census key of the internal nodes inside the traversal groups (sorted
-// after every depth).
-constexpr uint32_t kInGroup = 255u;
 
 // ------------------------------------------------------------ Morton -------
 
@@ -270,7 +267,6 @@ __global__ __launch_bounds__(256) void k_build(int n, const uint32_t* __restrict
                                                double4* __restrict__ nmc, double4* __restrict__ nvel,
                                                float4* __restrict__ nbox, float4* __restrict__ nrec,
                                                float4* __restrict__ gsh, int32_t* __restrict__ splitnode,
-                                               uint32_t* __restrict__ dkey, uint32_t* __restrict__ dval,
                                                uint32_t* __restrict__ flags, int leaf_records) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
@@ -325,9 +321,6 @@ __global__ __launch_bounds__(256) void k_build(int n, const uint32_t* __restrict
   parent[R] = n + i;
   grange[i] = make_int2(lo, hi);
   splitnode[gamma] = n + i;  // every split position belongs to exactly one internal node
-  // depth census entry: in-group until k_above_census says otherwise
-  dkey[i] = kInGroup;
-  dval[i] = (uint32_t)(n + i);
 }
 
 // Depth of every node and its escape link in the right-first preorder (the
@@ -361,8 +354,8 @@ __global__ __launch_bounds__(256) void k_depth(int n, const int32_t* __restrict_
 // split lies at a boundary between consecutive groups (a node above the
 // groups splits between two of them, and each boundary is one node's
 // split), so thread t < ngroups - 1 takes boundary t, walks up for the
-// node's depth and makes that depth the node's census key (k_build keyed
-// every internal node kInGroup). Thread ngroups - 1 + g walks up from group
+// node's depth and writes entry t of the compact (depth, node) list. Thread
+// ngroups - 1 + g walks up from group
 // g's root instead: with the deepest climb of the group's leaves
 // (k_group_flags) that gives the group's deepest internal node. The block's
 // largest internal-node depth goes to dmax[block].
@@ -383,8 +376,8 @@ __global__ __launch_bounds__(256) void k_above_census(int n, const int32_t* __re
                                                       const int32_t* __restrict__ groot,
                                                       const int32_t* __restrict__ climb,
                                                       const int32_t* __restrict__ parent, int32_t* __restrict__ depth,
-                                                      uint32_t* __restrict__ dkey, int32_t* __restrict__ dmax,
-                                                      uint32_t* __restrict__ flags) {
+                                                      uint32_t* __restrict__ cdep, uint32_t* __restrict__ cnode,
+                                                      int32_t* __restrict__ dmax, uint32_t* __restrict__ flags) {
   __shared__ int wmax[4];
   const int ng = (int)*gcount;
   int mx = 0;
@@ -393,7 +386,8 @@ __global__ __launch_bounds__(256) void k_above_census(int n, const int32_t* __re
       const int node = splitnode[gstart[t + 1] - 1];
       const int d = walk_depth(node, parent, flags);
       depth[node] = d;
-      dkey[node - n] = (uint32_t)d;
+      cdep[t] = (uint32_t)d;  // one entry per node above the groups
+      cnode[t] = (uint32_t)node;
       mx = max(mx, d);
     } else {
       const int g = t - (ng - 1);
@@ -409,43 +403,47 @@ __global__ __launch_bounds__(256) void k_above_census(int n, const int32_t* __re
   if (threadIdx.x == 0) dmax[blockIdx.x] = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
 }
 
-// Largest of count ints (one block), to out.
-__global__ __launch_bounds__(1024) void k_max_i32(const int32_t* __restrict__ in, int count, int32_t* __restrict__ out) {
-  __shared__ int wmax[16];
-  int m = 0;
-  for (int i = threadIdx.x; i < count; i += blockDim.x) m = max(m, in[i]);
-  for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
-  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+// The census in one workgroup: count the nodes above the groups per depth
+// (LDS histogram), write the level offsets, the number of levels and the
+// largest depth of any internal node (over the census blocks' maxima) to
+// levels[], and list the nodes level by level in bucket[]. The order inside a
+// level is arbitrary: each node's moments depend on its children only.
+constexpr int kCensusThreads = 1024;
+__global__ __launch_bounds__(kCensusThreads) void k_census_levels(const uint32_t* __restrict__ gcount,
+                                                                  const uint32_t* __restrict__ cdep,
+                                                                  const uint32_t* __restrict__ cnode,
+                                                                  const int32_t* __restrict__ dmax, int cblocks,
+                                                                  int32_t* __restrict__ bucket,
+                                                                  int32_t* __restrict__ levels) {
+  __shared__ uint32_t hist[kMaxLevels];
+  __shared__ uint32_t cur[kMaxLevels];
+  __shared__ int wmax[kCensusThreads / 64];
+  const int t = threadIdx.x;
+  const int cnt = (int)*gcount - 1;  // one node above the groups per boundary between groups
+  if (t < kMaxLevels) hist[t] = 0u;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    int r = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r = max(r, wmax[w]);
-    *out = r;
-  }
-}
-
-// Bucket bounds from the sorted keys (depths of the nodes above the groups,
-// then kInGroup): off[d] = first slot of depth d for d < L, off[L] = first
-// in-group slot (or count), off[kMaxLevels + 1] = L, the number of depths
-// above the groups. Those depths have no gaps (the parent of such a node is
-// one too, one level up).
-__global__ __launch_bounds__(256) void k_level_bounds(int count, const uint32_t* __restrict__ key,
-                                                      int32_t* __restrict__ off) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= count) return;
-  const uint32_t d = key[p];
-  const bool first = p == 0 || key[p - 1] != d;
-  if (d != kInGroup) {
-    if (first) off[d] = p;
-    if (p == count - 1) {
-      off[d + 1] = count;
-      off[kMaxLevels + 1] = (int32_t)d + 1;
+  for (int i = t; i < cnt; i += kCensusThreads) atomicAdd(&hist[cdep[i]], 1u);
+  int mx = 0;
+  for (int i = t; i < cblocks; i += kCensusThreads) mx = max(mx, dmax[i]);
+  for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off, 64));
+  if ((t & 63) == 0) wmax[t >> 6] = mx;
+  __syncthreads();
+  if (t == 0) {
+    uint32_t run = 0;
+    int nlev = 0;
+    for (int d = 0; d < kMaxLevels; ++d) {
+      cur[d] = run;
+      levels[d] = (int32_t)run;
+      if (hist[d]) nlev = d + 1;
+      run += hist[d];
     }
-  } else if (first) {
-    const int32_t L = p == 0 ? 0 : (int32_t)key[p - 1] + 1;
-    off[L] = p;
-    off[kMaxLevels + 1] = L;
+    int m = 0;
+    for (int w = 0; w < kCensusThreads / 64; ++w) m = max(m, wmax[w]);
+    levels[kMaxLevels + 1] = nlev;
+    levels[kMaxLevels + 2] = m;
   }
+  __syncthreads();
+  for (int i = t; i < cnt; i += kCensusThreads) bucket[atomicAdd(&cur[cdep[i]], 1u)] = (int32_t)cnode[i];
 }
 
 // Band of up to H levels above the groups, one thread per node at the band's
@@ -1355,7 +1353,6 @@ static int reserve_tree(aster_ctx* c, bool full) {
   ASTER_HIP(c, c->splitnode.reserve(n));
   ASTER_HIP(c, c->groot.reserve(n));
   ASTER_HIP(c, c->climb.reserve(n));
-  ASTER_HIP(c, c->dsorted.reserve(n));
   ASTER_HIP(c, c->topslot.reserve(n));
   ASTER_HIP(c, c->grange.reserve(n));
   ASTER_HIP(c, c->nmc.reserve(nn));
@@ -1471,7 +1468,7 @@ static int build_tree(aster_ctx* c, int mac, bool vel, bool full) {
   auto build = vel ? k_build<true> : k_build<false>;
   hipLaunchKernelGGL(build, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->codes.p, c->idx.p, c->xm4.p,
                      c->v.p, c->left.p, c->right.p, c->parent.p, c->grange.p, c->nmc.p, c->nvel.p, c->nbox.p,
-                     c->nrec.p, c->gsh.p, c->splitnode.p, c->codes2.p, c->idx2.p, c->flags.p, full ? 1 : 0);
+                     c->nrec.p, c->gsh.p, c->splitnode.p, c->flags.p, full ? 1 : 0);
   ASTER_HIP(c, hipGetLastError());
   // traversal groups (also the per-wavefront subtrees of the moment pass)
   if ((rc = build_groups(c))) return rc;
@@ -1479,24 +1476,16 @@ static int build_tree(aster_ctx* c, int mac, bool vel, bool full) {
   const int nn = 2 * n - 1;
   phase_begin(c, ASTER_PHASE_MOMENTS);
   // Depth census of the nodes above the groups: their depths (walks from the
-  // group boundaries, and from the group roots for the largest depth) key
-  // (depth, node) pairs, the in-group nodes keyed last, counting-sorted with
-  // one stable radix pass (depth < 128), so every level is a contiguous
-  // bucket. The sort scratch of the Morton pass holds the pairs.
-  const int ni = n - 1;
+  // group boundaries, and from the group roots for the largest depth) as a
+  // compact (depth, node) list, then one workgroup lists them level by level
+  // (depth < 128). The sort scratch of the Morton pass holds the list.
   const int cblocks = (int)std::min<int64_t>((2 * (int64_t)n / 32 + 2 + 255) / 256, 4096);
   ASTER_HIP(c, c->dmax.reserve((size_t)cblocks));
   hipLaunchKernelGGL(k_above_census, dim3(cblocks), dim3(256), 0, c->stream, n, c->gstart.p, c->gcount.p,
-                     c->splitnode.p, c->groot.p, c->climb.p, c->parent.p, c->depth.p, c->codes2.p, c->dmax.p,
-                     c->flags.p);
-  hipLaunchKernelGGL(k_max_i32, dim3(1), dim3(1024), 0, c->stream, c->dmax.p, cblocks, c->levels.p + kMaxLevels + 2);
-  const int nblocks = (ni + kRsTile - 1) / kRsTile;
-  hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(kRsThreads), 0, c->stream, c->codes2.p, ni, 0, c->hist.p,
-                     nblocks);
-  if ((rc = device_scan(c, c->hist.p, 256 * nblocks, nullptr))) return rc;
-  hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(kRsThreads), 0, c->stream, c->codes2.p, c->idx2.p,
-                     c->dsorted.p, reinterpret_cast<uint32_t*>(c->bucket.p), ni, 0, c->hist.p, nblocks);
-  hipLaunchKernelGGL(k_level_bounds, dim3((ni + 255) / 256), dim3(256), 0, c->stream, ni, c->dsorted.p, c->levels.p);
+                     c->splitnode.p, c->groot.p, c->climb.p, c->parent.p, c->depth.p, c->codes2.p, c->idx2.p,
+                     c->dmax.p, c->flags.p);
+  hipLaunchKernelGGL(k_census_levels, dim3(1), dim3(kCensusThreads), 0, c->stream, c->gcount.p, c->codes2.p,
+                     c->idx2.p, c->dmax.p, cblocks, c->bucket.p, c->levels.p);
   if (full)
     hipLaunchKernelGGL(k_depth, dim3((nn + 255) / 256), dim3(256), 0, c->stream, n, c->left.p, c->right.p,
                        c->parent.p, c->depth.p, c->escape.p, c->flags.p);
