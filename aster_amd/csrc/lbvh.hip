@@ -160,35 +160,40 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(uint32_t* __restric
   if (total && base < count && base + kScanPer >= count) *total = run;
 }
 
-// Stable scatter: rank of a key among equal digits = earlier rounds (run[]),
-// earlier waves of this round (wc[][]), earlier lanes of its wave (ballot
-// match mask). The tile is first reordered by digit in LDS, then written out
-// in that order, so consecutive lanes write consecutive addresses within
-// each digit's run.
+// Stable scatter. Each wavefront owns a contiguous quarter of the tile (512
+// keys, in 8 rounds of 64) and ranks its keys among equal digits with its own
+// running counters in LDS: rank = the wavefront's earlier keys of the digit
+// (wcnt[w][d]) + earlier lanes of the round (ballot match mask). LDS
+// operations of one wavefront execute in order, so the rounds need no
+// workgroup barrier. Then the tile's digit totals give each digit's start in
+// the tile (lpre) and each wavefront's offset inside it; the tile is
+// reordered by digit in LDS and written out with consecutive lanes on
+// consecutive addresses within each digit's run.
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t* __restrict__ kin,
                                                            const uint32_t* __restrict__ vin,
                                                            uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                            int n, int shift, const uint32_t* __restrict__ hist,
                                                            int nblocks) {
-  constexpr int R = kRsTile / kRsThreads;
-  __shared__ uint32_t run[256];
+  constexpr int W = kRsThreads / 64;   // wavefronts per tile
+  constexpr int C = kRsTile / W;       // keys per wavefront
+  constexpr int R = C / 64;            // rounds per wavefront
   __shared__ uint32_t gbase[256];
-  __shared__ uint32_t wc[4][256];
+  __shared__ uint32_t wcnt[W][256];
+  __shared__ uint32_t wsum[W];
   __shared__ uint32_t lpre[256];
   __shared__ uint32_t skey[kRsTile], sval[kRsTile];
   const int t = threadIdx.x;
   const int lane = t & 63, w = t >> 6;
-  run[t] = 0;
   gbase[t] = hist[(size_t)t * nblocks + blockIdx.x];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) wc[q][t] = 0;
+  for (int q = 0; q < W; ++q) wcnt[q][t] = 0;
   __syncthreads();
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int base = blockIdx.x * kRsTile;
+  const int base = blockIdx.x * kRsTile + w * C;
   uint32_t key[R], val[R], pos[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int p = base + r * kRsThreads + t;
+    const int p = base + r * 64 + lane;
     const bool valid = p < n;
     key[r] = valid ? kin[p] : 0u;
     val[r] = valid ? vin[p] : 0u;
@@ -201,42 +206,46 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t* __res
       m &= bit ? bal : ~bal;
     }
     const uint32_t rank = (uint32_t)__popcll(m & lt);
-    if (valid && rank == 0) wc[w][d] = (uint32_t)__popcll(m);
-    __syncthreads();
-    uint32_t off = run[d];
-    for (int q = 0; q < w; ++q) off += wc[q][d];
-    pos[r] = valid ? off + rank : 0xFFFFFFFFu;  // rank among this tile's keys of digit d
-    __syncthreads();
-    run[t] += wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) wc[q][t] = 0;
-    __syncthreads();
+    const uint32_t prev = wcnt[w][d];
+    wave_sync();  // every lane has read its digit's count before the leaders update it
+    if (valid && rank == 0) wcnt[w][d] = prev + (uint32_t)__popcll(m);
+    wave_sync();
+    pos[r] = valid ? prev + rank : 0xFFFFFFFFu;  // rank among this wavefront's keys of digit d
   }
-  // exclusive prefix of the tile's digit counts
+  __syncthreads();
+  // digit t: its total in the tile, each wavefront's offset inside it, and
+  // the exclusive prefix of the totals over the digits
   {
-    const uint32_t c = run[t];
-    uint32_t inc = c;
+    uint32_t c[W], tot = 0;
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+      c[q] = wcnt[q][t];
+      wcnt[q][t] = tot;
+      tot += c[q];
+    }
+    uint32_t inc = tot;
     for (int off = 1; off < 64; off <<= 1) {
       const uint32_t y = __shfl_up(inc, off, 64);
       if (lane >= off) inc += y;
     }
-    if (lane == 63) wc[0][w] = inc;
+    if (lane == 63) wsum[w] = inc;
     __syncthreads();
-    uint32_t pre = inc - c;
-    for (int q = 0; q < w; ++q) pre += wc[0][q];
+    uint32_t pre = inc - tot;
+    for (int q = 0; q < w; ++q) pre += wsum[q];
     lpre[t] = pre;
   }
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     if (pos[r] != 0xFFFFFFFFu) {
-      const uint32_t slot = lpre[(key[r] >> shift) & 255u] + pos[r];
+      const uint32_t d = (key[r] >> shift) & 255u;
+      const uint32_t slot = lpre[d] + wcnt[w][d] + pos[r];
       skey[slot] = key[r];
       sval[slot] = val[r];
     }
   }
   __syncthreads();
-  const int valid_here = min(kRsTile, n - base);
+  const int valid_here = min(kRsTile, n - (int)blockIdx.x * kRsTile);
   for (int j = t; j < valid_here; j += kRsThreads) {
     const uint32_t k = skey[j];
     const uint32_t d = (k >> shift) & 255u;
