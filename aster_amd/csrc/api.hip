@@ -150,7 +150,7 @@ int aster_destroy(aster_ctx* c) {
   c->depth.release(); c->nmc.release(); c->nvel.release(); c->nbox.release();
   c->levels.release(); c->bucket.release(); c->grange.release(); c->gscan.release();
   c->gstart.release(); c->gcount.release(); c->splitnode.release(); c->groot.release(); c->climb.release();
-  c->dmax.release();
+  c->dmax.release(); c->censusctr.release();
   c->nrec.release(); c->nlink.release(); c->kids.release(); c->topslot.release(); c->bh_counters.release(); c->gsh.release();
   auto& Bt = c->batch;
   Bt.x.release(); Bt.v.release(); Bt.a.release(); Bt.m.release(); Bt.ebody.release(); Bt.esys.release();

@@ -130,6 +130,7 @@ struct aster_ctx {
   aster::DBuf<int32_t> groot;                      // at a group's first position: the group's root node (n)
   aster::DBuf<int32_t> climb;                      // leaf k: levels from the leaf up to its group's root (n)
   aster::DBuf<int32_t> dmax;                       // per-block largest internal-node depth
+  aster::DBuf<uint32_t> censusctr;                 // census: per-depth counts and cursors (256)
   aster::DBuf<int32_t> levels;                     // per-depth bucket offsets, level count, max depth
   aster::DBuf<int32_t> bucket;                     // internal nodes grouped by depth, in-group ones last (n-1)
   aster::DBuf<int2> grange;                        // sorted-position range of internal node n+i

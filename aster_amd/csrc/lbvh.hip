@@ -386,8 +386,10 @@ __global__ __launch_bounds__(256) void k_above_census(int n, const int32_t* __re
                                                       const int32_t* __restrict__ climb,
                                                       const int32_t* __restrict__ parent, int32_t* __restrict__ depth,
                                                       uint32_t* __restrict__ cdep, uint32_t* __restrict__ cnode,
-                                                      int32_t* __restrict__ dmax, uint32_t* __restrict__ flags) {
+                                                      int32_t* __restrict__ dmax, uint32_t* __restrict__ gcnt,
+                                                      uint32_t* __restrict__ flags) {
   __shared__ int wmax[4];
+  if (blockIdx.x == 0) gcnt[threadIdx.x] = 0u;  // 256 census counters and cursors, for the passes after
   const int ng = (int)*gcount;
   int mx = 0;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < 2 * ng - 1; t += gridDim.x * blockDim.x) {
@@ -412,47 +414,117 @@ __global__ __launch_bounds__(256) void k_above_census(int n, const int32_t* __re
   if (threadIdx.x == 0) dmax[blockIdx.x] = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
 }
 
-// The census in one workgroup: count the nodes above the groups per depth
-// (LDS histogram), write the level offsets, the number of levels and the
-// largest depth of any internal node (over the census blocks' maxima) to
-// levels[], and list the nodes level by level in bucket[]. The order inside a
-// level is arbitrary: each node's moments depend on its children only.
-constexpr int kCensusThreads = 1024;
-__global__ __launch_bounds__(kCensusThreads) void k_census_levels(const uint32_t* __restrict__ gcount,
-                                                                  const uint32_t* __restrict__ cdep,
-                                                                  const uint32_t* __restrict__ cnode,
-                                                                  const int32_t* __restrict__ dmax, int cblocks,
-                                                                  int32_t* __restrict__ bucket,
-                                                                  int32_t* __restrict__ levels) {
-  __shared__ uint32_t hist[kMaxLevels];
-  __shared__ uint32_t cur[kMaxLevels];
-  __shared__ int wmax[kCensusThreads / 64];
-  const int t = threadIdx.x;
+// The census list ordered by level, in two passes over tiles of kRsTile
+// entries (8 per thread, 256 threads). k_census_count: per-tile depth counts
+// in LDS, added into gcnt[0..127]. k_census_place: every tile derives the
+// level offsets from gcnt, reserves a range of each level for its entries
+// with one atomic per depth on the cursors gcnt[128..255], and writes its
+// nodes there; tile 0 also writes levels[] (offsets, level count, largest
+// depth over the census blocks' maxima). The order inside a level is
+// arbitrary: each node's moments depend on its children only. Lanes of a
+// wavefront with equal depths (few distinct values, many nodes each) are
+// counted with one LDS atomic per distinct depth.
+__device__ __forceinline__ uint64_t depth_match(bool valid, uint32_t d, uint64_t lt, uint32_t& rank) {
+  uint64_t m = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 7; ++b) {  // depth < kMaxLevels = 2^7
+    const bool bit = (d >> b) & 1u;
+    const uint64_t bal = __ballot(bit);
+    m &= bit ? bal : ~bal;
+  }
+  rank = (uint32_t)__popcll(m & lt);
+  return m;
+}
+
+__global__ __launch_bounds__(kRsThreads) void k_census_count(const uint32_t* __restrict__ gcount,
+                                                             const uint32_t* __restrict__ cdep,
+                                                             uint32_t* __restrict__ gcnt) {
+  __shared__ uint32_t lcnt[kMaxLevels];
+  const int t = threadIdx.x, lane = t & 63;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int cnt = (int)*gcount - 1;  // one node above the groups per boundary between groups
-  if (t < kMaxLevels) hist[t] = 0u;
+  const int base = blockIdx.x * kRsTile;
+  if (base >= cnt) return;  // uniform over the block
+  if (t < kMaxLevels) lcnt[t] = 0u;
   __syncthreads();
-  for (int i = t; i < cnt; i += kCensusThreads) atomicAdd(&hist[cdep[i]], 1u);
-  int mx = 0;
-  for (int i = t; i < cblocks; i += kCensusThreads) mx = max(mx, dmax[i]);
-  for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off, 64));
-  if ((t & 63) == 0) wmax[t >> 6] = mx;
-  __syncthreads();
-  if (t == 0) {
-    uint32_t run = 0;
-    int nlev = 0;
-    for (int d = 0; d < kMaxLevels; ++d) {
-      cur[d] = run;
-      levels[d] = (int32_t)run;
-      if (hist[d]) nlev = d + 1;
-      run += hist[d];
-    }
-    int m = 0;
-    for (int w = 0; w < kCensusThreads / 64; ++w) m = max(m, wmax[w]);
-    levels[kMaxLevels + 1] = nlev;
-    levels[kMaxLevels + 2] = m;
+  uint32_t d[kRsTile / kRsThreads];
+#pragma unroll
+  for (int r = 0; r < kRsTile / kRsThreads; ++r) {
+    const int i = base + r * kRsThreads + t;
+    d[r] = i < cnt ? cdep[i] : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < kRsTile / kRsThreads; ++r) {
+    const bool valid = base + r * kRsThreads + t < cnt;
+    uint32_t rank;
+    const uint64_t m = depth_match(valid, d[r], lt, rank);
+    if (valid && rank == 0) atomicAdd(&lcnt[d[r]], (uint32_t)__popcll(m));
   }
   __syncthreads();
-  for (int i = t; i < cnt; i += kCensusThreads) bucket[atomicAdd(&cur[cdep[i]], 1u)] = (int32_t)cnode[i];
+  if (t < kMaxLevels && lcnt[t]) atomicAdd(&gcnt[t], lcnt[t]);
+}
+
+__global__ __launch_bounds__(kRsThreads) void k_census_place(const uint32_t* __restrict__ gcount,
+                                                             const uint32_t* __restrict__ cdep,
+                                                             const uint32_t* __restrict__ cnode,
+                                                             uint32_t* __restrict__ gcnt,
+                                                             const int32_t* __restrict__ dmax, int cblocks,
+                                                             int32_t* __restrict__ bucket,
+                                                             int32_t* __restrict__ levels) {
+  constexpr int R = kRsTile / kRsThreads;
+  __shared__ uint32_t off[kMaxLevels], lcnt[kMaxLevels], lbase[kMaxLevels];
+  __shared__ int wmax[kRsThreads / 64];
+  const int t = threadIdx.x, lane = t & 63;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int cnt = (int)*gcount - 1;
+  const int base = blockIdx.x * kRsTile;
+  if (blockIdx.x > 0 && base >= cnt) return;  // uniform over the block; tile 0 always writes levels[]
+  if (t < kMaxLevels) lcnt[t] = 0u;
+  if (t == 0) {
+    uint32_t run = 0;
+    for (int q = 0; q < kMaxLevels; ++q) {
+      off[q] = run;
+      run += gcnt[q];
+    }
+  }
+  __syncthreads();
+  uint32_t d[R], node[R], pos[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = base + r * kRsThreads + t;
+    d[r] = i < cnt ? cdep[i] : 0u;
+    node[r] = i < cnt ? cnode[i] : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const bool valid = base + r * kRsThreads + t < cnt;
+    uint32_t rank, first = 0;
+    const uint64_t m = depth_match(valid, d[r], lt, rank);
+    if (valid && rank == 0) first = atomicAdd(&lcnt[d[r]], (uint32_t)__popcll(m));
+    first = __shfl(first, (int)__builtin_ctzll(m | (1ull << lane)), 64);  // from the group's lowest lane
+    pos[r] = first + rank;
+  }
+  __syncthreads();
+  if (t < kMaxLevels && lcnt[t]) lbase[t] = off[t] + atomicAdd(&gcnt[kMaxLevels + t], lcnt[t]);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (base + r * kRsThreads + t < cnt) bucket[lbase[d[r]] + pos[r]] = (int32_t)node[r];
+  if (blockIdx.x == 0) {
+    int mx = 0;
+    for (int i = t; i < cblocks; i += kRsThreads) mx = max(mx, dmax[i]);
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+    if (lane == 0) wmax[t >> 6] = mx;
+    __syncthreads();
+    if (t < kMaxLevels) levels[t] = (int32_t)off[t];
+    if (t == 0) {
+      int nlev = 0;
+      for (int q = 0; q < kMaxLevels; ++q)
+        if (gcnt[q]) nlev = q + 1;
+      levels[kMaxLevels + 1] = nlev;
+      levels[kMaxLevels + 2] = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+    }
+  }
 }
 
 // Band of up to H levels above the groups, one thread per node at the band's
@@ -1486,15 +1558,19 @@ static int build_tree(aster_ctx* c, int mac, bool vel, bool full) {
   phase_begin(c, ASTER_PHASE_MOMENTS);
   // Depth census of the nodes above the groups: their depths (walks from the
   // group boundaries, and from the group roots for the largest depth) as a
-  // compact (depth, node) list, then one workgroup lists them level by level
+  // compact (depth, node) list, then two passes list them level by level
   // (depth < 128). The sort scratch of the Morton pass holds the list.
   const int cblocks = (int)std::min<int64_t>((2 * (int64_t)n / 32 + 2 + 255) / 256, 4096);
   ASTER_HIP(c, c->dmax.reserve((size_t)cblocks));
+  ASTER_HIP(c, c->censusctr.reserve(2 * kMaxLevels));
   hipLaunchKernelGGL(k_above_census, dim3(cblocks), dim3(256), 0, c->stream, n, c->gstart.p, c->gcount.p,
                      c->splitnode.p, c->groot.p, c->climb.p, c->parent.p, c->depth.p, c->codes2.p, c->idx2.p,
-                     c->dmax.p, c->flags.p);
-  hipLaunchKernelGGL(k_census_levels, dim3(1), dim3(kCensusThreads), 0, c->stream, c->gcount.p, c->codes2.p,
-                     c->idx2.p, c->dmax.p, cblocks, c->bucket.p, c->levels.p);
+                     c->dmax.p, c->censusctr.p, c->flags.p);
+  const int ctiles = (n + kRsTile - 1) / kRsTile;  // the list has fewer than n entries
+  hipLaunchKernelGGL(k_census_count, dim3(ctiles), dim3(kRsThreads), 0, c->stream, c->gcount.p, c->codes2.p,
+                     c->censusctr.p);
+  hipLaunchKernelGGL(k_census_place, dim3(ctiles), dim3(kRsThreads), 0, c->stream, c->gcount.p, c->codes2.p,
+                     c->idx2.p, c->censusctr.p, c->dmax.p, cblocks, c->bucket.p, c->levels.p);
   if (full)
     hipLaunchKernelGGL(k_depth, dim3((nn + 255) / 256), dim3(256), 0, c->stream, n, c->left.p, c->right.p,
                        c->parent.p, c->depth.p, c->escape.p, c->flags.p);
