@@ -158,20 +158,34 @@ __global__ __launch_bounds__(256) void k_box(int n, const double* __restrict__ x
   }
 }
 
-__global__ __launch_bounds__(64) void k_box_finish(const double* __restrict__ part, int nparts,
-                                                   const double* __restrict__ user, int use_user,
-                                                   double* __restrict__ box) {
+// One thread per partial (one round of loads), then a block maximum; fmax is
+// exact and order-independent.
+constexpr int kBoxFinishThreads = 1024;
+__global__ __launch_bounds__(kBoxFinishThreads) void k_box_finish(const double* __restrict__ part, int nparts,
+                                                                  const double* __restrict__ user, int use_user,
+                                                                  double* __restrict__ box) {
 #pragma clang fp contract(off)
+  __shared__ double wm[3][kBoxFinishThreads / 64];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  double r[3] = {0.0, 0.0, 0.0};
+  if (!use_user)
+    for (int i = t; i < nparts; i += kBoxFinishThreads)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) r[c] = fmax(r[c], part[3 * i + c]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    for (int off = 32; off > 0; off >>= 1) r[c] = fmax(r[c], __shfl_xor(r[c], off, 64));
+    if (lane == 0) wm[c][w] = r[c];
+  }
+  __syncthreads();
+  if (t != 0) return;
   double b[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    double r = 0.0;
-    if (!use_user)
-      for (int i = threadIdx.x; i < nparts; i += 64) r = fmax(r, part[3 * i + c]);
-    for (int off = 32; off > 0; off >>= 1) r = fmax(r, __shfl_xor(r, off, 64));
-    b[c] = use_user ? user[c] : r;
+    double m = 0.0;
+    for (int q = 0; q < kBoxFinishThreads / 64; ++q) m = fmax(m, wm[c][q]);
+    b[c] = use_user ? user[c] : m;
   }
-  if (threadIdx.x != 0) return;
   box[0] = b[0];
   box[1] = b[1];
   box[2] = b[2];
@@ -253,7 +267,7 @@ int launch_box(aster_ctx* c, bool user_box) {
     hipLaunchKernelGGL(k_box, dim3(blocks), dim3(256), 0, c->stream, n, c->x.p, c->boxpart.p);
     ASTER_HIP(c, hipGetLastError());
   }
-  hipLaunchKernelGGL(k_box_finish, dim3(1), dim3(64), 0, c->stream, c->boxpart.p, blocks, c->boxd.p + 4,
+  hipLaunchKernelGGL(k_box_finish, dim3(1), dim3(kBoxFinishThreads), 0, c->stream, c->boxpart.p, blocks, c->boxd.p + 4,
                      user ? 1 : 0, c->boxd.p);
   ASTER_HIP(c, hipGetLastError());
   return ASTER_OK;
