@@ -119,6 +119,30 @@ def bench_bh(A, ctx, steps, warmup, method, n=1 << 20, theta=0.5, dt=5e-4, refer
         errs["reference_mac"] = err_stats(rel_err(acc, ref))
         errs["reference_mac"]["interactions_per_body"] = inter_ref / n
         log(f"[bench] reference MAC at theta={theta}: {errs['reference_mac']}")
+        # the workload's other opening angles: accuracy against the same
+        # reference, then a short timed run from the same initial state
+        sweep = {}
+        for th in (0.3, 0.7):
+            params(A, ctx, G=1.0, softening=0.005 ** 2, theta=th, precision=A.FP32)
+            ctx.bh_stats()
+            e = err_stats(rel_err(ctx.forces(method), ref))
+            inter_th, _ = ctx.bh_stats()
+            ctx.set_state(pos, vel, mass)
+            ctx.step(method, A.LEAPFROG, dt, warmup)
+            ctx.sync()
+            ctx.bh_stats()
+            t0 = time.perf_counter()
+            ctx.step(method, A.LEAPFROG, dt, steps)
+            ctx.sync()
+            wall_th = time.perf_counter() - t0
+            inter_steps, _ = ctx.bh_stats()
+            e.update(interactions_per_body=inter_th / n, ms_per_step=wall_th / steps * 1e3,
+                     interactions_per_s=inter_steps / wall_th)
+            sweep[str(th)] = e
+            log(f"[bench] theta={th}: {e}")
+        errs["theta_sweep"] = sweep
+        params(A, ctx, G=1.0, softening=0.005 ** 2, theta=theta, precision=A.FP32)
+        ctx.set_state(pos, vel, mass)
     # warmup, then K timed leapfrog steps (no per-phase events inside the
     # timed region: they cost ~2.5% of a step)
     ctx.step(method, A.LEAPFROG, dt, warmup)
