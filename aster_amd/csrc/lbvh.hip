@@ -389,7 +389,7 @@ __global__ __launch_bounds__(256) void k_above_census(int n, const int32_t* __re
                                                       int32_t* __restrict__ dmax, uint32_t* __restrict__ gcnt,
                                                       uint32_t* __restrict__ flags) {
   __shared__ int wmax[4];
-  if (blockIdx.x == 0) gcnt[threadIdx.x] = 0u;  // 256 census counters and cursors, for the passes after
+  if (blockIdx.x == 0) gcnt[threadIdx.x] = 0u;  // the 2 kMaxLevels = 256 census counters and cursors
   const int ng = (int)*gcount;
   int mx = 0;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < 2 * ng - 1; t += gridDim.x * blockDim.x) {
@@ -424,6 +424,7 @@ __global__ __launch_bounds__(256) void k_above_census(int n, const int32_t* __re
 // arbitrary: each node's moments depend on its children only. Lanes of a
 // wavefront with equal depths (few distinct values, many nodes each) are
 // counted with one LDS atomic per distinct depth.
+static_assert(kMaxLevels == 128, "depth_match compares 7 bits; the census keeps 2 x 128 counters");
 __device__ __forceinline__ uint64_t depth_match(bool valid, uint32_t d, uint64_t lt, uint32_t& rank) {
   uint64_t m = __ballot(valid);
 #pragma unroll
