@@ -167,6 +167,28 @@ def test_bh_group_matches_oracle_lists(ctx, orc, n, seed):
         assert np.median(e) < 1e-6 and e.max() < 1e-4, (n, theta, np.median(e), e.max())
 
 
+def test_bh_group_matches_oracle_user_box_and_wide_angles(ctx, orc):
+    """The same exact-list check with a user Morton box (a different tree)
+    and opening angles of 1 and more."""
+    n = 30000
+    pos, vel, mass = mixed(n, seed=31)
+    box = np.array([11.0, 10.5, 12.0])
+    for theta in (1.0, 1.5):
+        load(ctx, pos, vel, mass, theta=theta, box=tuple(box))
+        try:
+            ctx.bh_stats()
+        except A.AsterError:
+            pass
+        acc = ctx.forces(A.BH_GROUP)
+        inter, visits = ctx.bh_stats()
+        ref, ref_inter, ref_visits = orc.bh_group(pos, mass, box, theta, soft=2.5e-5)
+        assert (inter, visits) == (ref_inter, ref_visits), theta
+        scale = np.linalg.norm(ref, axis=1)
+        e = np.linalg.norm(acc - ref, axis=1) / np.maximum(scale, np.median(scale))
+        assert np.median(e) < 1e-6 and e.max() < 1e-4, (theta, np.median(e), e.max())
+    ctx.set_params(box=(0.0, 0.0, 0.0))  # back to the data's box for the tests that share the context
+
+
 def force_errors(ctx, pos, vel, mass, method, thetas):
     load(ctx, pos, vel, mass)
     ref = ctx.forces(A.ALLPAIRS_FP64)
