@@ -186,7 +186,6 @@ def test_bh_group_matches_oracle_user_box_and_wide_angles(ctx, orc):
         scale = np.linalg.norm(ref, axis=1)
         e = np.linalg.norm(acc - ref, axis=1) / np.maximum(scale, np.median(scale))
         assert np.median(e) < 1e-6 and e.max() < 1e-4, (theta, np.median(e), e.max())
-    ctx.set_params(box=(0.0, 0.0, 0.0))  # back to the data's box for the tests that share the context
 
 
 def force_errors(ctx, pos, vel, mass, method, thetas):
