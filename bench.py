@@ -493,8 +493,8 @@ def main():
                     help="run one section only (profiling); no contract line")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, "
-              "'More than one GPU'); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path; run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
     import aster_amd as A
     ctx = A.Context(0)

@@ -17,5 +17,5 @@ def test_cpp_solver_shims():
     assert r.returncode == 0 and "cpp shim: ok" in r.stdout
 
 
-def test_cpp_driver_is_built():
+def test_cpp_program_is_built():
     assert os.path.exists(BIN)
