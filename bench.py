@@ -4,8 +4,9 @@
 Headline (the `value`): LBVH Barnes-Hut at N = 2^20 bodies (half Plummer
 sphere, half uniform cube [-4, 4]^3, FP32 positions/masses, eps = 0.005,
 theta = 0.5). One step = one leapfrog step: box + Morton codes + radix sort +
-Karras build + bottom-up moments + links + traversal + kick/drift, with inputs
-resident in HBM. value = accepted body-node interactions per second.
+Karras build + bottom-up moments (which also write the traversal's child
+records) + traversal + kick/drift, with inputs resident in HBM. value =
+accepted body-node interactions per second.
 
 Beside it, in the same JSON line: all-pairs FP32 (Kahan) at N = 65536 with its
 own roofline, per-phase HIP-event timings, force errors against FP64
@@ -41,7 +42,6 @@ HBM_PEAK_GBPS = 8000.0    # MI355X published HBM3E bandwidth
 #  kickdrift: FP64 x, v, a read (72 B), x, v written (48 B).
 PHASE_BYTES_PER_BODY = {"morton": 24 + 24 + 8 + 4 + 4 + 32, "sort": 4 * (4 + 8 + 8), "build": 4 + 4 + 32 + 48 + 24 + 12,
                         "kickdrift": 72 + 48}
-HBM_PEAK_GBS = 8000.0
 
 
 def traffic_per_launch(kernel):
