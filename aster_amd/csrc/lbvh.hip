@@ -124,10 +124,13 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// Thread t owns entries [base + 16 t, base + 16 t + 16) of its chunk.
+// Thread t owns entries [base + 16 t, base + 16 t + 16) of its chunk. With
+// `emit` (0/1 flags in data) it also compacts: emit[prefix] = position of
+// every set flag, and emit[total] = count.
 __global__ __launch_bounds__(kScanThreads) void k_scan_apply(uint32_t* __restrict__ data, int count,
                                                              const uint32_t* __restrict__ part,
-                                                             uint32_t* __restrict__ total) {
+                                                             uint32_t* __restrict__ total,
+                                                             int32_t* __restrict__ emit) {
   __shared__ uint32_t ws[4];
   __shared__ uint32_t wsum[4];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -154,10 +157,16 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(uint32_t* __restric
   for (int q = 0; q < w; ++q) run += wsum[q];
 #pragma unroll
   for (int r = 0; r < kScanPer; ++r) {
-    if (base + r < count) data[base + r] = run;
+    if (base + r < count) {
+      data[base + r] = run;
+      if (emit && v[r]) emit[run] = base + r;
+    }
     run += v[r];
   }
-  if (total && base < count && base + kScanPer >= count) *total = run;
+  if (total && base < count && base + kScanPer >= count) {
+    *total = run;
+    if (emit) emit[run] = count;
+  }
 }
 
 // Stable scatter. Each wavefront owns a contiguous quarter of the tile (512
@@ -1104,17 +1113,6 @@ __global__ __launch_bounds__(256) void k_group_flags(int n, const int32_t* __res
   if (lo == k) groot[k] = v;
 }
 
-// From the exclusive scan of the flags: gstart[g] = first position of group g,
-// gstart[ngroups] = n.
-__global__ __launch_bounds__(256) void k_group_emit(int n, const uint32_t* __restrict__ ex,
-                                                    const uint32_t* __restrict__ total, int32_t* __restrict__ gstart) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const uint32_t e = ex[k];
-  const uint32_t next = (k + 1 < n) ? ex[k + 1] : *total;
-  if (next != e) gstart[e] = k;
-  if (k == n - 1) gstart[*total] = n;
-}
 
 // Wavefront-cooperative traversal, one group per wavefront (lanes = the
 // group's bodies, at most 64). The wavefront walks the tree once for the
@@ -1410,13 +1408,13 @@ static int reserve_sort(aster_ctx* c) {
 
 // In-place exclusive scan of data[0, count); the total goes to *total when
 // given (a device pointer).
-static int device_scan(aster_ctx* c, uint32_t* data, int count, uint32_t* total) {
+static int device_scan(aster_ctx* c, uint32_t* data, int count, uint32_t* total, int32_t* emit = nullptr) {
   const int chunks = (count + kScanChunk - 1) / kScanChunk;
   if (chunks == 0) return ASTER_OK;
   ASTER_HIP(c, c->scanpart.reserve((size_t)chunks));
   hipLaunchKernelGGL(k_scan_reduce, dim3(chunks), dim3(kScanThreads), 0, c->stream, data, count, c->scanpart.p);
   hipLaunchKernelGGL(k_scan_apply, dim3(chunks), dim3(kScanThreads), 0, c->stream, data, count, c->scanpart.p,
-                     total);
+                     total, emit);
   ASTER_HIP(c, hipGetLastError());
   return ASTER_OK;
 }
@@ -1506,10 +1504,9 @@ static int build_groups(aster_ctx* c) {
   ASTER_HIP(c, c->gcount.reserve(1));
   hipLaunchKernelGGL(k_group_flags, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->parent.p, c->grange.p,
                      c->gscan.p, c->climb.p, c->groot.p);
-  int rc = device_scan(c, c->gscan.p, n, c->gcount.p);
+  // the scan of the start flags also writes the group starts and the sentinel n
+  int rc = device_scan(c, c->gscan.p, n, c->gcount.p, c->gstart.p);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_group_emit, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->gscan.p, c->gcount.p,
-                     c->gstart.p);
   ASTER_HIP(c, hipGetLastError());
   return ASTER_OK;
 }
