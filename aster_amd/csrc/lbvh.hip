@@ -537,14 +537,8 @@ __global__ __launch_bounds__(kRsThreads) void k_census_place(const uint32_t* __r
   }
 }
 
-// Band of up to H levels above the groups, one thread per node at the band's
-// top depth. The thread gathers the band below its node level by level (all
-// child ids of one level in one round of loads, with the range sizes that
-// tell which children lie above the groups), then the records of every child
-// that is already complete (group roots, leaves, nodes of the band below) in
-// one round, and forms the band's nodes bottom-up in registers: H levels cost
-// H + 2 dependent load rounds and one launch. Bottom-up order is what unite
-// (barnes_hut.cpp:137-152) needs: kernel boundaries order the bands.
+// A node's moments: FP64 centre of mass and mass (and centre-of-mass velocity),
+// FP32 bounding box.
 struct MomRec {
   double4 com, vel;
   float4 lo, hi;
@@ -627,6 +621,32 @@ __device__ __forceinline__ MomRec unite_moments(const MomRec& cl, const MomRec& 
   return me;
 }
 
+template <bool VEL>
+__device__ __forceinline__ MomRec shfl_xor_rec(const MomRec& r, int m) {
+  MomRec o;
+  o.com = make_double4(__shfl_xor(r.com.x, m, 64), __shfl_xor(r.com.y, m, 64), __shfl_xor(r.com.z, m, 64),
+                       __shfl_xor(r.com.w, m, 64));
+  o.vel = VEL ? make_double4(__shfl_xor(r.vel.x, m, 64), __shfl_xor(r.vel.y, m, 64), __shfl_xor(r.vel.z, m, 64), 0.0)
+              : make_double4(0.0, 0.0, 0.0, 0.0);
+  o.lo = make_float4(__shfl_xor(r.lo.x, m, 64), __shfl_xor(r.lo.y, m, 64), __shfl_xor(r.lo.z, m, 64), 0.f);
+  o.hi = make_float4(__shfl_xor(r.hi.x, m, 64), __shfl_xor(r.hi.y, m, 64), __shfl_xor(r.hi.z, m, 64), 0.f);
+  return o;
+}
+
+// Band of up to H <= 6 levels above the groups, formed by a team of 2^H lanes
+// per node at the band's top depth. Lane s of a team stands for slot s of the
+// band's bottom level: a node at band level l covers the slots
+// [k 2^(H-l), (k+1) 2^(H-l)) and is handled by the first of them, its leader.
+// Going down, the leaders of the nodes formed here load their children and
+// leaf ranges (one round of loads per level; the ranges tell which children
+// lie above the groups) and the other lanes of the node take theirs by
+// shuffle. Then the leader of every child that is already complete (group
+// roots, leaves, nodes of the band below) loads its record, all in one round.
+// Going up, a leader unites its own record (its node's left child) with the
+// one held half its span away (the right child). H levels cost H + 1
+// dependent load rounds and one launch; a team is at most one wavefront, so
+// no barrier is needed. Bottom-up order is what unite (barnes_hut.cpp:137-152)
+// needs: kernel boundaries order the bands.
 template <bool VEL, int H>
 __global__ __launch_bounds__(256) void k_moments_band(int n, const int32_t* __restrict__ bucket, int count,
                                                       const int32_t* __restrict__ left,
@@ -635,71 +655,66 @@ __global__ __launch_bounds__(256) void k_moments_band(int n, const int32_t* __re
                                                       double4* __restrict__ nvel, float4* __restrict__ nbox,
                                                       float4* __restrict__ nrec, double theta,
                                                       float4* __restrict__ kids) {
-  constexpr int W = 1 << H;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= count) return;
-  // id[l][k]: node at band level l (children of k at 2k, 2k + 1 of level
-  // l + 1), -1 when absent; in[l][k]: formed here (above the groups)
-  int id[H + 1][W];
-  bool in[H + 1][W];
-#pragma unroll
-  for (int l = 0; l <= H; ++l)
-#pragma unroll
-    for (int k = 0; k < W; ++k) {
-      id[l][k] = -1;
-      in[l][k] = false;
-    }
-  id[0][0] = bucket[t];
-  in[0][0] = true;
-  // one round of loads per level: a node's children and its own leaf range,
-  // which gives the children's ranges (the left child covers [lo, gamma], the
-  // right one [gamma + 1, hi], gamma being the split)
+  static_assert(H >= 1 && H <= 6, "a band's team is at most one wavefront");
+  constexpr int S = 1 << H;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int team = g >> H;
+  if (team >= count) return;  // teams are aligned to S lanes and leave whole
+  const int lane = threadIdx.x & 63;
+  const int s = g & (S - 1);
+  int ids[H];            // node covering this slot at band level l
+  uint32_t inmask = 0;   // bit l: that node is formed here
+  int cur = bucket[team];
+  bool cin = true;
+  int clev = 0;  // band level at which the complete node covering this slot appears
 #pragma unroll
   for (int l = 0; l < H; ++l) {
-#pragma unroll
-    for (int k = 0; k < (1 << l); ++k) {
-      if (in[l][k]) {
-        const int node = id[l][k];
-        const int L = left[node], R = right[node];
-        const int2 r = grange[node - n];
-        id[l + 1][2 * k] = L;
-        id[l + 1][2 * k + 1] = R;
-        if (l + 1 < H) {
-          const int gamma = L < n ? L : L - n;
-          in[l + 1][2 * k] = L >= n && gamma - r.x + 1 > kGroup;
-          in[l + 1][2 * k + 1] = R >= n && r.y - gamma > kGroup;
-        }
-      }
+    const int span = S >> l;
+    ids[l] = cur;
+    if (cin) inmask |= 1u << l;
+    int L = 0, R = 0, fl = 0;
+    if (cin && (s & (span - 1)) == 0) {
+      L = left[cur];
+      R = right[cur];
+      const int2 r = grange[cur - n];  // children cover [r.x, gamma] and [gamma + 1, r.y]
+      const int gamma = L < n ? L : L - n;
+      if (l + 1 < H)
+        fl = (L >= n && gamma - r.x + 1 > kGroup ? 1 : 0) | (R >= n && r.y - gamma > kGroup ? 2 : 0);
+    }
+    const int src = lane & ~(span - 1);
+    L = __shfl(L, src, 64);
+    R = __shfl(R, src, 64);
+    fl = __shfl(fl, src, 64);
+    if (cin) {
+      const bool rt = (s & (span >> 1)) != 0;
+      cur = rt ? R : L;
+      cin = ((rt ? fl >> 1 : fl) & 1) != 0;
+      clev = l + 1;
     }
   }
-  // records of the complete children, all levels at once
-  MomRec rec[H + 1][W];
-#pragma unroll
-  for (int l = 1; l <= H; ++l)
-#pragma unroll
-    for (int k = 0; k < (1 << l); ++k)
-      if (id[l][k] >= 0 && !in[l][k]) rec[l][k] = load_moments<VEL>(n, id[l][k], nmc, nvel, nbox);
-  // bottom-up
+  MomRec rec{};
+  if ((s & ((S >> clev) - 1)) == 0) rec = load_moments<VEL>(n, cur, nmc, nvel, nbox);
+  int cid = cur;  // node whose record this lane holds
 #pragma unroll
   for (int l = H - 1; l >= 0; --l) {
-#pragma unroll
-    for (int k = 0; k < (1 << l); ++k) {
-      if (!in[l][k]) continue;
-      const int node = id[l][k];
-      const MomRec me = unite_moments<VEL>(rec[l + 1][2 * k], rec[l + 1][2 * k + 1]);
+    const int half = S >> (l + 1);
+    const MomRec o = shfl_xor_rec<VEL>(rec, half);
+    const int oid = __shfl_xor(cid, half, 64);
+    if (((inmask >> l) & 1u) && (s & (2 * half - 1)) == 0) {
+      const int node = ids[l];
+      const MomRec me = unite_moments<VEL>(rec, o);
       nmc[node] = me.com;
       if (VEL) nvel[node] = me.vel;
       nbox[2 * node] = me.lo;
       nbox[2 * node + 1] = me.hi;
       if (!kids) nrec[node] = node_rec(me.com);
       if (kids) {  // this node's child slots (and the root's own slot)
-        const MomRec& a = rec[l + 1][2 * k];
-        const MomRec& b = rec[l + 1][2 * k + 1];
-        put_kid(kids, kid_slot(n, node, 0), id[l + 1][2 * k], a.com, a.lo, a.hi, theta);
-        put_kid(kids, kid_slot(n, node, 1), id[l + 1][2 * k + 1], b.com, b.lo, b.hi, theta);
+        put_kid(kids, kid_slot(n, node, 0), cid, rec.com, rec.lo, rec.hi, theta);
+        put_kid(kids, kid_slot(n, node, 1), oid, o.com, o.lo, o.hi, theta);
         if (node == n) put_kid(kids, 2 * (n - 1), n, me.com, me.lo, me.hi, theta);
       }
-      rec[l][k] = me;
+      rec = me;
+      cid = node;
     }
   }
 }
@@ -1511,24 +1526,32 @@ static int build_groups(aster_ctx* c) {
   return ASTER_OK;
 }
 
-constexpr int kBand = 3;  // levels per moments launch above the groups
+constexpr int kBand = 6;  // levels per moments launch above the groups (a team of 2^6 lanes)
 
 static int launch_moments_band(aster_ctx* c, bool vel, int h, int off, int cnt, float4* kids) {
   const int n = (int)c->n;
-  const dim3 grid((cnt + 255) / 256), block(256);
+  if (h < 1 || h > kBand) return fail(c, ASTER_ERR_STATE, "moments band height out of range");
+  const int64_t threads = (int64_t)cnt << h;
+  if (threads > INT32_MAX) return fail(c, ASTER_ERR_STATE, "moments band too wide");
+  const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
   const int32_t* b = c->bucket.p + off;
 #define ASTER_BAND(V, H)                                                                                          \
   hipLaunchKernelGGL((k_moments_band<V, H>), grid, block, 0, c->stream, n, b, cnt, c->left.p, c->right.p,        \
                      c->grange.p, c->nmc.p, c->nvel.p, c->nbox.p, c->nrec.p, c->P.theta, kids)
-  switch (h * 2 + (vel ? 1 : 0)) {
-    case 2: ASTER_BAND(false, 1); break;
-    case 3: ASTER_BAND(true, 1); break;
-    case 4: ASTER_BAND(false, 2); break;
-    case 5: ASTER_BAND(true, 2); break;
-    case 6: ASTER_BAND(false, 3); break;
-    case 7: ASTER_BAND(true, 3); break;
-    default: return fail(c, ASTER_ERR_STATE, "moments band height out of range");
+#define ASTER_BAND_H(H)                 \
+  case H:                               \
+    if (vel) ASTER_BAND(true, H);       \
+    else ASTER_BAND(false, H);          \
+    break;
+  switch (h) {
+    ASTER_BAND_H(1)
+    ASTER_BAND_H(2)
+    ASTER_BAND_H(3)
+    ASTER_BAND_H(4)
+    ASTER_BAND_H(5)
+    ASTER_BAND_H(6)
   }
+#undef ASTER_BAND_H
 #undef ASTER_BAND
   ASTER_HIP(c, hipGetLastError());
   return ASTER_OK;
