@@ -14,7 +14,8 @@
 // library's message instead of exit(-1) (Check(), basic.h:25-29).
 //
 // Inside Aster, HipAllPairs/HipBarnesHut derive from Aster::Solver instead of
-// aster_hip::Solver and read Simulation::bodies; see INTEGRATION.md.
+// aster_hip::Solver and read Simulation::bodies in place of the BodyArray
+// struct below; the bodies of load() and compute_forces() stay as written.
 #pragma once
 
 #include <stdexcept>
