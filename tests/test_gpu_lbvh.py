@@ -105,6 +105,19 @@ def test_bh_fp64_parity_pn2_pn25_bit_identical(ctx, orc):
         ctx.forces(A.BH_GROUP)  # post-Newtonian terms need the FP64 parity traversal
 
 
+def test_bh_fp64_parity_node_velocities_in_moment_bands(ctx, orc):
+    """At 2^18 bodies most nodes above the traversal groups are formed by the
+    multi-level moment bands (not the top-levels workgroup), velocities
+    included; their mass-weighted node velocities feed the 2PN and 2.5PN
+    terms, so the forces are the same bits only if those moments are."""
+    flags = A.NEWTON | A.PN1 | A.PN2 | A.PN25
+    pos, vel, mass = mixed(1 << 18, seed=6)
+    load(ctx, pos, vel, mass, theta=0.4, precision=A.FP64, c=30.0, pn_flags=flags)
+    acc = ctx.forces(A.BH_PARITY)
+    ref = orc.bh(pos, vel, mass, orc.box_auto(pos), 0.4, c=30.0, soft=2.5e-5, pn=flags)
+    assert np.array_equal(acc, ref)
+
+
 def test_bh_fp64_user_box_bit_identical(ctx, orc):
     pos, vel, mass = mixed(3000, seed=5)
     box = np.array([10.5, 10.25, 11.0])
