@@ -643,8 +643,9 @@ __device__ __forceinline__ MomRec shfl_xor_rec(const MomRec& r, int m) {
 // shuffle. Then the leader of every child that is already complete (group
 // roots, leaves, nodes of the band below) loads its record, all in one round.
 // Going up, a leader unites its own record (its node's left child) with the
-// one held half its span away (the right child). H levels cost H + 1
-// dependent load rounds and one launch; a team is at most one wavefront, so
+// one held half its span away (the right child). H levels cost H + 2
+// dependent load rounds (the band top, H levels of children, the records)
+// and one launch; a team is at most one wavefront, so
 // no barrier is needed. Bottom-up order is what unite (barnes_hut.cpp:137-152)
 // needs: kernel boundaries order the bands.
 template <bool VEL, int H>
