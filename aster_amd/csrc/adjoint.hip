@@ -40,14 +40,15 @@ __device__ __forceinline__ float unpack_mass_adj(double w) {
 }
 
 // Sources for the fused kernel: positions (FP64, FP32 mass in the low word of
-// w) and {w, m} in FP32, padded with zero-mass bodies.
-__global__ __launch_bounds__(256) void k_adj_src(int n, int npad, const double* __restrict__ x,
+// w) and {w, m} in FP32, padded with zero-mass bodies. Masses are scaled by
+// msc (Fp32Scales), so a and J^T w come out scaled by 2^-k.
+__global__ __launch_bounds__(256) void k_adj_src(int n, int npad, float msc, const double* __restrict__ x,
                                                  const double* __restrict__ m, const double* __restrict__ w,
                                                  double4* __restrict__ src, float4* __restrict__ wsrc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npad) return;
   if (i < n) {
-    const float mf = (float)m[i];
+    const float mf = (float)m[i] * msc;
     src[i] = make_double4(x[i], x[n + i], x[2 * n + i],
                           __longlong_as_double((long long)(unsigned long long)__float_as_uint(mf)));
     wsrc[i] = make_float4((float)w[i], (float)w[n + i], (float)w[2 * n + i], mf);
@@ -55,6 +56,54 @@ __global__ __launch_bounds__(256) void k_adj_src(int n, int npad, const double* 
     src[i] = make_double4(0.0, 0.0, 0.0, __longlong_as_double(0));
     wsrc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
+}
+
+// One 64-source tile of the fused kernel. SELF: the tile holds the targets of
+// this wavefront (source k is lane k's own body), whose pair is skipped: its
+// d = 0, but u = m_k w_k - m_k w_k comes out of the FMA as the rounding error
+// of m_k w_k, which eps^-3 would amplify.
+template <bool MASK, bool SELF>
+__device__ __forceinline__ void adj_tile(const double4* __restrict__ src, const float4* __restrict__ wsrc, int t,
+                                         double px, double py, double pz, const float4& wk, float mk, float eps2,
+                                         int lane, float* __restrict__ S, float* __restrict__ C) {
+  float ax = 0.f, ay = 0.f, az = 0.f, gx = 0.f, gy = 0.f, gz = 0.f, gm = 0.f;
+#pragma unroll 8
+  for (int k = 0; k < kApTile; ++k) {
+    const double4 q = src[t + k];
+    const float4 p = wsrc[t + k];
+    const float dx = (float)(q.x - px), dy = (float)(q.y - py), dz = (float)(q.z - pz);
+    float r2 = fmaf(dx, dx, eps2);
+    r2 = fmaf(dy, dy, r2);
+    r2 = fmaf(dz, dz, r2);
+    float inv = __builtin_amdgcn_rsqf(r2);
+    if (MASK) inv = (r2 > 0.f) ? inv : 0.f;
+    if (SELF) inv = (k == lane) ? 0.f : inv;
+    const float inv2 = inv * inv;
+    const float inv3 = inv2 * inv;
+    const float inv5 = inv3 * inv2;
+    const float mj = unpack_mass_adj(q.w);
+    const float f = mj * inv3;
+    ax = fmaf(f, dx, ax);
+    ay = fmaf(f, dy, ay);
+    az = fmaf(f, dz, az);
+    const float ux = fmaf(mk, p.x, -(mj * wk.x));
+    const float uy = fmaf(mk, p.y, -(mj * wk.y));
+    const float uz = fmaf(mk, p.z, -(mj * wk.z));
+    const float du = fmaf(dx, ux, fmaf(dy, uy, dz * uz));
+    const float c3 = (du * inv5) * 3.f;
+    gx = fmaf(-c3, dx, fmaf(ux, inv3, gx));
+    gy = fmaf(-c3, dy, fmaf(uy, inv3, gy));
+    gz = fmaf(-c3, dz, fmaf(uz, inv3, gz));
+    const float dw = fmaf(dx, p.x, fmaf(dy, p.y, dz * p.z));
+    gm = fmaf(-dw, inv3, gm);
+  }
+  kahan_add32(S[0], C[0], ax);
+  kahan_add32(S[1], C[1], ay);
+  kahan_add32(S[2], C[2], az);
+  kahan_add32(S[3], C[3], gx);
+  kahan_add32(S[4], C[4], gy);
+  kahan_add32(S[5], C[5], gz);
+  kahan_add32(S[6], C[6], gm);
 }
 
 template <bool MASK>
@@ -69,46 +118,17 @@ __global__ __launch_bounds__(256) void k_adj_fused(const double4* __restrict__ s
   const double px = x[ii], py = x[n + ii], pz = x[2 * n + ii];
   const float4 wk = wsrc[ii];
   const float mk = wk.w;
+  const int lane = threadIdx.x & 63;
+  // the wavefront's targets i0 .. i0 + 63 are sources of tile i0 (tiles are
+  // 64-aligned, as are the slices)
+  const int self_tile = __builtin_amdgcn_readfirstlane(i - lane);
   float S[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   float C[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   for (int t = jb; t < je; t += kApTile) {
-    float ax = 0.f, ay = 0.f, az = 0.f, gx = 0.f, gy = 0.f, gz = 0.f, gm = 0.f;
-#pragma unroll 8
-    for (int k = 0; k < kApTile; ++k) {
-      const double4 q = src[t + k];
-      const float4 p = wsrc[t + k];
-      const float dx = (float)(q.x - px), dy = (float)(q.y - py), dz = (float)(q.z - pz);
-      float r2 = fmaf(dx, dx, eps2);
-      r2 = fmaf(dy, dy, r2);
-      r2 = fmaf(dz, dz, r2);
-      float inv = __builtin_amdgcn_rsqf(r2);
-      if (MASK) inv = (r2 > 0.f) ? inv : 0.f;
-      const float inv2 = inv * inv;
-      const float inv3 = inv2 * inv;
-      const float inv5 = inv3 * inv2;
-      const float mj = unpack_mass_adj(q.w);
-      const float f = mj * inv3;
-      ax = fmaf(f, dx, ax);
-      ay = fmaf(f, dy, ay);
-      az = fmaf(f, dz, az);
-      const float ux = fmaf(mk, p.x, -(mj * wk.x));
-      const float uy = fmaf(mk, p.y, -(mj * wk.y));
-      const float uz = fmaf(mk, p.z, -(mj * wk.z));
-      const float du = fmaf(dx, ux, fmaf(dy, uy, dz * uz));
-      const float c3 = (du * inv5) * 3.f;
-      gx = fmaf(-c3, dx, fmaf(ux, inv3, gx));
-      gy = fmaf(-c3, dy, fmaf(uy, inv3, gy));
-      gz = fmaf(-c3, dz, fmaf(uz, inv3, gz));
-      const float dw = fmaf(dx, p.x, fmaf(dy, p.y, dz * p.z));
-      gm = fmaf(-dw, inv3, gm);
-    }
-    kahan_add32(S[0], C[0], ax);
-    kahan_add32(S[1], C[1], ay);
-    kahan_add32(S[2], C[2], az);
-    kahan_add32(S[3], C[3], gx);
-    kahan_add32(S[4], C[4], gy);
-    kahan_add32(S[5], C[5], gz);
-    kahan_add32(S[6], C[6], gm);
+    if (t == self_tile)
+      adj_tile<MASK, true>(src, wsrc, t, px, py, pz, wk, mk, eps2, lane, S, C);
+    else
+      adj_tile<MASK, false>(src, wsrc, t, px, py, pz, wk, mk, eps2, lane, S, C);
   }
   if (i < n) {
     double* o = out + (size_t)sl * 7 * n;
@@ -117,15 +137,16 @@ __global__ __launch_bounds__(256) void k_adj_fused(const double4* __restrict__ s
   }
 }
 
-// Sum slice partials in slice order and scale by G: a (3 planes), gx (3), gm.
-__global__ void k_adj_combine(int n, int S, const double* __restrict__ part, double G, double* __restrict__ a,
-                              double* __restrict__ gx, double* __restrict__ gm) {
+// Sum slice partials in slice order and scale: a (3 planes) and gx (3), which
+// the scaled masses enter once, by G 2^k (gsc); gm, free of masses, by G.
+__global__ void k_adj_combine(int n, int S, const double* __restrict__ part, double gsc, double G,
+                              double* __restrict__ a, double* __restrict__ gx, double* __restrict__ gm) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   for (int q = 0; q < 7; ++q) {
     double s = 0.0;
     for (int k = 0; k < S; ++k) s += part[((size_t)k * 7 + q) * n + i];
-    s *= G;
+    s *= q < 6 ? gsc : G;
     if (q < 3) a[(size_t)q * n + i] = s;
     else if (q < 6) gx[(size_t)(q - 3) * n + i] = s;
     else gm[i] = s;
@@ -282,8 +303,11 @@ int traj_backward(aster_ctx* c, const double* gsnap, double* gx, double* gv, dou
   int64_t slice = round_up((npad + Sl - 1) / Sl, kApTile);
   Sl = (int)((npad + slice - 1) / slice);
   ASTER_HIP(c, S.part.reserve((size_t)Sl * 7 * n));
-  const float eps2 = (float)c->P.softening;
-  const bool mask = !(c->P.softening > 0.0);
+  Fp32Scales sc;
+  int rc0 = fp32_scales(c, &sc);
+  if (rc0) return rc0;
+  const float eps2 = sc.eps2;
+  const bool mask = sc.mask;
   float4* wsrc = S.wsrc.p;
   double* gxp = S.g.p;
   double* gmp = S.g.p + n3;
@@ -297,8 +321,8 @@ int traj_backward(aster_ctx* c, const double* gsnap, double* gx, double* gv, dou
                        anchor ? S.ckpt_x.p + co : (const double*)nullptr);
     phase_end(c, ASTER_PHASE_KICKDRIFT);
     phase_begin(c, ASTER_PHASE_ADJOINT);
-    hipLaunchKernelGGL(k_adj_src, dim3(npad / 256), eb, 0, c->stream, n, npad, c->x.p, c->m.p, S.w.p, c->src64.p,
-                       wsrc);
+    hipLaunchKernelGGL(k_adj_src, dim3(npad / 256), eb, 0, c->stream, n, npad, sc.msc, c->x.p, c->m.p, S.w.p,
+                       c->src64.p, wsrc);
     const dim3 grid(npad / kApBlock, Sl);
     if (mask)
       hipLaunchKernelGGL((k_adj_fused<true>), grid, dim3(kApBlock), 0, c->stream, c->src64.p, wsrc, c->x.p, n, npad,
@@ -306,7 +330,7 @@ int traj_backward(aster_ctx* c, const double* gsnap, double* gx, double* gv, dou
     else
       hipLaunchKernelGGL((k_adj_fused<false>), grid, dim3(kApBlock), 0, c->stream, c->src64.p, wsrc, c->x.p, n, npad,
                          (int)slice, eps2, S.part.p);
-    hipLaunchKernelGGL(k_adj_combine, eg, eb, 0, c->stream, n, Sl, S.part.p, c->P.G, c->a.p, gxp, gmp);
+    hipLaunchKernelGGL(k_adj_combine, eg, eb, 0, c->stream, n, Sl, S.part.p, sc.gsc, c->P.G, c->a.p, gxp, gmp);
     phase_end(c, ASTER_PHASE_ADJOINT);
     const double* gs = nullptr;
     if (q >= 0 && S.snap_steps[q] == s) {

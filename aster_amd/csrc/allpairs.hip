@@ -16,6 +16,9 @@
 // src/physics/tool_chain_impl.cpp:120-137), one target per lane, sequential
 // sources, IEEE division and square root, no contraction: bit-identical to the
 // CPU path.
+#include <cfloat>
+#include <cmath>
+
 #include "common.hpp"
 #include "device.hpp"
 
@@ -27,16 +30,20 @@ enum { ACC_PLAIN = 0, ACC_TILE_KAHAN = 1, ACC_FULL_KAHAN = 2 };
 
 __device__ __forceinline__ void kahan_add(float& s, float& c, float t) { kahan_add32(s, c, t); }
 
-// Sources, padded to npad with zero-mass bodies at the origin.
+// Sources, padded to npad with zero-mass bodies at the origin. Bodies outside
+// the source range [lower, upper) (Solver::set_bounds) get zero mass, which
+// adds exactly zero force. Masses are scaled by msc (Fp32Scales).
 //   k_src32: FP32 {x, y, z, m} (positions rounded to FP32 once).
 //   k_src64: FP64 {x, y, z, m}; the kernel forms x_j - x_i in FP64 and rounds
 //            the separation to FP32, so the pair force is FP32 but a close pair
 //            does not lose ~|x|/r ulps to position rounding.
-__global__ __launch_bounds__(256) void k_src32(int n, int npad, const double* __restrict__ x,
-                                               const double* __restrict__ m, float4* __restrict__ src) {
+__global__ __launch_bounds__(256) void k_src32(int n, int npad, int lower, int upper, float msc,
+                                               const double* __restrict__ x, const double* __restrict__ m,
+                                               float4* __restrict__ src) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npad) return;
-  src[i] = i < n ? make_float4((float)x[i], (float)x[n + i], (float)x[2 * n + i], (float)m[i])
+  const bool in = i >= lower && i < upper;
+  src[i] = i < n ? make_float4((float)x[i], (float)x[n + i], (float)x[2 * n + i], in ? (float)m[i] * msc : 0.f)
                  : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
@@ -49,11 +56,13 @@ __device__ __forceinline__ float unpack_mass(double w) {
   return __uint_as_float((unsigned)(unsigned long long)__double_as_longlong(w));
 }
 
-__global__ __launch_bounds__(256) void k_src64(int n, int npad, const double* __restrict__ x,
-                                               const double* __restrict__ m, double4* __restrict__ src) {
+__global__ __launch_bounds__(256) void k_src64(int n, int npad, int lower, int upper, float msc,
+                                               const double* __restrict__ x, const double* __restrict__ m,
+                                               double4* __restrict__ src) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npad) return;
-  src[i] = i < n ? make_double4(x[i], x[n + i], x[2 * n + i], pack_mass((float)m[i]))
+  const bool in = i >= lower && i < upper;
+  src[i] = i < n ? make_double4(x[i], x[n + i], x[2 * n + i], pack_mass(in ? (float)m[i] * msc : 0.f))
                  : make_double4(0.0, 0.0, 0.0, pack_mass(0.f));
 }
 
@@ -137,7 +146,8 @@ __global__ __launch_bounds__(256) void k_allpairs_f32(const typename SrcT<SRC64>
 }
 
 // Sum S slice partials (and the optional FP64 1PN term) in slice order, then
-// scale by G: a[c][i] = G * sum_s part[s][c][i] + pn[c][i].
+// scale by G (times the mass scale's 2^k): a[c][i] = G * sum_s part[s][c][i]
+// + pn[c][i].
 __global__ void k_combine(int n, int S, const double* __restrict__ part, const double* __restrict__ pn, double G,
                           double* __restrict__ a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -308,6 +318,26 @@ int choose_split(aster_ctx* c) {
   return (int)S;
 }
 
+int fp32_scales(aster_ctx* c, Fp32Scales* sc) {
+  double M = 0.0, R = 0.0;
+  int rc = total_mass(c, &M, &R);
+  if (rc) return rc;
+  const float e2 = (float)c->P.softening;
+  sc->mask = !(e2 >= FLT_MIN);
+  sc->eps2 = sc->mask ? 0.f : e2;
+  // log2 of the smallest separation the bound assumes
+  const double lr = sc->mask ? std::log2(R > 0.0 ? R : 1.0) - 40.0 : 0.5 * std::log2((double)sc->eps2);
+  const double lb = std::log2(M) + std::max(-3.0 * lr, -2.0 * lr);
+  int k = (int)std::ceil(lb) - 120;
+  if (k < 0) k = 0;
+  if (k > 126)
+    return fail(c, ASTER_ERR_INVALID,
+                "masses and softening are beyond the FP32 force range (use an FP64 method or a larger softening)");
+  sc->msc = std::ldexp(1.0f, -k);
+  sc->gsc = std::ldexp(c->P.G, k);
+  return ASTER_OK;
+}
+
 int launch_allpairs(aster_ctx* c, int method) {
   const int n = (int)c->n;
   const int npad = (int)c->npad;
@@ -347,56 +377,66 @@ int launch_allpairs(aster_ctx* c, int method) {
   }
   if (method != ASTER_ALLPAIRS_FP32 && method != ASTER_ALLPAIRS_FP32_KAHAN)
     return fail(c, ASTER_ERR_INVALID, "unknown all-pairs method");
-  if (lower != 0 || upper != n)
-    return fail(c, ASTER_ERR_INVALID, "FP32 all-pairs supports only the full source range; use ASTER_ALLPAIRS_FP64");
+  Fp32Scales sc;
+  int rc = fp32_scales(c, &sc);
+  if (rc) return rc;
   // Padded source array: FP32 for the plain method, FP64 for the mixed one.
   const bool src64 = method == ASTER_ALLPAIRS_FP32_KAHAN;
   phase_begin(c, ASTER_PHASE_TILES);
   if (src64) {
     ASTER_HIP(c, c->src64.reserve((size_t)npad));
-    hipLaunchKernelGGL(k_src64, dim3(npad / 256), dim3(256), 0, c->stream, n, npad, c->x.p, c->m.p, c->src64.p);
+    hipLaunchKernelGGL(k_src64, dim3(npad / 256), dim3(256), 0, c->stream, n, npad, lower, upper, sc.msc, c->x.p,
+                       c->m.p, c->src64.p);
   } else {
-    hipLaunchKernelGGL(k_src32, dim3(npad / 256), dim3(256), 0, c->stream, n, npad, c->x.p, c->m.p, c->sh.p);
+    hipLaunchKernelGGL(k_src32, dim3(npad / 256), dim3(256), 0, c->stream, n, npad, lower, upper, sc.msc, c->x.p,
+                       c->m.p, c->sh.p);
   }
   ASTER_HIP(c, hipGetLastError());
   phase_end(c, ASTER_PHASE_TILES);
-  // Slices over the padded source range (padding bodies have zero mass).
+  // Slices over the tiles that hold the source range (sources outside it and
+  // the padding bodies have zero mass).
+  const int jb = lower / kApTile * kApTile;
+  const int je = std::max<int>(jb + kApTile, (int)round_up(upper, kApTile));
+  const int span = je - jb;
   int S = choose_split(c);
-  int64_t slice = round_up((npad + S - 1) / S, kApTile);
-  S = (int)((npad + slice - 1) / slice);
+  if ((int64_t)S * 1024 > span) S = std::max(1, span / 1024);
+  int64_t slice = round_up((span + S - 1) / S, kApTile);
+  S = (int)((span + slice - 1) / slice);
   ASTER_HIP(c, c->apart.reserve((size_t)S * 3 * n));
-  const float eps2 = (float)c->P.softening;
-  const bool mask = !(c->P.softening > 0.0);
+  const float eps2 = sc.eps2;
+  const bool mask = sc.mask;
   dim3 grid(npad / kApBlock, S);
   double* out = c->apart.p;
   phase_begin(c, ASTER_PHASE_ALLPAIRS);
   if (src64) {
+    const double4* sp = c->src64.p + jb;
     if (mask)
-      hipLaunchKernelGGL((k_allpairs_f32<ACC_TILE_KAHAN, true, true>), grid, dim3(kApBlock), 0, c->stream,
-                         c->src64.p, c->x.p, n, npad, (int)slice, eps2, out);
+      hipLaunchKernelGGL((k_allpairs_f32<ACC_TILE_KAHAN, true, true>), grid, dim3(kApBlock), 0, c->stream, sp,
+                         c->x.p, n, span, (int)slice, eps2, out);
     else
-      hipLaunchKernelGGL((k_allpairs_f32<ACC_TILE_KAHAN, false, true>), grid, dim3(kApBlock), 0, c->stream,
-                         c->src64.p, c->x.p, n, npad, (int)slice, eps2, out);
+      hipLaunchKernelGGL((k_allpairs_f32<ACC_TILE_KAHAN, false, true>), grid, dim3(kApBlock), 0, c->stream, sp,
+                         c->x.p, n, span, (int)slice, eps2, out);
   } else {
+    const float4* sp = c->sh.p + jb;
     if (mask)
-      hipLaunchKernelGGL((k_allpairs_f32<ACC_PLAIN, true, false>), grid, dim3(kApBlock), 0, c->stream, c->sh.p,
-                         c->x.p, n, npad, (int)slice, eps2, out);
+      hipLaunchKernelGGL((k_allpairs_f32<ACC_PLAIN, true, false>), grid, dim3(kApBlock), 0, c->stream, sp, c->x.p, n,
+                         span, (int)slice, eps2, out);
     else
-      hipLaunchKernelGGL((k_allpairs_f32<ACC_PLAIN, false, false>), grid, dim3(kApBlock), 0, c->stream, c->sh.p,
-                         c->x.p, n, npad, (int)slice, eps2, out);
+      hipLaunchKernelGGL((k_allpairs_f32<ACC_PLAIN, false, false>), grid, dim3(kApBlock), 0, c->stream, sp, c->x.p,
+                         n, span, (int)slice, eps2, out);
   }
   ASTER_HIP(c, hipGetLastError());
   phase_end(c, ASTER_PHASE_ALLPAIRS);
   const double* pnp = nullptr;
   if (pn) {
     ASTER_HIP(c, c->apn.reserve((size_t)3 * n));
-    hipLaunchKernelGGL((k_allpairs_f64<true, 1>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, 0, n, c->x.p,
-                       c->v.p, c->m.p, c->P.G, ic, pnf, c->P.softening, c->apn.p);
+    hipLaunchKernelGGL((k_allpairs_f64<true, 1>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, lower, upper,
+                       c->x.p, c->v.p, c->m.p, c->P.G, ic, pnf, c->P.softening, c->apn.p);
     ASTER_HIP(c, hipGetLastError());
     pnp = c->apn.p;
   }
   phase_begin(c, ASTER_PHASE_COMBINE);
-  hipLaunchKernelGGL(k_combine, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, S, c->apart.p, pnp, c->P.G,
+  hipLaunchKernelGGL(k_combine, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, S, c->apart.p, pnp, sc.gsc,
                      c->a.p);
   ASTER_HIP(c, hipGetLastError());
   phase_end(c, ASTER_PHASE_COMBINE);

@@ -152,6 +152,7 @@ int aster_destroy(aster_ctx* c) {
   c->gstart.release(); c->gcount.release(); c->splitnode.release(); c->groot.release(); c->climb.release();
   c->dmax.release(); c->censusctr.release();
   c->nrec.release(); c->nlink.release(); c->kids.release(); c->topslot.release(); c->bh_counters.release(); c->gsh.release();
+  c->sub_x.release(); c->sub_v.release(); c->sub_a.release(); c->sub_m.release();
   auto& Bt = c->batch;
   Bt.x.release(); Bt.v.release(); Bt.a.release(); Bt.m.release(); Bt.ebody.release(); Bt.esys.release();
   Bt.doff.release(); Bt.lane_body.release(); Bt.lane_seg.release(); Bt.wave_max.release();
@@ -189,6 +190,7 @@ int aster_set_params(aster_ctx* c, const aster_params* p) {
   }
   c->acc_valid = false;
   c->tree_valid = false;
+  batch_params_changed(c);
   return ASTER_OK;
 }
 

@@ -618,6 +618,14 @@ static void drop_graph(BatchState& B) {
   B.g_steps = -1;
 }
 
+// New parameters (G, c, softening, force law, precision): the cached
+// accelerations and the captured launches hold the old ones.
+void batch_params_changed(aster_ctx* c) {
+  if (c->batch.exec) (void)hipStreamSynchronize(c->stream);  // a replay may still run
+  drop_graph(c->batch);
+  c->batch.a_valid = false;
+}
+
 static BatchArgs make_args(aster_ctx* c, int64_t nsteps, int nsub, const double* h, int init) {
   BatchState& B = c->batch;
   BatchArgs P;
@@ -728,9 +736,10 @@ int batch_step(aster_ctx* c, int integrator, double dt, int64_t nsteps, int spl,
   if (wh) B.a_valid = false;
   if (!use_graph) return enqueue();
   const bool same = B.exec && B.g_steps == nsteps && B.g_integrator == integrator && B.g_spl == spl &&
-                    B.g_dt == dt && B.g_pn == (int)pn && B.g_exact == exact && B.g_wpb == B.waves_per_block &&
+                    B.g_dt == dt && B.g_pn == c->P.pn_flags && B.g_exact == exact && B.g_wpb == B.waves_per_block &&
                     B.g_pack == B.pack && B.g_G == c->P.G && B.g_c == c->P.c && B.g_soft == c->P.softening;
   if (!same) {
+    if (B.exec) ASTER_HIP(c, hipStreamSynchronize(c->stream));  // a replay of the old graph may still run
     drop_graph(B);
     ASTER_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     const int lrc = enqueue();
@@ -747,7 +756,7 @@ int batch_step(aster_ctx* c, int integrator, double dt, int64_t nsteps, int spl,
     B.g_integrator = integrator;
     B.g_spl = spl;
     B.g_dt = dt;
-    B.g_pn = pn;
+    B.g_pn = c->P.pn_flags;  // the captured arguments carry every force-law flag
     B.g_exact = exact;
     B.g_wpb = B.waves_per_block;
     B.g_pack = B.pack;

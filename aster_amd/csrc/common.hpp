@@ -117,6 +117,9 @@ struct aster_ctx {
   aster::DBuf<double> stage;    // AoS-d3 staging for host transfers (7 n)
   double e0 = 0.0;              // first Hamiltonian after a state load (error collector)
   bool e0_valid = false;
+  double mass_total = 0.0;      // sum of the masses and largest |coordinate| when they were taken (FP32
+  double extent = 0.0;          // force scaling), valid while mass_total_valid (the state was not reloaded)
+  bool mass_total_valid = false;
   aster::DBuf<double> boxd;     // box half-extent used (3) + |box| (1)
 
   // Barnes-Hut / LBVH.
@@ -146,6 +149,7 @@ struct aster_ctx {
   aster::DBuf<float4> kids;                        // group traversal: per child slot {record}, {node, crit2} (2(2n-1))
   aster::DBuf<unsigned long long> bh_counters;     // BHC_COUNT
   aster::DBuf<float4> gsh;                         // bodies in sorted order (FP32), n
+  aster::DBuf<double> sub_x, sub_v, sub_a, sub_m;  // body range [lower, upper) of a bounded Barnes-Hut evaluation
   bool tree_valid = false;
   int tree_mac = -1;
   int tree_levels = 0;                             // depths of internal nodes: 0 .. tree_levels-1
@@ -183,17 +187,36 @@ constexpr int kApTile = 64;    // sources per accumulation tile
 
 inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 
+// Scaling of the FP32 pair-force kernels. They sum m_j d / (d.d + eps^2)^1.5
+// with G factored out; in SI units (m ~ 1e30 kg, eps^2 ~ 1e-14 m^2) a body's
+// own pair (d = 0, always in the group lists) or two coincident bodies give
+// m eps^-3 ~ 1e51, beyond FP32, and inf * 0 = NaN. The kernels therefore see
+// masses times msc = 2^-k and the sums are multiplied by gsc = G 2^k, with
+// k >= 0 the smallest power that keeps (total mass) msc max(r^-3, r^-2) below
+// 2^120 for every r >= eps. Powers of two scale exactly, so k = 0 (every
+// problem in code units) leaves the arithmetic unchanged, and a larger k only
+// moves far, light contributions toward FP32 underflow. With eps^2 not a
+// positive normal FP32 number the kernels run their masked variant (zero
+// force at r = 0) with eps^2 = 0, and the bound takes r >= 2^-40 times the
+// largest |coordinate| instead of eps.
+struct Fp32Scales {
+  float eps2;   // FP32 softening eps^2 (0 when masked)
+  bool mask;    // zero the pair term at r^2 = 0
+  float msc;    // mass multiplier 2^-k
+  double gsc;   // result multiplier G 2^k
+};
 // Launchers (each returns an aster_status).
 int launch_allpairs(aster_ctx* c, int method);           // writes c->a
 int launch_kick_drift(aster_ctx* c, double dt_prev, double dt);
 int launch_kick(aster_ctx* c, double dt);
 int launch_saba_stage(aster_ctx* c, bool kick, double d, double cdrift, double dt);
-int launch_shadow(aster_ctx* c);
 int launch_energy(aster_ctx* c, double* out6_host);
 int import_state(aster_ctx* c, int64_t n, const double* d_pos, const double* d_vel, const double* d_mass);
 int export_plane(aster_ctx* c, const double* soa, double* d_aos);
 int grow_state(aster_ctx* c, int64_t add, int64_t* base);  // appends `add` bodies; *base = first new index
 int center_of_mass(aster_ctx* c, double* com);
+int total_mass(aster_ctx* c, double* mtot, double* extent);  // sum of the masses, largest |coordinate| (cached)
+int fp32_scales(aster_ctx* c, Fp32Scales* s);
 int launch_reset_flags(aster_ctx* c);
 int launch_box(aster_ctx* c, bool user_box);             // fills c->boxd
 int bh_forces(aster_ctx* c, int method);                 // writes c->a
@@ -201,6 +224,7 @@ int bh_morton_only(aster_ctx* c);
 int morton_sort(aster_ctx* c, bool user_box);            // c->codes sorted, c->idx = perm
 int batch_step(aster_ctx* c, int integrator, double dt, int64_t nsteps, int spl, int use_graph);
 int batch_energy(aster_ctx* c, double* e, double* L);
+void batch_params_changed(aster_ctx* c);
 int batch_autotune(aster_ctx* c, int integrator, double dt, int32_t* spb, int32_t* spl);
 int traj_forward(aster_ctx* c, double dt, int64_t nsteps, int64_t ckpt, int32_t nsnap, const int64_t* snaps,
                  double* snap_pos);

@@ -73,17 +73,6 @@ __global__ __launch_bounds__(256) void k_kick(int n, double* __restrict__ v, con
   }
 }
 
-// FP32 shadow {x,y,z,m} over npad entries; padding bodies have zero mass.
-__global__ void k_shadow(int n, int npad, const double* __restrict__ x, const double* __restrict__ m,
-                         float4* __restrict__ sh) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npad) return;
-  if (i < n)
-    sh[i] = make_float4((float)x[i], (float)x[n + i], (float)x[2 * n + i], (float)m[i]);
-  else
-    sh[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-}
-
 // Per-body energy terms: out[0][i] = kinetic, out[1][i] = sum_j G m_i m_j /
 // (|r| + soft), out[2][i] = sum_j G m_i m_j / sqrt(r^2 + soft), out[3..5][i] =
 // m x cross v. Pair sums run over all j != i (each pair counted twice).
@@ -228,14 +217,6 @@ int launch_saba_stage(aster_ctx* c, bool kick, double d, double cdrift, double d
 int launch_kick(aster_ctx* c, double dt) {
   const int n = (int)c->n;
   hipLaunchKernelGGL(k_kick, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->v.p, c->a.p, dt);
-  ASTER_HIP(c, hipGetLastError());
-  return ASTER_OK;
-}
-
-int launch_shadow(aster_ctx* c) {
-  const int npad = (int)c->npad;
-  hipLaunchKernelGGL(k_shadow, dim3((npad + 255) / 256), dim3(256), 0, c->stream, (int)c->n, npad, c->x.p, c->m.p,
-                     c->sh.p);
   ASTER_HIP(c, hipGetLastError());
   return ASTER_OK;
 }

@@ -21,6 +21,8 @@
 //    every accepted/skipped node continues at its escape link, which is the
 //    next node of that same order, so the FP64 traversal accumulates in the
 //    reference's order and is bit-identical to get_node_body (:202-261).
+#include <utility>
+
 #include "common.hpp"
 #include "device.hpp"
 
@@ -960,7 +962,7 @@ __global__ __launch_bounds__(256) void k_links(int n, const int32_t* __restrict_
 // FP32 per-body stackless traversal (lanes = consecutive sorted bodies).
 template <bool MASK>
 __global__ __launch_bounds__(256) void k_bh_f32(int n, const float4* __restrict__ gsh, const float4* __restrict__ nrec,
-                                                const int4* __restrict__ nlink, float eps2, double G,
+                                                const int4* __restrict__ nlink, float eps2, float msc, double G,
                                                 double* __restrict__ a, unsigned long long* __restrict__ counters,
                                                 uint32_t* __restrict__ flags) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -988,7 +990,7 @@ __global__ __launch_bounds__(256) void k_bh_f32(int n, const float4* __restrict_
       const float r2 = d2 + eps2;
       float inv = __builtin_amdgcn_rsqf(r2);
       if (MASK) inv = (r2 > 0.f) ? inv : 0.f;
-      const float f = (r.w * inv) * (inv * inv);
+      const float f = ((r.w * msc) * inv) * (inv * inv);
       ax = fmaf(f, dx, ax);
       ay = fmaf(f, dy, ay);
       az = fmaf(f, dz, az);
@@ -1228,7 +1230,7 @@ template <bool MASK>
 __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restrict__ gstart,
                                                  const uint32_t* __restrict__ gcount, const float4* __restrict__ gsh,
                                                  const float4* __restrict__ kids, const uint32_t* __restrict__ perm,
-                                                 float eps2, double G,
+                                                 float eps2, float msc, double G,
                                                  double* __restrict__ a, unsigned long long* __restrict__ counters,
                                                  uint32_t* __restrict__ flags, int stack_cap) {
   extern __shared__ float4 grp_lds[];
@@ -1324,7 +1326,7 @@ __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restric
       const int root = n, rs = 2 * (n - 1);
       const float4 r = kids[2 * rs];
       if (dmin2(r) > kids[2 * rs + 1].y) {
-        if (lane == 0) lst[0] = r;
+        if (lane == 0) lst[0] = make_float4(r.x, r.y, r.z, r.w * msc);
         cnt = 1;
       } else {
         if (lane == 0) stk[0] = root;
@@ -1372,7 +1374,8 @@ __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restric
         if (lane == 0) atomicOr(flags, (uint32_t)FLAG_TREE);
         overflow = true;
       }
-      if (acc) lst[cnt + __popcll(am & lt)] = r;
+      // list entries carry the scaled mass (one multiply per entry, not per pair)
+      if (acc) lst[cnt + __popcll(am & lt)] = make_float4(r.x, r.y, r.z, r.w * msc);
       if (opn && !over) stk[sp + __popcll(om & lt)] = node;
       cnt += (int)__popcll(am);
       sp = over ? 0 : sp + nopen;
@@ -1656,24 +1659,21 @@ static int build_tree(aster_ctx* c, int mac, bool vel, bool full) {
   return ASTER_OK;
 }
 
-int bh_forces(aster_ctx* c, int method) {
+// Barnes-Hut over every body of the context's state.
+static int bh_forces_all(aster_ctx* c, int method) {
   const int n = (int)c->n;
   const int pnf = c->P.pn_flags & ASTER_PN_TERMS;
   const bool pn = pnf != 0;
   const bool f64 = method == ASTER_BH_PARITY && c->P.precision == ASTER_FP64;
-  if (pnf & ASTER_EIH) return fail(c, ASTER_ERR_INVALID, "ASTER_EIH needs ASTER_ALLPAIRS_FP64");
-  if (pn && !f64)
-    return fail(c, ASTER_ERR_INVALID, "Barnes-Hut with post-Newtonian terms needs ASTER_BH_PARITY with precision ASTER_FP64");
-  if (!(c->P.theta >= 0.0)) return fail(c, ASTER_ERR_INVALID, "theta must be >= 0");
-  if (c->P.lower > 0 || (c->P.upper >= 0 && c->P.upper < n))
-    return fail(c, ASTER_ERR_INVALID, "Barnes-Hut evaluates the full body range only");
   if (n == 1) {
     ASTER_HIP(c, hipMemsetAsync(c->a.p, 0, 3 * sizeof(double), c->stream));
     return ASTER_OK;
   }
   const int mac = (method == ASTER_BH_PARITY) ? 0 : 1;
-  int rc = build_tree(c, mac, pn, method != ASTER_BH_GROUP);
-  if (rc) return rc;
+  Fp32Scales sc{};
+  int rc;
+  if (!f64 && (rc = fp32_scales(c, &sc))) return rc;
+  if ((rc = build_tree(c, mac, pn, method != ASTER_BH_GROUP))) return rc;
   phase_begin(c, ASTER_PHASE_TRAVERSE);
   const double ic = 1.0 / c->P.c;
   if (f64) {
@@ -1686,8 +1686,8 @@ int bh_forces(aster_ctx* c, int method) {
                          c->nmc.p, c->nvel.p, c->nlink.p, c->depth.p, c->boxd.p, c->P.theta, c->P.G, ic, pnf,
                          c->P.softening, c->a.p, c->bh_counters.p, c->flags.p);
   } else {
-    const float eps2 = (float)c->P.softening;
-    const bool mask = !(c->P.softening > 0.0);
+    const float eps2 = sc.eps2;
+    const bool mask = sc.mask;
     if (method == ASTER_BH_GROUP) {
       // One workgroup per group while groups average more than 32 bodies
       // (the usual case); workgroups loop when there are more groups. The
@@ -1698,18 +1698,82 @@ int bh_forces(aster_ctx* c, int method) {
       const size_t lds = (kGrpList + kGrpListPad) * sizeof(float4) + (size_t)cap * sizeof(int);
       auto kern = mask ? k_bh_group<true> : k_bh_group<false>;
       hipLaunchKernelGGL(kern, grid, block, lds, c->stream, n, c->gstart.p, c->gcount.p, c->gsh.p, c->kids.p,
-                         c->idx.p, eps2, c->P.G, c->a.p, c->bh_counters.p, c->flags.p, cap);
+                         c->idx.p, eps2, sc.msc, sc.gsc, c->a.p, c->bh_counters.p, c->flags.p, cap);
     } else if (mask) {
       hipLaunchKernelGGL((k_bh_f32<true>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->gsh.p, c->nrec.p,
-                         c->nlink.p, eps2, c->P.G, c->a.p, c->bh_counters.p, c->flags.p);
+                         c->nlink.p, eps2, sc.msc, sc.gsc, c->a.p, c->bh_counters.p, c->flags.p);
     } else {
       hipLaunchKernelGGL((k_bh_f32<false>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->gsh.p, c->nrec.p,
-                         c->nlink.p, eps2, c->P.G, c->a.p, c->bh_counters.p, c->flags.p);
+                         c->nlink.p, eps2, sc.msc, sc.gsc, c->a.p, c->bh_counters.p, c->flags.p);
     }
   }
   ASTER_HIP(c, hipGetLastError());
   phase_end(c, ASTER_PHASE_TRAVERSE);
   return ASTER_OK;
+}
+
+// Solver::set_bounds on Barnes-Hut (Barnes_Hut::make_tree and compute_forces,
+// src/solvers/barnes_hut.cpp:309-322,414-417): the tree holds the bodies
+// [lo, hi) only, and only those bodies get new accelerations; the others keep
+// theirs. (The reference indexes the leaves from 0 instead of lower and takes
+// the root at the full body count, so its own bounded tree is only right for
+// the full range; this follows what the bounds mean.) The range is copied to
+// contiguous sub-state planes, evaluated as a state of hi - lo bodies, and
+// the accelerations copied back.
+static int bh_forces_range(aster_ctx* c, int method, int lo, int hi) {
+  const int64_t n = c->n, r = (int64_t)hi - lo;
+  ASTER_HIP(c, c->sub_x.reserve((size_t)3 * r));
+  ASTER_HIP(c, c->sub_v.reserve((size_t)3 * r));
+  ASTER_HIP(c, c->sub_a.reserve((size_t)3 * r));
+  ASTER_HIP(c, c->sub_m.reserve((size_t)r));
+  for (int q = 0; q < 3; ++q) {
+    ASTER_HIP(c, hipMemcpyAsync(c->sub_x.p + q * r, c->x.p + q * n + lo, r * sizeof(double), hipMemcpyDeviceToDevice,
+                                c->stream));
+    ASTER_HIP(c, hipMemcpyAsync(c->sub_v.p + q * r, c->v.p + q * n + lo, r * sizeof(double), hipMemcpyDeviceToDevice,
+                                c->stream));
+  }
+  ASTER_HIP(c, hipMemcpyAsync(c->sub_m.p, c->m.p + lo, r * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  // evaluate on the sub-state, then put the full state back whatever happened
+  auto swap_state = [c]() {
+    std::swap(c->x, c->sub_x);
+    std::swap(c->v, c->sub_v);
+    std::swap(c->a, c->sub_a);
+    std::swap(c->m, c->sub_m);
+  };
+  const double mt = c->mass_total;
+  const bool mt_valid = c->mass_total_valid;
+  swap_state();
+  c->n = r;
+  c->npad = round_up(r, kApBlock);
+  c->mass_total_valid = false;
+  int rc = bh_forces_all(c, method);
+  swap_state();
+  c->n = n;
+  c->npad = round_up(n, kApBlock);
+  c->mass_total = mt;
+  c->mass_total_valid = mt_valid;
+  c->tree_valid = false;  // the tree holds the sub-state's bodies
+  if (rc) return rc;
+  for (int q = 0; q < 3; ++q)
+    ASTER_HIP(c, hipMemcpyAsync(c->a.p + q * n + lo, c->sub_a.p + q * r, r * sizeof(double), hipMemcpyDeviceToDevice,
+                                c->stream));
+  return ASTER_OK;
+}
+
+int bh_forces(aster_ctx* c, int method) {
+  const int n = (int)c->n;
+  const int pnf = c->P.pn_flags & ASTER_PN_TERMS;
+  const bool f64 = method == ASTER_BH_PARITY && c->P.precision == ASTER_FP64;
+  if (pnf & ASTER_EIH) return fail(c, ASTER_ERR_INVALID, "ASTER_EIH needs ASTER_ALLPAIRS_FP64");
+  if (pnf && !f64)
+    return fail(c, ASTER_ERR_INVALID, "Barnes-Hut with post-Newtonian terms needs ASTER_BH_PARITY with precision ASTER_FP64");
+  if (!(c->P.theta >= 0.0)) return fail(c, ASTER_ERR_INVALID, "theta must be >= 0");
+  const int lo = c->P.lower < 0 ? 0 : c->P.lower;
+  const int hi = (c->P.upper < 0 || c->P.upper > n) ? n : c->P.upper;
+  if (lo > hi) return fail(c, ASTER_ERR_INVALID, "body bounds: lower > upper");
+  if (lo == 0 && hi == n) return bh_forces_all(c, method);
+  if (lo == hi) return ASTER_OK;  // no bodies in range: nothing is evaluated (the reference warns and returns)
+  return bh_forces_range(c, method, lo, hi);
 }
 
 }  // namespace aster

@@ -41,38 +41,48 @@ __global__ __launch_bounds__(256) void k_export(int64_t n, const double* __restr
   for (int c = 0; c < 3; ++c) aos[3 * i + c] = soa[c * n + i];
 }
 
-// Per-block sums of m and m x (x, y, z), in a fixed order: each thread sums a
-// strided set of bodies, then the block's waves are summed in lane order.
+// Per-block sums of m and m x (x, y, z), in a fixed order (each thread sums a
+// strided set of bodies, then the block's waves are summed in lane order),
+// and the largest |coordinate|.
 constexpr int kComBlocks = 512;
+constexpr int kComQ = 5;  // m, m x, m y, m z sums; max |x|
 
 __global__ __launch_bounds__(256) void k_com_partial(int64_t n, const double* __restrict__ x,
                                                      const double* __restrict__ m, double* __restrict__ part) {
-  __shared__ double red[4][256];
-  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  __shared__ double red[kComQ][256];
+  double s[kComQ] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const double mi = m[i];
     s[0] += mi;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) s[1 + c] += mi * x[c * n + i];
+    for (int c = 0; c < 3; ++c) {
+      const double xc = x[c * n + i];
+      s[1 + c] += mi * xc;
+      s[4] = fmax(s[4], fabs(xc));
+    }
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = s[q];
+  for (int q = 0; q < kComQ; ++q) red[q][threadIdx.x] = s[q];
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
+    if ((int)threadIdx.x < w) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+      red[4][threadIdx.x] = fmax(red[4][threadIdx.x], red[4][threadIdx.x + w]);
+    }
     __syncthreads();
   }
-  if (threadIdx.x < 4) part[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < kComQ) part[kComQ * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
 __global__ void k_com_final(int nparts, const double* __restrict__ part, double* __restrict__ out) {
   if (threadIdx.x != 0) return;
-  double s[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int b = 0; b < nparts; ++b)
-    for (int q = 0; q < 4; ++q) s[q] += part[4 * b + q];
-  for (int q = 0; q < 4; ++q) out[q] = s[q];
+  double s[kComQ] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int b = 0; b < nparts; ++b) {
+    for (int q = 0; q < 4; ++q) s[q] += part[kComQ * b + q];
+    s[4] = fmax(s[4], part[kComQ * b + 4]);
+  }
+  for (int q = 0; q < kComQ; ++q) out[q] = s[q];
 }
 
 static int reserve_state(aster_ctx* c, int64_t n) {
@@ -112,6 +122,7 @@ int import_state(aster_ctx* c, int64_t n, const double* d_pos, const double* d_v
   c->tree_valid = false;
   c->adj.ready = false;
   c->e0_valid = false;
+  c->mass_total_valid = false;
   return ASTER_OK;
 }
 
@@ -159,6 +170,7 @@ int grow_state(aster_ctx* c, int64_t add, int64_t* base) {
   c->tree_valid = false;
   c->adj.ready = false;
   c->e0_valid = false;
+  c->mass_total_valid = false;
   *base = n0;
   return ASTER_OK;
 }
@@ -171,20 +183,41 @@ int export_plane(aster_ctx* c, const double* soa, double* d_aos) {
   return ASTER_OK;
 }
 
-// Total mass and mass-weighted position sums, summed in a fixed order.
-int center_of_mass(aster_ctx* c, double* com) {
+// Total mass and mass-weighted position sums, summed in a fixed order, and
+// the largest |coordinate|.
+static int mass_sums(aster_ctx* c, double* s) {
   const int64_t n = c->n;
   int blocks = (int)std::min<int64_t>((n + 255) / 256, kComBlocks);
-  ASTER_HIP(c, c->ework.reserve((size_t)4 * kComBlocks + 4));
+  ASTER_HIP(c, c->ework.reserve((size_t)kComQ * kComBlocks + kComQ));
   double* part = c->ework.p;
-  double* out = c->ework.p + 4 * kComBlocks;
+  double* out = c->ework.p + kComQ * kComBlocks;
   hipLaunchKernelGGL(k_com_partial, dim3(blocks), dim3(256), 0, c->stream, n, c->x.p, c->m.p, part);
   hipLaunchKernelGGL(k_com_final, dim3(1), dim3(64), 0, c->stream, blocks, part, out);
   ASTER_HIP(c, hipGetLastError());
-  double s[4];
-  ASTER_HIP(c, hipMemcpyAsync(s, out, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+  ASTER_HIP(c, hipMemcpyAsync(s, out, kComQ * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   ASTER_HIP(c, hipStreamSynchronize(c->stream));
+  c->mass_total = s[0];
+  c->extent = s[4];
+  c->mass_total_valid = true;
+  return ASTER_OK;
+}
+
+int center_of_mass(aster_ctx* c, double* com) {
+  double s[kComQ];
+  int rc = mass_sums(c, s);
+  if (rc) return rc;
   for (int q = 0; q < 3; ++q) com[q] = s[1 + q] / s[0];
+  return ASTER_OK;
+}
+
+int total_mass(aster_ctx* c, double* mtot, double* extent) {
+  if (!c->mass_total_valid) {
+    double s[kComQ];
+    int rc = mass_sums(c, s);
+    if (rc) return rc;
+  }
+  *mtot = c->mass_total;
+  if (extent) *extent = c->extent;
   return ASTER_OK;
 }
 

@@ -103,8 +103,11 @@ typedef struct aster_params {
                        all zero = symmetric max |coordinate| of the state */
   int32_t pn_flags; /* ASTER_NEWTON | ASTER_PN1 | ASTER_PN2 | ASTER_PN25, or ASTER_NEWTON | ASTER_EIH */
   int32_t precision;/* ASTER_FP32 / ASTER_FP64 for ASTER_BH_PARITY */
-  int32_t lower;    /* source range [lower, upper) for all-pairs, as */
-  int32_t upper;    /* Solver::set_bounds (sim_obj.h:45); upper < 0 = n */
+  int32_t lower;    /* Solver::set_bounds (sim_obj.h:45), upper < 0 = n:  */
+  int32_t upper;    /* all-pairs: source range [lower, upper) (Parallelized::compute_forces,
+                       builder_endpoint.cpp:729-757); Barnes-Hut: body range, the tree holds
+                       bodies [lower, upper) and only they get new accelerations
+                       (Barnes_Hut::compute_forces, barnes_hut.cpp:309-322,414-417) */
   int32_t split;    /* all-pairs source slices (0 = autotuned choice) */
   int32_t reserved;
 } aster_params;

@@ -51,8 +51,15 @@ inline void check(aster_ctx* ctx, int rc) {
   if (rc != ASTER_OK) throw std::runtime_error(std::string("aster_hip: ") + aster_last_error(ctx));
 }
 
-// Owns one device context; bodies are uploaded at load() and whenever the
-// host copy is declared dirty.
+// Owns one device context. Which copy of the bodies is current:
+//  - compute_forces() uploads the host bodies before it evaluates, because
+//    Aster's host updaters (update_leapfrog, tool_chain_impl.cpp:169-185)
+//    move positions and velocities on the host between evaluations; the one
+//    exception is a device updater that ran without reading the state back,
+//    after which the device copy is the current one;
+//  - the device updaters (update_*_hip below) advance the device copy and
+//    read it back; they upload first only after load() or mark_dirty() (the
+//    host bodies were edited).
 class Solver {
  public:
   Solver(BodyArray* bodies, Meta* meta, int method, int device = 0) : b_(bodies), meta_(meta), method_(method) {
@@ -64,8 +71,11 @@ class Solver {
   Solver(const Solver&) = delete;
   Solver& operator=(const Solver&) = delete;
 
-  // Solver::set_bounds (builder_endpoint.cpp:64-67): source range for
-  // all-pairs.
+  // Solver::set_bounds (builder_endpoint.cpp:64-67): the source range of the
+  // all-pairs methods (Parallelized::compute_forces); the body range of
+  // Barnes-Hut, whose tree holds those bodies and whose evaluation writes
+  // their accelerations only (Barnes_Hut::compute_forces,
+  // barnes_hut.cpp:309-322,414-417).
   void set_bounds(int lower, int upper = -1) {
     lower_ = lower;
     upper_ = upper;
@@ -77,15 +87,41 @@ class Solver {
     params();
     upload();
   }
-  // Solver::compute_forces postcondition: accs[i] = acceleration of body i.
+  // Solver::compute_forces postcondition: accs[i] = acceleration of body i
+  // (for Barnes-Hut with bounds: of the bodies in range; the others keep
+  // their accs).
   virtual void compute_forces() {
-    if (dirty_) upload();
-    b_->accs.resize(b_->positions.size());
-    check(ctx_, aster_forces(ctx_, method_, &b_->accs[0].x));
+    if (!device_ahead_) upload();
+    const size_t n = b_->positions.size();
+    b_->accs.resize(n);
+    const size_t lo = lower_ > 0 ? (size_t)lower_ : 0;
+    const size_t hi = (upper_ < 0 || (size_t)upper_ > n) ? n : (size_t)upper_;
+    if (bounds_select_targets() && (lo > 0 || hi < n)) {
+      scratch_.resize(n);
+      check(ctx_, aster_forces(ctx_, method_, &scratch_[0].x));
+      for (size_t i = lo; i < hi; ++i) b_->accs[i] = scratch_[i];
+    } else {
+      check(ctx_, aster_forces(ctx_, method_, &b_->accs[0].x));
+    }
   }
-  // Host positions/velocities changed outside the device (e.g. user edits).
+  // Host positions/velocities changed outside the device (e.g. user edits):
+  // the next device updater uploads them first.
   void mark_dirty() { dirty_ = true; }
-  void download() { check(ctx_, aster_get_state(ctx_, &b_->positions[0].x, &b_->velocities[0].x)); }
+  void download() {
+    check(ctx_, aster_get_state(ctx_, &b_->positions[0].x, &b_->velocities[0].x));
+    device_ahead_ = false;
+  }
+  // Device updaters: the host copy, if edited, goes up first; afterwards
+  // either the state is read back or the device copy is ahead of the host.
+  void before_device_step() {
+    if (dirty_) upload();
+  }
+  void after_device_step(bool read_back) {
+    if (read_back)
+      download();
+    else
+      device_ahead_ = true;
+  }
   aster_ctx* context() { return ctx_; }
   int method() const { return method_; }
 
@@ -104,12 +140,15 @@ class Solver {
     check(ctx_, aster_set_params(ctx_, &p));
   }
   virtual void configure(aster_params&) {}
+  // true when bounds select the bodies that get accelerations (Barnes-Hut)
+  virtual bool bounds_select_targets() const { return false; }
   void upload() {
     const int64_t n = (int64_t)b_->positions.size();
     if (b_->velocities.size() != (size_t)n || b_->masses.size() != (size_t)n)
       throw std::runtime_error("aster_hip: positions, velocities and masses must have the same length");
     check(ctx_, aster_set_state(ctx_, n, &b_->positions[0].x, &b_->velocities[0].x, b_->masses.data()));
     dirty_ = false;
+    device_ahead_ = false;
   }
 
   BodyArray* b_;
@@ -117,7 +156,9 @@ class Solver {
   int method_;
   aster_ctx* ctx_ = nullptr;
   int lower_ = 0, upper_ = -1;
-  bool dirty_ = true;
+  bool dirty_ = true;          // host bodies edited since the last upload
+  bool device_ahead_ = false;  // a device updater advanced the state without reading it back
+  std::vector<vec3> scratch_;
 };
 
 // Replaces SimpleGPU (src/solvers/SimpleGPU.cpp) and Parallelized
@@ -150,6 +191,7 @@ class HipBarnesHut : public Solver {
     p.precision = precision_;
     for (int k = 0; k < 3; ++k) p.box[k] = box_[k];
   }
+  bool bounds_select_targets() const override { return true; }
   int precision_;
   double box_[3] = {0, 0, 0};
 };
@@ -158,26 +200,30 @@ class HipBarnesHut : public Solver {
 // the device and copy positions/velocities back, as read_bodies_gpu does
 // when always_read_pos is set (builder_endpoint.cpp:280-281).
 inline void update_leapfrog_hip(Solver& s, const Meta& meta, int64_t steps = 1, bool read_back = true) {
+  s.before_device_step();
   check(s.context(), aster_step(s.context(), s.method(), ASTER_LEAPFROG, meta.dt, steps));
-  if (read_back) s.download();
+  s.after_device_step(read_back);
 }
 
 inline void update_yoshida4_hip(Solver& s, const Meta& meta, int64_t steps = 1, bool read_back = true) {
+  s.before_device_step();
   check(s.context(), aster_step(s.context(), s.method(), ASTER_YOSHIDA4, meta.dt, steps));
-  if (read_back) s.download();
+  s.after_device_step(read_back);
 }
 
 // update_SABA1..10 (src/integrators/SABA.cpp:19-553), picked by order as
 // bake_update_function(SABA, ord) does (src/utils/updater.cpp:144-152);
 // order 1 is update_euler (tool_chain_impl.cpp:158-167).
 inline void update_saba_hip(Solver& s, const Meta& meta, int order, int64_t steps = 1, bool read_back = true) {
+  s.before_device_step();
   check(s.context(), aster_integrate(s.context(), s.method(), ASTER_SABA, order, meta.dt, steps));
-  if (read_back) s.download();
+  s.after_device_step(read_back);
 }
 
 inline void update_euler_hip(Solver& s, const Meta& meta, int64_t steps = 1, bool read_back = true) {
+  s.before_device_step();
   check(s.context(), aster_integrate(s.context(), s.method(), ASTER_EULER, 1, meta.dt, steps));
-  if (read_back) s.download();
+  s.after_device_step(read_back);
 }
 
 }  // namespace aster_hip

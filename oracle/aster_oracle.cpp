@@ -839,20 +839,34 @@ int orc_yoshida_weights(double* w) {
 // update_leapfrog, tool_chain_impl.cpp:169-185: F(x); x += v dt + (0.5 a) dt dt;
 // v += (0.5 a) dt; F(x); v += (0.5 a) dt. integrator 2 = leapfrog, 5 =
 // Yoshida-4 (three leapfrog steps of w_k dt). method 0 = all-pairs, 1 =
-// Barnes-Hut with `box`/`theta`. `acc` (3n) receives the last accelerations.
+// Barnes-Hut with `box`/`theta`, 2 = the group traversal (orc_bh_group,
+// Newtonian) with `theta`; for methods 1 and 2 an all-zero box is the
+// symmetric box of the positions at each evaluation, as the library's
+// automatic box. `acc` (3n) receives the last accelerations.
 // Pinned by: order-2/order-4 convergence on a Kepler orbit, time reversibility,
 // energy drift bounds.
 int orc_integrate(int64_t n, double* pos, double* vel, const double* mass, double G, double c, double soft,
                   int pn, int method, const double* box, double theta, int integrator, double dt, int64_t nsteps,
                   double* acc_out, int nthreads) {
   if (integrator != 2 && integrator != 5) return -1;
+  if (method < 0 || method > 2) return -1;
   std::vector<double> acc(3 * n, 0.0);
   Law L{G, c, soft, pn};
+  const bool auto_box = method != 0 && box[0] == 0.0 && box[1] == 0.0 && box[2] == 0.0;
+  double bx[3] = {0.0, 0.0, 0.0};
   auto forces = [&]() {
-    if (method == 0)
+    if (method == 0) {
       allpairs(n, pos, vel, mass, L, 0, n, acc.data(), nthreads);
+      return;
+    }
+    if (auto_box)
+      orc_box_auto(n, pos, bx);
     else
-      orc_bh(n, pos, vel, mass, box, theta, G, c, soft, pn, acc.data(), nullptr, nthreads);
+      for (int k = 0; k < 3; ++k) bx[k] = box[k];
+    if (method == 1)
+      orc_bh(n, pos, vel, mass, bx, theta, G, c, soft, pn, acc.data(), nullptr, nthreads);
+    else
+      orc_bh_group(n, pos, mass, bx, theta, G, soft, acc.data(), nullptr, nullptr, nthreads);
   };
   double w[3] = {1.0, 0.0, 0.0};
   int nsub = 1;

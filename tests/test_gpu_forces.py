@@ -283,6 +283,8 @@ def test_errors_are_reported(ctx):
         ctx.step(A.ALLPAIRS_FP64, 3, 0.1, 1)
     with pytest.raises(aster_amd.AsterError):
         ctx.set_params(softening=-1.0)
-    ctx.set_params(lower=0, upper=10)
+    ctx.set_params(lower=10, upper=5)
     with pytest.raises(aster_amd.AsterError):
-        ctx.forces(A.ALLPAIRS_FP32_KAHAN)
+        ctx.forces(A.ALLPAIRS_FP32_KAHAN)  # an inverted range
+    with pytest.raises(aster_amd.AsterError):
+        ctx.forces(A.BH_GROUP)

@@ -425,6 +425,24 @@ def test_bh_group_theta_zero_is_allpairs_on_fp32_input(orc):
     np.testing.assert_allclose(a, ref, rtol=1e-11, atol=1e-13 * np.abs(ref).max())
 
 
+def test_integrate_over_group_forces_at_theta_zero_is_allpairs_leapfrog(orc):
+    # integrate method 2 (leapfrog over the group traversal, automatic box per
+    # evaluation) with every node opened is update_leapfrog over all-pairs, up
+    # to the FP32 rounding of the group records (~1e-7 of each force)
+    import aster_amd
+    n, dt, steps = 400, 1e-3, 3
+    pos, vel, mass = aster_amd.ic_mixed(n, 23)
+    p2, v2, a2 = orc.integrate(pos, vel, mass, dt, steps, integrator=2, method=2, soft=2.5e-5, theta=1e-6)
+    p0, v0, a0 = orc.integrate(pos, vel, mass, dt, steps, integrator=2, method=0, soft=2.5e-5)
+    amax = np.abs(a0).max()
+    assert np.abs(p2 - p0).max() < 1e-6 * steps ** 2 * dt ** 2 * amax
+    assert np.abs(v2 - v0).max() < 1e-6 * steps * dt * amax
+    assert np.abs(a2 - a0).max() < 1e-5 * amax
+    # and it is not all-pairs at an ordinary opening angle
+    p5, _, _ = orc.integrate(pos, vel, mass, dt, steps, integrator=2, method=2, soft=2.5e-5, theta=0.7)
+    assert np.abs(p5 - p0).max() > 1e-4 * steps ** 2 * dt ** 2 * amax
+
+
 @pytest.mark.parametrize("theta", [0.3, 0.5, 0.7, 1.5])
 def test_bh_group_walk_matches_bruteforce(orc, theta):
     import aster_amd
