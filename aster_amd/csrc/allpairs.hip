@@ -304,18 +304,21 @@ __global__ __launch_bounds__(256) void k_eih_pass2(int n, const double* __restri
 
 // --------------------------------------------------------------- launch ----
 
-int choose_split(aster_ctx* c) {
-  if (c->P.split > 0) return c->P.split;
-  if (c->tuned_split > 0) return c->tuned_split;
+int default_split(int64_t npad) {
   // Aim for about 8 waves per SIMD (256 CUs x 4 SIMDs) while keeping each
   // slice at least 1024 sources.
-  const int64_t waves = c->npad / 64;
+  const int64_t waves = npad / 64;
   int64_t S = (8 * 1024 + waves - 1) / waves;
-  const int64_t max_by_len = c->npad / 1024;
+  const int64_t max_by_len = npad / 1024;
   if (S > max_by_len) S = max_by_len;
   if (S < 1) S = 1;
   if (S > 64) S = 64;
   return (int)S;
+}
+
+int choose_split(aster_ctx* c) {
+  if (c->P.split > 0) return c->P.split;
+  return default_split(c->npad);
 }
 
 int fp32_scales(aster_ctx* c, Fp32Scales* sc) {
@@ -325,12 +328,8 @@ int fp32_scales(aster_ctx* c, Fp32Scales* sc) {
   const float e2 = (float)c->P.softening;
   sc->mask = !(e2 >= FLT_MIN);
   sc->eps2 = sc->mask ? 0.f : e2;
-  // log2 of the smallest separation the bound assumes
-  const double lr = sc->mask ? std::log2(R > 0.0 ? R : 1.0) - 40.0 : 0.5 * std::log2((double)sc->eps2);
-  const double lb = std::log2(M) + std::max(-3.0 * lr, -2.0 * lr);
-  int k = (int)std::ceil(lb) - 120;
-  if (k < 0) k = 0;
-  if (k > 126)
+  const int k = fp32_mass_shift(M, R, sc->eps2, sc->mask);
+  if (k > kMaxMassShift)
     return fail(c, ASTER_ERR_INVALID,
                 "masses and softening are beyond the FP32 force range (use an FP64 method or a larger softening)");
   sc->msc = std::ldexp(1.0f, -k);

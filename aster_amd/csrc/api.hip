@@ -29,6 +29,8 @@ int hip_fail(aster_ctx* c, hipError_t e, const char* what) {
 
 static thread_local std::string g_create_err = "no error";
 
+void set_global_error(const std::string& msg) { g_create_err = msg; }
+
 static int check_flags(aster_ctx* c) {
   uint32_t f[4] = {0, 0, 0, 0};
   ASTER_HIP(c, hipMemcpyAsync(f, c->flags.p, sizeof(f), hipMemcpyDeviceToHost, c->stream));

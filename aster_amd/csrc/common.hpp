@@ -168,7 +168,6 @@ struct aster_ctx {
   aster::BatchState batch;
   aster::AdjointState adj;
   std::vector<aster_cluster> clusters;  // queued, not yet generated (aster_cluster_load)
-  int tuned_split = 0;
 };
 
 namespace aster {
@@ -205,6 +204,25 @@ struct Fp32Scales {
   float msc;    // mass multiplier 2^-k
   double gsc;   // result multiplier G 2^k
 };
+
+// k of Fp32Scales from the total mass, the largest |coordinate| and the FP32
+// softening, in integer exponent arithmetic (ilogb is exact), so the host and
+// the device derive the same k: mtot < 2^em, every separation r >= 2^er (eps
+// >= 2^floor(e2 / 2) with eps^2 >= 2^e2; masked: 2^-40 of the extent), and
+// max(r^-3, r^-2) <= 2^max(-3 er, -2 er).
+__host__ __device__ inline int fp32_mass_shift(double mtot, double extent, float eps2, bool mask) {
+  const int em = ilogb(mtot) + 1;
+  int er;
+  if (mask) {
+    er = (extent > 0.0 ? ilogb(extent) : 0) - 40;
+  } else {
+    const int e2 = ilogb((double)eps2);
+    er = e2 >= 0 ? e2 / 2 : -((1 - e2) / 2);
+  }
+  const int k = em + (er < 0 ? -3 * er : -2 * er) - 120;
+  return k < 0 ? 0 : k;
+}
+constexpr int kMaxMassShift = 126;  // 2^-k must stay a normal FP32 number
 // Launchers (each returns an aster_status).
 int launch_allpairs(aster_ctx* c, int method);           // writes c->a
 int launch_kick_drift(aster_ctx* c, double dt_prev, double dt);
@@ -232,7 +250,18 @@ int traj_backward(aster_ctx* c, const double* gsnap, double* gx, double* gv, dou
 int batch_traj_forward(aster_ctx* c, double dt, int64_t nsteps, int32_t nsnap, const int64_t* snaps,
                        double* snap_pos);
 int batch_traj_backward(aster_ctx* c, const double* gsnap, double* gx, double* gv, double* gm);
+// Mass sums (fixed order) and the largest |coordinate| over SoA planes:
+// per-block partials of kComQ values (m, m x, m y, m z, max |x|) over at most
+// kComBlocks blocks of 256 threads, then one thread sums them in block order.
+constexpr int kComBlocks = 512;
+constexpr int kComQ = 5;
+__global__ void k_com_partial(int64_t n, const double* __restrict__ x, const double* __restrict__ m,
+                              double* __restrict__ part);
+__global__ void k_com_final(int nparts, const double* __restrict__ part, double* __restrict__ out);
+
 int choose_split(aster_ctx* c);
+int default_split(int64_t npad);                           // source slices of the FP32 all-pairs kernels
+void set_global_error(const std::string& msg);            // aster_last_error(NULL) for calls without a context
 int compute_forces(aster_ctx* c, int method);             // all-pairs or Barnes-Hut into c->a
 void phase_begin(aster_ctx* c, int phase);
 void phase_end(aster_ctx* c, int phase);

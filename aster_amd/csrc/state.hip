@@ -44,9 +44,6 @@ __global__ __launch_bounds__(256) void k_export(int64_t n, const double* __restr
 // Per-block sums of m and m x (x, y, z), in a fixed order (each thread sums a
 // strided set of bodies, then the block's waves are summed in lane order),
 // and the largest |coordinate|.
-constexpr int kComBlocks = 512;
-constexpr int kComQ = 5;  // m, m x, m y, m z sums; max |x|
-
 __global__ __launch_bounds__(256) void k_com_partial(int64_t n, const double* __restrict__ x,
                                                      const double* __restrict__ m, double* __restrict__ part) {
   __shared__ double red[kComQ][256];
