@@ -122,6 +122,7 @@ SIGNATURES = {
     "aster_set_state": (ctypes.c_int, [_P, _I64, _D, _D, _D]),
     "aster_get_state": (ctypes.c_int, [_P, _D, _D]),
     "aster_set_state_device": (ctypes.c_int, [_P, _I64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "aster_batch_info": (ctypes.c_int, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(ctypes.c_int32)]),
     "aster_get_state_device": (ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_void_p]),
     "aster_collect": (ctypes.c_int, [_P, ctypes.c_int, _I64, _D]),
     "aster_center_of_mass": (ctypes.c_int, [_P, _D]),
@@ -422,6 +423,12 @@ class Context:
     def batch_step(self, integrator, dt, nsteps, steps_per_launch=0, use_graph=True):
         self._ck(lib().aster_batch_step(self.h, integrator, float(dt), int(nsteps), int(steps_per_launch),
                                         1 if use_graph else 0))
+
+    def batch_info(self):
+        """(pair interactions, lane-pair slots, wavefronts) of one force evaluation over the batch."""
+        p, l, w = _I64(), _I64(), ctypes.c_int32()
+        self._ck(lib().aster_batch_info(self.h, ctypes.byref(p), ctypes.byref(l), ctypes.byref(w)))
+        return p.value, l.value, w.value
 
     def batch_energy(self):
         e = np.empty(self.batch_nsys)

@@ -320,18 +320,22 @@ int traj_backward(aster_ctx* c, const double* gsnap, double* gx, double* gv, dou
     hipLaunchKernelGGL(k_adj_pre, eg, eb, 0, c->stream, n, dt, cw, c->x.p, c->v.p, c->a.p, S.lx.p, S.lv.p, S.w.p,
                        anchor ? S.ckpt_x.p + co : (const double*)nullptr);
     phase_end(c, ASTER_PHASE_KICKDRIFT);
-    phase_begin(c, ASTER_PHASE_ADJOINT);
+    phase_begin(c, ASTER_PHASE_TILES);
     hipLaunchKernelGGL(k_adj_src, dim3(npad / 256), eb, 0, c->stream, n, npad, sc.msc, c->x.p, c->m.p, S.w.p,
                        c->src64.p, wsrc);
+    phase_end(c, ASTER_PHASE_TILES);
     const dim3 grid(npad / kApBlock, Sl);
+    phase_begin(c, ASTER_PHASE_ADJOINT);
     if (mask)
       hipLaunchKernelGGL((k_adj_fused<true>), grid, dim3(kApBlock), 0, c->stream, c->src64.p, wsrc, c->x.p, n, npad,
                          (int)slice, eps2, S.part.p);
     else
       hipLaunchKernelGGL((k_adj_fused<false>), grid, dim3(kApBlock), 0, c->stream, c->src64.p, wsrc, c->x.p, n, npad,
                          (int)slice, eps2, S.part.p);
-    hipLaunchKernelGGL(k_adj_combine, eg, eb, 0, c->stream, n, Sl, S.part.p, sc.gsc, c->P.G, c->a.p, gxp, gmp);
     phase_end(c, ASTER_PHASE_ADJOINT);
+    phase_begin(c, ASTER_PHASE_COMBINE);
+    hipLaunchKernelGGL(k_adj_combine, eg, eb, 0, c->stream, n, Sl, S.part.p, sc.gsc, c->P.G, c->a.p, gxp, gmp);
+    phase_end(c, ASTER_PHASE_COMBINE);
     const double* gs = nullptr;
     if (q >= 0 && S.snap_steps[q] == s) {
       gs = S.snapg.p + (size_t)q * n3;

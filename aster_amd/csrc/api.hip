@@ -595,6 +595,21 @@ int aster_batch_energy(aster_ctx* c, double* e, double* L) {
   return batch_energy(c, e, L);
 }
 
+int aster_batch_info(aster_ctx* c, int64_t* pair_interactions, int64_t* lane_slots, int32_t* nwaves) {
+  if (!c) return ASTER_ERR_INVALID;
+  const BatchState& B = c->batch;
+  if (B.nsys == 0) return fail(c, ASTER_ERR_STATE, "no batch loaded");
+  int64_t pairs = 0;
+  for (int64_t s = 0; s < B.nsys; ++s) {
+    const int64_t k = B.offsets[s + 1] - B.offsets[s];
+    pairs += k * (k - 1);
+  }
+  if (pair_interactions) *pair_interactions = pairs;
+  if (lane_slots) *lane_slots = B.lane_slots;
+  if (nwaves) *nwaves = B.nwaves;
+  return ASTER_OK;
+}
+
 int aster_batch_autotune(aster_ctx* c, int integrator, double dt, int32_t* spb, int32_t* spl) {
   if (!c) return ASTER_ERR_INVALID;
   ASTER_HIP(c, hipSetDevice(c->device));

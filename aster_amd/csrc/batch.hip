@@ -600,6 +600,8 @@ static int upload_plan(aster_ctx* c) {
     used += sz;
   }
   B.nwaves = (int)wm.size();
+  B.lane_slots = 0;
+  for (int32_t mx : wm) B.lane_slots += 64 * (int64_t)mx;
   ASTER_HIP(c, B.lane_body.reserve(lb.size()));
   ASTER_HIP(c, B.lane_seg.reserve(ls.size()));
   ASTER_HIP(c, B.wave_max.reserve(wm.size()));

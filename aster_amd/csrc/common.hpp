@@ -55,6 +55,7 @@ struct BatchState {
   int waves_per_block = 4;
   int steps_per_launch = 0;          // 0: not tuned yet
   int nwaves = 0;
+  int64_t lane_slots = 0;            // sum over wavefronts of 64 x its largest system: lane-pair slots per evaluation
   DBuf<int32_t> lane_body, lane_seg, wave_max;
   bool a_valid = false;
   // Captured launch sequence and the arguments it was captured with.

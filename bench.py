@@ -28,6 +28,17 @@ sys.path.insert(0, ROOT)
 
 METRIC = "pair interactions/s (all-pairs N=64K, BH N=1M); adjoint/forward step time ratio"
 FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
+FP64_PEAK_TFLOPS = 78.6   # MI355X published FP64 vector peak
+# FP32 operations (an FMA counts 2) per source-target pair of the fused
+# adjoint kernel k_adj_fused: separation 3, r^2 6, rsqrt 1, inverse powers 3,
+# m inv^3 1, acceleration 6, u = m_k w_j - m_j w_k 9, d.u 5, 3 (d.u) inv^5 2,
+# J^T w 12, w.d 5, mass gradient 2.
+ADJ_FLOP_PER_PAIR = 55
+# FP64 operations per ordered pair and force evaluation of the ensemble
+# kernel's fast mode with pairwise 1PN (what bench_batch runs): Newton 32
+# (separation 3, r^2 6, rsqrt estimate + two Newton steps 13, G m inv^3 4,
+# accumulation 6) + 1PN 87 (unit vector 3, get_pn1 78, accumulation 6).
+BATCH_FLOP_PER_PAIR = 119
 HBM_PEAK_GBPS = 8000.0    # MI355X published HBM3E bandwidth
 
 # Algorithmic HBM bytes per body of the bandwidth-bound phases of a BH step
@@ -71,11 +82,17 @@ def err_stats(e):
 
 
 def host_threads():
+    """Host threads of the CPU baselines: the process's CPU share, i.e.
+    OMP_NUM_THREADS when it is set (it is, on the GPU box), else every core
+    of the affinity mask."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
-    return max(1, min(16, n))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
 
 
 def cpu_model():
@@ -102,12 +119,14 @@ def bench_bh(A, ctx, steps, warmup, method, n=1 << 20, theta=0.5, dt=5e-4, refer
     params(A, ctx, G=1.0, softening=0.005 ** 2, theta=theta, precision=A.FP32)
     ctx.set_state(pos, vel, mass)
     errs = None
+    ref = None
     if reference:
         # accuracy of one evaluation against FP64 all-pairs (device path that
         # is bit-identical to the reference CPU path)
         log(f"[bench] BH n={n}: FP64 all-pairs reference")
         ref = ctx.forces(A.ALLPAIRS_FP64)
         acc = ctx.forces(method)
+        acc_headline = acc
         ctx.bh_stats()
         errs = err_stats(rel_err(acc, ref))
         log(f"[bench] BH force error vs FP64 all-pairs: {errs}")
@@ -166,7 +185,7 @@ def bench_bh(A, ctx, steps, warmup, method, n=1 << 20, theta=0.5, dt=5e-4, refer
     ctx.bh_stats()
     log(f"[bench] BH {steps} steps: wall {wall * 1e3:.2f} ms, events {ev_ms:.2f} ms, phases {phases}")
     return dict(wall_s=wall, event_ms=ev_ms, phases=phases, interactions=inter, visits=visits, errors=errs,
-                n=n, theta=theta, dt=dt)
+                n=n, theta=theta, dt=dt, ref=ref, acc=acc_headline if reference else None, state=(pos, vel, mass))
 
 
 def bench_scaling(A, ctx, method, sizes=(18, 20, 22, 24, 26)):
@@ -343,9 +362,23 @@ def bench_adjoint(A, ctx, n=65536, T=512, ckpt=64, dt=1e-3):
     t_bwd = time.perf_counter() - t0
     ratio = t_bwd / t_fwd
     log(f"[bench] adjoint n={n} T={T}: forward {t_fwd:.3f}s backward {t_bwd:.3f}s ratio {ratio:.3f}, loss {loss:.3e}")
+    # the fused kernel's own time, from HIP events over a short backward pass
+    ctx.set_state(pos, vel, mass)
+    ctx.traj_forward(dt, ckpt, ckpt, np.array([ckpt]))
+    ctx.profile(True)
+    ctx.traj_backward(np.zeros((1, n, 3)))
+    ph = ctx.phase_times()
+    ctx.profile(False)
+    k_ms = ph["adjoint"][0] / ph["adjoint"][1]
+    achieved = ADJ_FLOP_PER_PAIR * float(n) * (n - 1) / (k_ms * 1e-3) / 1e12
+    log(f"[bench] k_adj_fused: {k_ms:.3f} ms per launch, {achieved:.1f} TFLOP/s")
     return {"n": n, "steps": T, "checkpoint_every": ckpt, "forward_s": t_fwd, "backward_s": t_bwd,
             "forward_ms_per_step": t_fwd / T * 1e3, "backward_ms_per_step": t_bwd / T * 1e3,
-            "ratio": ratio, "loss": loss, "grad_norms": [float(np.linalg.norm(g)) for g in (gx, gv, gm)]}
+            "ratio": ratio, "loss": loss, "grad_norms": [float(np.linalg.norm(g)) for g in (gx, gv, gm)],
+            "roofline": {"bound": "valu", "kernel": "k_adj_fused", "kernel_ms": k_ms, "achieved": achieved,
+                         "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                         "traffic": traffic_per_launch("k_adj_fused"),
+                         "flop_per_pair": ADJ_FLOP_PER_PAIR}}
 
 
 def bench_batch(A, ctx, nsys=8192, steps=10000, dt=1e-3):
@@ -368,10 +401,14 @@ def bench_batch(A, ctx, nsys=8192, steps=10000, dt=1e-3):
     de = np.abs(e1 - e0) / np.abs(e0)
     sizes = np.diff(off)
     pairs = float((sizes * (sizes - 1)).sum())
+    ipairs, lane_slots, nwaves = ctx.batch_info()
+    assert ipairs == pairs
     # two force evaluations per Yoshida substep with velocity-dependent forces
     inter = pairs * steps * 3 * 2
+    achieved = BATCH_FLOP_PER_PAIR * inter / wall / 1e12
     log(f"[bench] batch {nsys} systems x {steps} steps: {wall:.3f}s, {inter / wall:.3e} pair interactions/s, "
-        f"wpb {wpb} spl {spl}, |dE/E| median {np.median(de):.2e}")
+        f"{achieved:.1f} FP64 TFLOP/s, lane use {pairs / lane_slots:.3f}, wpb {wpb} spl {spl}, "
+        f"|dE/E| median {np.median(de):.2e}")
     # Newtonian comparison on the same ensemble: Wisdom-Holman (central mass =
     # body 0) against Yoshida-4 at the same coarse step (dt = 0.02, ~80 steps
     # per orbit of the innermost planets, a = 0.4), 500 steps each, where the
@@ -422,6 +459,12 @@ def bench_batch(A, ctx, nsys=8192, steps=10000, dt=1e-3):
     log(f"[bench] ensemble EIH vs pairwise 1PN, exact mode, {ne} steps: {eih}")
     return {"systems": nsys, "bodies": int(off[-1]), "steps": steps, "wall_s": wall,
             "system_steps_per_s": nsys * steps / wall, "pair_interactions_per_s": inter / wall,
+            "roofline": {"bound": "valu", "kernel": "k_batch (fast FP64 mode, 1PN)", "achieved": achieved,
+                         "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                         "traffic": None, "flop_per_pair": BATCH_FLOP_PER_PAIR,
+                         "note": "time = wall time of the graph replays (back to back on the device)"},
+            "lane_use": {"pair_slots_used": pairs, "lane_pair_slots": lane_slots, "wavefronts": nwaves,
+                         "frac": pairs / lane_slots},
             "waves_per_block": wpb, "steps_per_launch": spl,
             "energy_drift": {"median": float(np.median(de)), "p90": float(np.quantile(de, 0.9)),
                              "max": float(de.max())},
@@ -453,17 +496,90 @@ def bench_ic(A, n=1 << 24, host_n=1 << 20, reps=3):
                     "and barycentre sums, recentring); host: aster_ic_plummer, one thread"}
 
 
-def cpu_baseline_bh(orc, A, threads, n=1 << 20, theta=0.5, stride=1):
-    pos, vel, mass = A.ic_mixed(n, 20240)
+def gpu_parity_eval(A, ctx, theta, reps=3):
+    """The device's FP64 reference-MAC traversal (ASTER_BH_PARITY, bit-identical
+    to the reference's Barnes_Hut) on the loaded state: ms per evaluation
+    (tree build + traversal, HIP events) and interactions per evaluation."""
+    params(A, ctx, G=1.0, softening=0.005 ** 2, theta=theta, precision=A.FP64)
+    acc = ctx.forces(A.BH_PARITY)
+    ctx.sync()
+    ctx.bh_stats()
+    ctx.mark(2)
+    for _ in range(reps):
+        ctx.forces(A.BH_PARITY, out=False)
+    ctx.mark(3)
+    ms = ctx.elapsed(2, 3) / reps
+    inter, _ = ctx.bh_stats()
+    return acc, ms, inter / reps
+
+
+def cpu_baselines_bh(orc, A, ctx, threads, bh, target_s=12.0):
+    """CPU baselines of the headline Barnes-Hut evaluation, on its state
+    (2^20 bodies, theta = 0.5), on this host's cores:
+    1. equal work: the oracle's restatement of the group traversal (the same
+       accepted lists as the device, FP64 sums) over every body, tree build
+       included; interactions/s in the headline's unit;
+    2. the reference's Barnes_Hut (make_tree + get_node_body, its MAC) at the
+       same theta, restated, beside the device's bit-identical FP64 parity
+       traversal on the same input (same algorithm, precision and count);
+    3. the reference's Barnes_Hut at the theta where its median force error
+       reaches the headline's (searched with the device's bit-identical FP64
+       parity path), on a bounded sample of bodies: bodies/s at equal
+       accuracy."""
+    pos, vel, mass = bh["state"]
+    n, theta, ref = bh["n"], bh["theta"], bh["ref"]
     box = orc.box_auto(pos)
-    tb, tt, inter = orc.bh_sample(pos, vel, mass, box, theta, stride, soft=0.005 ** 2, threads=threads)
-    value = inter / (tt + tb / stride)
-    log(f"[bench] CPU BH sample: build {tb:.2f}s traverse {tt:.2f}s inter {inter} -> {value:.3e}/s")
-    return {"value": value, "unit": "interactions/s", "cores": threads, "kind": "port",
-            "sample": f"Aster CPU Barnes_Hut restated (FP64, reference MAC theta={theta}): make_tree over all "
-                      f"{n} bodies ({tb:.2f} s) + traversal of every {stride}th body ({tt:.2f} s, {inter} "
-                      f"interactions); build time amortised per sampled body",
-            "cpu": cpu_model()}
+    params(A, ctx, G=1.0, softening=0.005 ** 2, theta=theta, precision=A.FP64)
+    ctx.set_state(pos, vel, mass)
+    t0 = time.perf_counter()
+    _, inter_g, _ = orc.bh_group(pos, mass, box, theta, soft=0.005 ** 2, threads=threads)
+    t_group = time.perf_counter() - t0
+    log(f"[bench] CPU group traversal (equal work): {t_group:.2f}s, {inter_g / t_group:.3e} interactions/s")
+    out = {"value": inter_g / t_group, "unit": "interactions/s", "cores": threads, "kind": "port",
+           "sample": f"equal work: the oracle's group traversal (the device's accepted lists, FP64 sums) over all "
+                     f"{n} bodies at theta={theta}, tree build included ({t_group:.2f} s, {inter_g} interactions)",
+           "bodies_per_s": n / t_group, "cpu": cpu_model()}
+    # 2. the reference's algorithm at the same theta, CPU and device
+    tb, tt, inter_r = orc.bh_sample(pos, vel, mass, box, theta, 1, soft=0.005 ** 2, threads=threads)
+    acc_p, ms_p, inter_p = gpu_parity_eval(A, ctx, theta)
+    e = rel_err(acc_p, ref)
+    same = {"theta": theta, "interactions_per_body": inter_r / n, "median_error": float(np.median(e)),
+            "cpu_s_per_eval": tb + tt, "cpu_build_s": tb, "cpu_interactions_per_s": inter_r / (tb + tt),
+            "cpu_bodies_per_s": n / (tb + tt), "gpu_ms_per_eval": ms_p,
+            "gpu_interactions_per_s": inter_p / (ms_p * 1e-3), "gpu_bodies_per_s": n / (ms_p * 1e-3),
+            "same_interaction_count": bool(inter_p == inter_r)}
+    log(f"[bench] reference Barnes_Hut theta={theta}: {same}")
+    # 3. equal accuracy: the reference MAC's theta for the headline's median error
+    target = float(np.median(rel_err(bh["acc"], ref)))
+    th, found = 1e-4, None
+    for _ in range(8):
+        acc_t, ms_t, inter_t = gpu_parity_eval(A, ctx, th, reps=1)
+        med = float(np.median(rel_err(acc_t, ref)))
+        log(f"[bench] reference MAC theta={th:.3g}: median error {med:.3e}, {inter_t / n:.0f} interactions per body, "
+            f"device {ms_t:.1f} ms")
+        if med <= target:
+            found = (th, med, ms_t, inter_t)
+            break
+        th *= 0.5
+    eq = None
+    if found:
+        th, med, ms_t, inter_t = found
+        probe = max(1, n // 2048)
+        _, tp, _ = orc.bh_sample(pos, vel, mass, box, th, probe, soft=0.005 ** 2, threads=threads)
+        per_body = tp / ((n + probe - 1) // probe)
+        stride = max(1, int(np.ceil(per_body * n / target_s)))
+        tb, tt, inter_s = orc.bh_sample(pos, vel, mass, box, th, stride, soft=0.005 ** 2, threads=threads)
+        sampled = (n + stride - 1) // stride
+        full_s = tb + tt * n / sampled
+        eq = {"theta": th, "median_error": med, "target_median_error": target, "interactions_per_body": inter_t / n,
+              "cpu_sampled_bodies": sampled, "cpu_build_s": tb, "cpu_traversal_s": tt,
+              "cpu_s_per_eval_extrapolated": full_s, "cpu_bodies_per_s": n / full_s,
+              "cpu_interactions_per_s": inter_s / tt, "gpu_ms_per_eval": ms_t, "gpu_bodies_per_s": n / (ms_t * 1e-3)}
+        log(f"[bench] reference Barnes_Hut at equal accuracy: {eq}")
+    out["reference_barnes_hut"] = {"same_theta": same, "equal_accuracy": eq}
+    # restore the headline parameters
+    params(A, ctx, G=1.0, softening=0.005 ** 2, theta=theta, precision=A.FP32)
+    return out
 
 
 def cpu_baseline_allpairs(orc, pos, vel, mass, threads, targets=16384):
@@ -503,6 +619,7 @@ def main():
         # One section, for profiling runs; prints that section's result only.
         if args.only == "bh":
             r = bench_bh(A, ctx, args.steps, args.warmup, method, reference=False)
+            r = {k: v for k, v in r.items() if k not in ("ref", "acc", "state")}
         elif args.only == "allpairs":
             r = bench_allpairs(A, ctx, max(args.steps, 10))[0]
         elif args.only == "adjoint":
@@ -541,7 +658,7 @@ def main():
     if not args.no_cpu:
         from oracle import oracle as orc
         th = host_threads()
-        cpu = cpu_baseline_bh(orc, A, th)
+        cpu = cpu_baselines_bh(orc, A, ctx, th, bh)
         ap_res["cpu_baseline"] = cpu_baseline_allpairs(orc, pos64k, vel64k, mass64k, th)
 
     phases = {k: {"ms_per_launch": v[0] / v[1], "launches_per_step": v[1] / args.steps}
@@ -554,6 +671,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_per_step,
+        "bodies_per_s": bh["n"] / (ms_per_step * 1e-3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

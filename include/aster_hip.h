@@ -275,7 +275,7 @@ enum aster_phase {
   ASTER_PHASE_ALLPAIRS = 6, /* all-pairs pair kernel */
   ASTER_PHASE_COMBINE = 7,  /* slice combine / 1PN add */
   ASTER_PHASE_KICKDRIFT = 8,/* kick/drift update */
-  ASTER_PHASE_ADJOINT = 9,  /* fused force + adjoint kernel */
+  ASTER_PHASE_ADJOINT = 9,  /* fused force + adjoint kernel (its sources: TILES, its slice sums: COMBINE) */
   ASTER_PHASE_TILES = 10,   /* anchored FP32 source tiles for all-pairs */
   ASTER_NUM_PHASES = 12
 };
@@ -314,6 +314,14 @@ int aster_batch_energy(aster_ctx* ctx, double* e, double* L);
  * the fastest. Writes the chosen wavefronts per block and steps per launch. */
 int aster_batch_autotune(aster_ctx* ctx, int integrator, double dt, int32_t* systems_per_block,
                          int32_t* steps_per_launch);
+
+/* Work of one force evaluation over the loaded batch, for throughput and
+ * lane-use figures (no reference counterpart): ordered pair interactions
+ * sum n (n - 1) over the systems, and lane-pair slots, the iterations the
+ * lanes run (each wavefront loops over its largest system, so a packed
+ * wavefront's smaller systems idle part of it), and the wavefront count of
+ * the current packing. Any pointer may be NULL. */
+int aster_batch_info(aster_ctx* ctx, int64_t* pair_interactions, int64_t* lane_slots, int32_t* nwaves);
 
 /* Trajectory adjoint of the loaded ensemble (no reference counterpart):
  * leapfrog with Newtonian FP64 forces (fast mode) for nsteps, keeping the
