@@ -123,6 +123,8 @@ SIGNATURES = {
     "aster_get_state": (ctypes.c_int, [_P, _D, _D]),
     "aster_set_state_device": (ctypes.c_int, [_P, _I64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "aster_batch_info": (ctypes.c_int, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(ctypes.c_int32)]),
+    "aster_forces_noalloc": (ctypes.c_int, [_I64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_void_p, ctypes.c_void_p]),
     "aster_get_state_device": (ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_void_p]),
     "aster_collect": (ctypes.c_int, [_P, ctypes.c_int, _I64, _D]),
     "aster_center_of_mass": (ctypes.c_int, [_P, _D]),
@@ -472,6 +474,17 @@ class Context:
         gm = np.empty(self.n)
         self._ck(lib().aster_traj_backward(self.h, _dp(g), _dp(gx), _dp(gv), _dp(gm)))
         return gx, gv, gm
+
+
+NOALLOC_MAX_BODIES = 65536  # ASTER_NOALLOC_MAX_BODIES
+
+
+def forces_noalloc(n, pos_ptr, mass_ptr, G, softening, acc_ptr, stream=None):
+    """aster_forces_noalloc: ASTER_ALLPAIRS_FP32_KAHAN accelerations of n
+    bodies on device arrays of the current device (AoS-d3 positions and
+    output, n masses), enqueued on `stream` (None = the default stream) with
+    no device allocation."""
+    _check(lib().aster_forces_noalloc(int(n), pos_ptr, mass_ptr, float(G), float(softening), acc_ptr, stream))
 
 
 def yoshida_weights():

@@ -16,8 +16,11 @@
 // src/physics/tool_chain_impl.cpp:120-137), one target per lane, sequential
 // sources, IEEE division and square root, no contraction: bit-identical to the
 // CPU path.
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <mutex>
+#include <string>
 
 #include "common.hpp"
 #include "device.hpp"
@@ -316,6 +319,16 @@ int default_split(int64_t npad) {
   return (int)S;
 }
 
+// Slices of a span of sources (a multiple of kApTile) for the FP32 kernels:
+// the requested count, at least 1024 sources each, rounded to whole tiles.
+static void slice_span(int span, int requested, int* S, int* slice) {
+  int s = requested;
+  if ((int64_t)s * 1024 > span) s = std::max(1, span / 1024);
+  const int64_t sl = round_up((span + s - 1) / s, kApTile);
+  *slice = (int)sl;
+  *S = (int)((span + sl - 1) / sl);
+}
+
 int choose_split(aster_ctx* c) {
   if (c->P.split > 0) return c->P.split;
   return default_split(c->npad);
@@ -397,10 +410,8 @@ int launch_allpairs(aster_ctx* c, int method) {
   const int jb = lower / kApTile * kApTile;
   const int je = std::max<int>(jb + kApTile, (int)round_up(upper, kApTile));
   const int span = je - jb;
-  int S = choose_split(c);
-  if ((int64_t)S * 1024 > span) S = std::max(1, span / 1024);
-  int64_t slice = round_up((span + S - 1) / S, kApTile);
-  S = (int)((span + slice - 1) / slice);
+  int S = 0, slice = 0;
+  slice_span(span, choose_split(c), &S, &slice);
   ASTER_HIP(c, c->apart.reserve((size_t)S * 3 * n));
   const float eps2 = sc.eps2;
   const bool mask = sc.mask;
@@ -443,3 +454,154 @@ int launch_allpairs(aster_ctx* c, int method) {
 }
 
 }  // namespace aster
+
+// ------------------------------------------------------ allocation-free ----
+//
+// aster_forces_noalloc: the ASTER_ALLPAIRS_FP32_KAHAN evaluation (same
+// kernels, split and mass scale as aster_forces) on caller-owned device
+// arrays, with no device allocation. Its working memory is the file-scope
+// array `scratch` below: each GPU the library runs on holds one copy until
+// the process ends, and every call on that GPU uses it, so calls that
+// overlap on one GPU race. Nothing in it needs clearing: every call writes
+// each word it reads first.
+namespace aster {
+
+constexpr int64_t kNaMax = ASTER_NOALLOC_MAX_BODIES;
+static_assert(kNaMax % kApBlock == 0, "the body limit must be a whole number of all-pairs blocks");
+// Bound on S n of default_split over n <= kNaMax (the largest is 587520, at
+// n = 65280 with S = 9).
+constexpr int64_t kNaPartMax = 589824;
+// Layout in doubles: SoA positions | double4 sources | slice partials |
+// mass-sum partials | the two scales.
+constexpr size_t kNaX = 0;
+constexpr size_t kNaSrc = kNaX + 3 * kNaMax;
+constexpr size_t kNaPart = kNaSrc + 4 * kNaMax;
+constexpr size_t kNaRed = kNaPart + 3 * kNaPartMax;
+constexpr size_t kNaScale = kNaRed + (size_t)kComQ * (kComBlocks + 1);
+constexpr size_t kNaTotal = kNaScale + 2;
+
+static __device__ __align__(256) double scratch[kNaTotal];
+
+// AoS-d3 positions -> SoA planes.
+__global__ __launch_bounds__(256) void k_na_planes(int n, const double* __restrict__ pos, double* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) x[(size_t)c * n + i] = pos[3 * (size_t)i + c];
+}
+
+// The mass scale of fp32_scales, formed on the device from the mass sums:
+// scale[0] = 2^-k, scale[1] = G 2^k. k beyond the FP32 range is clamped (the
+// result is then not finite; aster_forces reports an error instead).
+__global__ void k_na_scales(const double* __restrict__ sums, float eps2, int mask, double G,
+                            double* __restrict__ scale) {
+  if (threadIdx.x != 0) return;
+  int k = fp32_mass_shift(sums[0], sums[4], eps2, mask != 0);
+  if (k > kMaxMassShift) k = kMaxMassShift;
+  scale[0] = (double)ldexpf(1.0f, -k);
+  scale[1] = ldexp(G, k);
+}
+
+// k_src64 with the mass scale read from the device.
+__global__ __launch_bounds__(256) void k_na_src(int n, int npad, const double* __restrict__ scale,
+                                                const double* __restrict__ x, const double* __restrict__ m,
+                                                double4* __restrict__ src) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npad) return;
+  const float msc = (float)scale[0];
+  src[i] = i < n ? make_double4(x[i], x[n + i], x[2 * n + i], pack_mass((float)m[i] * msc))
+                 : make_double4(0.0, 0.0, 0.0, pack_mass(0.f));
+}
+
+// k_combine (no post-Newtonian term) writing AoS-d3.
+__global__ void k_na_combine(int n, int S, const double* __restrict__ part, const double* __restrict__ scale,
+                             double* __restrict__ acc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double G = scale[1];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double s = 0.0;
+    for (int k = 0; k < S; ++k) s += part[((size_t)k * 3 + c) * n + i];
+    acc[3 * (size_t)i + c] = G * s;
+  }
+}
+
+constexpr int kNaMaxDevices = 64;
+static std::mutex g_na_lock;
+static double* g_na_ptr[kNaMaxDevices] = {};
+
+// The device address of `scratch` on device `dev` (current), looked up once
+// per device. In host code the array's name is only a stand-in: kernels get
+// this address.
+static hipError_t na_scratch(int dev, double** out) {
+  std::lock_guard<std::mutex> guard(g_na_lock);
+  if (!g_na_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_na_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_na_ptr[dev];
+  return hipSuccess;
+}
+
+static int na_fail(int code, const std::string& msg) {
+  set_global_error(msg);
+  return code;
+}
+
+}  // namespace aster
+
+extern "C" int aster_forces_noalloc(int64_t n, const double* d_pos, const double* d_mass, double G, double softening,
+                                    double* d_acc, void* stream) {
+  using namespace aster;
+  if (n < 1 || !d_pos || !d_mass || !d_acc)
+    return na_fail(ASTER_ERR_INVALID, "aster_forces_noalloc: need n >= 1 and device positions, masses, output");
+  if (n > kNaMax)
+    return na_fail(ASTER_ERR_CAPACITY, "aster_forces_noalloc: at most ASTER_NOALLOC_MAX_BODIES bodies");
+  if (!(softening >= 0.0) || !std::isfinite(G)) return na_fail(ASTER_ERR_INVALID, "invalid G or softening");
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return na_fail(ASTER_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kNaMaxDevices) return na_fail(ASTER_ERR_CAPACITY, "device index beyond 63");
+  double* base = nullptr;
+  if ((e = na_scratch(dev, &base)) != hipSuccess)
+    return na_fail(ASTER_ERR_HIP, std::string("hipGetSymbolAddress: ") + hipGetErrorString(e));
+  const int ni = (int)n;
+  const int npad = (int)round_up(n, kApBlock);
+  // the slicing of launch_allpairs over the full source range
+  const int span = (int)round_up(n, kApTile);
+  int S = 0, slice = 0;
+  slice_span(span, default_split(npad), &S, &slice);
+  if ((int64_t)S * n > kNaPartMax) return na_fail(ASTER_ERR_CAPACITY, "slice partials beyond the scratch layout");
+  double* x = base + kNaX;
+  double4* src = reinterpret_cast<double4*>(base + kNaSrc);
+  double* part = base + kNaPart;
+  double* red = base + kNaRed;
+  double* sums = red + (size_t)kComQ * kComBlocks;
+  double* scale = base + kNaScale;
+  const float e2 = (float)softening;
+  const bool mask = !(e2 >= FLT_MIN);
+  const float eps2 = mask ? 0.f : e2;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 b256(256), gn((ni + 255) / 256);
+  const int rblocks = (int)std::min<int64_t>((n + 255) / 256, kComBlocks);
+  hipLaunchKernelGGL(k_na_planes, gn, b256, 0, st, ni, d_pos, x);
+  hipLaunchKernelGGL(k_com_partial, dim3(rblocks), b256, 0, st, n, (const double*)x, d_mass, red);
+  hipLaunchKernelGGL(k_com_final, dim3(1), dim3(64), 0, st, rblocks, (const double*)red, sums);
+  hipLaunchKernelGGL(k_na_scales, dim3(1), dim3(64), 0, st, (const double*)sums, eps2, mask ? 1 : 0, G, scale);
+  hipLaunchKernelGGL(k_na_src, dim3(npad / 256), b256, 0, st, ni, npad, (const double*)scale, (const double*)x,
+                     d_mass, src);
+  const dim3 grid(npad / kApBlock, S);
+  if (mask)
+    hipLaunchKernelGGL((k_allpairs_f32<ACC_TILE_KAHAN, true, true>), grid, dim3(kApBlock), 0, st,
+                       (const double4*)src, (const double*)x, ni, span, slice, eps2, part);
+  else
+    hipLaunchKernelGGL((k_allpairs_f32<ACC_TILE_KAHAN, false, true>), grid, dim3(kApBlock), 0, st,
+                       (const double4*)src, (const double*)x, ni, span, slice, eps2, part);
+  hipLaunchKernelGGL(k_na_combine, gn, b256, 0, st, ni, S, (const double*)part, (const double*)scale, d_acc);
+  if ((e = hipGetLastError()) != hipSuccess)
+    return na_fail(ASTER_ERR_HIP, std::string("aster_forces_noalloc launch: ") + hipGetErrorString(e));
+  return ASTER_OK;
+}

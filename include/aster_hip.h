@@ -150,6 +150,24 @@ int aster_forces(aster_ctx* ctx, int method, double* acc);
 /* Copy the last accelerations to the host. */
 int aster_get_acc(aster_ctx* ctx, double* acc);
 
+/* Allocation-free all-pairs. No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc). The ASTER_ALLPAIRS_FP32_KAHAN
+ * evaluation of aster_forces (the same kernels, source split and FP32 mass
+ * scale, so the same bits as aster_forces with those G and softening, full
+ * range, Newtonian) on caller-owned device arrays of the current device:
+ * d_pos and d_acc AoS-d3 (3n doubles), d_mass n doubles. It allocates
+ * nothing: its working memory is a static device array of about 17 MiB per
+ * GPU, held until the process ends, shared by every call on that GPU, so
+ * calls that overlap on one GPU race. Work is enqueued on `stream` (a
+ * hipStream_t; NULL = the default stream) and the call returns without
+ * waiting; inputs are not checked on the device (aster_set_state_device
+ * does that). n above ASTER_NOALLOC_MAX_BODIES is ASTER_ERR_CAPACITY.
+ * Errors read through aster_last_error(NULL). */
+#define ASTER_NOALLOC_MAX_BODIES 65536
+int aster_forces_noalloc(int64_t n, const double* d_pos, const double* d_mass, double G, double softening,
+                         double* d_acc, void* stream);
+
 /* Advance nsteps with `integrator` and force `method` entirely on the device
  * (update_leapfrog, tool_chain_impl.cpp:169-185; Yoshida-4 is the composition
  * of three such steps with w1 dt, w0 dt, w1 dt). Newtonian accelerations
