@@ -366,8 +366,12 @@ int aster_step(aster_ctx* c, int method, int integrator, double dt, int64_t nste
         if ((rc = compute_forces(c, method))) return rc;
         pending = 0.0;
       }
-      if ((rc = launch_kick_drift(c, pending, h))) return rc;
-      if ((rc = compute_forces(c, method))) return rc;
+      // Barnes-Hut: the drift also forms the first stage of the next tree's
+      // automatic box, consumed by that force evaluation alone
+      if ((rc = launch_kick_drift(c, pending, h, is_bh(method)))) return rc;
+      rc = compute_forces(c, method);
+      c->box_parts = 0;
+      if (rc) return rc;
       pending = h;
     }
   if ((rc = launch_kick(c, pending))) return rc;

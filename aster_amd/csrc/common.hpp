@@ -115,6 +115,11 @@ struct aster_ctx {
   aster::DBuf<double> eout;     // 8 doubles
   aster::DBuf<uint32_t> flags;  // device error word(s), 4 words
   aster::DBuf<double> boxpart;  // per-block |x| maxima for the auto box [blocks][3]
+  // boxpart holds box_parts partials of the positions a fused kick/drift just
+  // wrote (state buffer box_parts_x, box_parts_n bodies); 0 = none pending.
+  int box_parts = 0;
+  const double* box_parts_x = nullptr;
+  int64_t box_parts_n = 0;
   aster::DBuf<double> stage;    // AoS-d3 staging for host transfers (7 n)
   double e0 = 0.0;              // first Hamiltonian after a state load (error collector)
   bool e0_valid = false;
@@ -226,7 +231,7 @@ __host__ __device__ inline int fp32_mass_shift(double mtot, double extent, float
 constexpr int kMaxMassShift = 126;  // 2^-k must stay a normal FP32 number
 // Launchers (each returns an aster_status).
 int launch_allpairs(aster_ctx* c, int method);           // writes c->a
-int launch_kick_drift(aster_ctx* c, double dt_prev, double dt);
+int launch_kick_drift(aster_ctx* c, double dt_prev, double dt, bool box = false);
 int launch_kick(aster_ctx* c, double dt);
 int launch_saba_stage(aster_ctx* c, bool kick, double d, double cdrift, double dt);
 int launch_energy(aster_ctx* c, double* out6_host);

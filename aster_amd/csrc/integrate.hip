@@ -8,30 +8,55 @@
 // trajectory is bit-identical to the CPU path. The second half-kick of a step
 // and the first half-kick of the next use the same accelerations, so they are
 // applied in one pass (k_kick_drift with dt_prev != 0).
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace aster {
 
+// BOX: the block also writes the largest |coordinate| of its drifted bodies
+// per axis to part[3 block + axis], the first stage of the automatic Morton
+// box of the next force evaluation (k_box's partials for these positions).
+template <bool BOX>
 __global__ __launch_bounds__(256) void k_kick_drift(int n, double* __restrict__ x, double* __restrict__ v,
                                                     const double* __restrict__ a, double dt_prev, double dt,
-                                                    uint32_t* __restrict__ flags) {
+                                                    uint32_t* __restrict__ flags, double* __restrict__ part) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  bool finite = true;
+  if (!BOX && i >= n) return;
+  double mx[3] = {0.0, 0.0, 0.0};
+  if (i < n) {
+    bool finite = true;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const size_t k = (size_t)c * n + i;
-    const double ac = a[k];
-    double vc = v[k];
-    if (dt_prev != 0.0) vc = vc + 0.5 * ac * dt_prev;
-    const double xc = x[k] + (vc * dt + 0.5 * ac * dt * dt);
-    vc = vc + 0.5 * ac * dt;
-    x[k] = xc;
-    v[k] = vc;
-    finite = finite && isfinite(xc) && isfinite(vc);
+    for (int c = 0; c < 3; ++c) {
+      const size_t k = (size_t)c * n + i;
+      const double ac = a[k];
+      double vc = v[k];
+      if (dt_prev != 0.0) vc = vc + 0.5 * ac * dt_prev;
+      const double xc = x[k] + (vc * dt + 0.5 * ac * dt * dt);
+      vc = vc + 0.5 * ac * dt;
+      x[k] = xc;
+      v[k] = vc;
+      mx[c] = fabs(xc);
+      finite = finite && isfinite(xc) && isfinite(vc);
+    }
+    if (!finite) atomicOr(flags, (uint32_t)FLAG_NONFINITE);
   }
-  if (!finite) atomicOr(flags, (uint32_t)FLAG_NONFINITE);
+  if (BOX) {
+    __shared__ double wm[3][4];
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double r = mx[c];
+      for (int off = 32; off > 0; off >>= 1) r = fmax(r, __shfl_xor(r, off, 64));
+      if ((threadIdx.x & 63) == 0) wm[c][w] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+      const int c = threadIdx.x;
+      part[3 * blockIdx.x + c] = fmax(fmax(wm[c][0], wm[c][1]), fmax(wm[c][2], wm[c][3]));
+    }
+  }
 }
 
 // One SABA stage after its force evaluation, as the per-body lambdas of
@@ -194,13 +219,26 @@ int launch_reset_flags(aster_ctx* c) {
   return ASTER_OK;
 }
 
-int launch_kick_drift(aster_ctx* c, double dt_prev, double dt) {
+int launch_kick_drift(aster_ctx* c, double dt_prev, double dt, bool box) {
   const int n = (int)c->n;
+  const int blocks = (n + 255) / 256;
+  c->box_parts = 0;
+  // (at least what launch_box reserves, so that it never reallocates the partials)
+  if (box) ASTER_HIP(c, c->boxpart.reserve((size_t)3 * std::max(blocks, kBoxBlocks)));
   phase_begin(c, ASTER_PHASE_KICKDRIFT);
-  hipLaunchKernelGGL(k_kick_drift, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->x.p, c->v.p, c->a.p,
-                     dt_prev, dt, c->flags.p);
+  if (box)
+    hipLaunchKernelGGL(k_kick_drift<true>, dim3(blocks), dim3(256), 0, c->stream, n, c->x.p, c->v.p, c->a.p, dt_prev,
+                       dt, c->flags.p, c->boxpart.p);
+  else
+    hipLaunchKernelGGL(k_kick_drift<false>, dim3(blocks), dim3(256), 0, c->stream, n, c->x.p, c->v.p, c->a.p, dt_prev,
+                       dt, c->flags.p, nullptr);
   ASTER_HIP(c, hipGetLastError());
   phase_end(c, ASTER_PHASE_KICKDRIFT);
+  if (box) {
+    c->box_parts = blocks;
+    c->box_parts_x = c->x.p;
+    c->box_parts_n = c->n;
+  }
   return ASTER_OK;
 }
 
@@ -242,12 +280,18 @@ int launch_box(aster_ctx* c, bool user_box) {
   // A user box was uploaded to boxd[4..6] by aster_set_params.
   const bool user = user_box && (c->P.box[0] != 0.0 || c->P.box[1] != 0.0 || c->P.box[2] != 0.0);
   int blocks = 0;
-  if (!user) {
+  // Partials of these very positions left by the drift just before (same
+  // state buffer and body count): only their final maximum remains.
+  const bool fused = c->box_parts > 0 && c->box_parts_x == c->x.p && c->box_parts_n == c->n;
+  if (!user && fused) {
+    blocks = c->box_parts;
+  } else if (!user) {
     blocks = (n + 255) / 256;
     if (blocks > kBoxBlocks) blocks = kBoxBlocks;
     hipLaunchKernelGGL(k_box, dim3(blocks), dim3(256), 0, c->stream, n, c->x.p, c->boxpart.p);
     ASTER_HIP(c, hipGetLastError());
   }
+  c->box_parts = 0;
   hipLaunchKernelGGL(k_box_finish, dim3(1), dim3(kBoxFinishThreads), 0, c->stream, c->boxpart.p, blocks, c->boxd.p + 4,
                      user ? 1 : 0, c->boxd.p);
   ASTER_HIP(c, hipGetLastError());

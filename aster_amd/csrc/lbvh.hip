@@ -80,20 +80,41 @@ __global__ __launch_bounds__(256) void k_morton(int n, const double* __restrict_
 
 // -------------------------------------------------------- radix sort ------
 
-__global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint32_t* __restrict__ keys, int n, int shift,
-                                                        uint32_t* __restrict__ hist, int nblocks) {
-  __shared__ uint32_t cnt[256];
+// LSD radix sort of (Morton code, body) pairs, 4 passes of 8 bits. A block
+// handles a tile of kRsPer * THREADS keys. Per pass, k_rs_hist writes each
+// tile's digit counts (hist[d * stride + b]); the scatter needs every digit's
+// start, i.e. the keys of smaller digits in all tiles plus those of the same
+// digit in earlier tiles. With at most kRsRawMaxBlocks tiles the scatter
+// forms these starts itself from the raw counts (two launches per pass); with
+// more, an exclusive scan of the count matrix runs between the two kernels.
+constexpr int kRsPer = 8;
+constexpr int kRsRawMaxBlocks = 128;
+
+__host__ __device__ constexpr int rs_stride(int nblocks) { return (nblocks + 3) & ~3; }
+
+template <int THREADS>
+__global__ __launch_bounds__(THREADS) void k_rs_hist(const uint32_t* __restrict__ keys, int n, int shift,
+                                                     uint32_t* __restrict__ hist, int stride) {
+  constexpr int TILE = kRsPer * THREADS, H = THREADS / 256;  // H copies of the counters
+  static_assert(THREADS % 256 == 0, "one counter per thread and copy");
+  __shared__ uint32_t cnt[H][256];
   const int t = threadIdx.x;
-  cnt[t] = 0;
+  cnt[t >> 8][t & 255] = 0;
   __syncthreads();
-  const int base = blockIdx.x * kRsTile;
+  uint32_t* mine = cnt[(t >> 6) % H];  // wavefront w counts into copy w mod H
+  const int base = blockIdx.x * TILE;
 #pragma unroll
-  for (int r = 0; r < kRsTile / kRsThreads; ++r) {
-    const int p = base + r * kRsThreads + t;
-    if (p < n) atomicAdd(&cnt[(keys[p] >> shift) & 255u], 1u);
+  for (int r = 0; r < kRsPer; ++r) {
+    const int p = base + r * THREADS + t;
+    if (p < n) atomicAdd(&mine[(keys[p] >> shift) & 255u], 1u);
   }
   __syncthreads();
-  hist[(size_t)t * nblocks + blockIdx.x] = cnt[t];
+  if (t < 256) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int h = 0; h < H; ++h) s += cnt[h][t];
+    hist[(size_t)t * stride + blockIdx.x] = s;
+  }
 }
 
 // Exclusive prefix sum of a u32 array, in place, in two passes over chunks of
@@ -171,36 +192,86 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(uint32_t* __restric
   }
 }
 
-// Stable scatter. Each wavefront owns a contiguous quarter of the tile (512
-// keys, in 8 rounds of 64) and ranks its keys among equal digits with its own
-// running counters in LDS: rank = the wavefront's earlier keys of the digit
+// Stable scatter. Each wavefront owns 512 consecutive keys of the tile (8
+// rounds of 64) and ranks its keys among equal digits with its own running
+// counters in LDS: rank = the wavefront's earlier keys of the digit
 // (wcnt[w][d]) + earlier lanes of the round (ballot match mask). LDS
 // operations of one wavefront execute in order, so the rounds need no
 // workgroup barrier. Then the tile's digit totals give each digit's start in
 // the tile (lpre) and each wavefront's offset inside it; the tile is
 // reordered by digit in LDS and written out with consecutive lanes on
 // consecutive addresses within each digit's run.
-__global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t* __restrict__ kin,
-                                                           const uint32_t* __restrict__ vin,
-                                                           uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                           int n, int shift, const uint32_t* __restrict__ hist,
-                                                           int nblocks) {
-  constexpr int W = kRsThreads / 64;   // wavefronts per tile
-  constexpr int C = kRsTile / W;       // keys per wavefront
-  constexpr int R = C / 64;            // rounds per wavefront
+// RAW: hist holds the raw tile counts and the block forms its digit starts:
+// Q = THREADS / 256 threads per digit sum the digit's row (16-byte loads),
+// then a scan over the 256 digit totals. Otherwise hist holds the starts
+// (the exclusive scan of the count matrix).
+template <int THREADS, bool RAW>
+__global__ __launch_bounds__(THREADS) void k_rs_scatter(const uint32_t* __restrict__ kin,
+                                                        const uint32_t* __restrict__ vin,
+                                                        uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                        int n, int shift, const uint32_t* __restrict__ hist,
+                                                        int nblocks, int stride) {
+  constexpr int W = THREADS / 64;       // wavefronts per tile
+  constexpr int TILE = kRsPer * THREADS;
+  constexpr int C = TILE / W;           // keys per wavefront
+  constexpr int R = C / 64;             // rounds per wavefront
   __shared__ uint32_t gbase[256];
   __shared__ uint32_t wcnt[W][256];
-  __shared__ uint32_t wsum[W];
+  __shared__ uint32_t wsum[4];
   __shared__ uint32_t lpre[256];
-  __shared__ uint32_t skey[kRsTile], sval[kRsTile];
+  __shared__ uint32_t skey[TILE], sval[TILE];
   const int t = threadIdx.x;
   const int lane = t & 63, w = t >> 6;
-  gbase[t] = hist[(size_t)t * nblocks + blockIdx.x];
+  const int blk = (int)blockIdx.x;
+  if constexpr (RAW) {
+    constexpr int Q = THREADS / 256;
+    const int d = t / Q, q = t - d * Q;
+    const uint4* __restrict__ row = reinterpret_cast<const uint4*>(hist + (size_t)d * stride);
+    uint32_t pre = 0, tot = 0;
+    for (int k = q; 4 * k < nblocks; k += Q) {
+      const uint4 v = row[k];
+      const uint32_t e[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-  for (int q = 0; q < W; ++q) wcnt[q][t] = 0;
+      for (int u = 0; u < 4; ++u) {
+        const int b = 4 * k + u;
+        const uint32_t c = b < nblocks ? e[u] : 0u;
+        tot += c;
+        pre += b < blk ? c : 0u;
+      }
+    }
+#pragma unroll
+    for (int off = 1; off < Q; off <<= 1) {  // the Q threads of a digit are adjacent lanes
+      pre += __shfl_xor(pre, off, 64);
+      tot += __shfl_xor(tot, off, 64);
+    }
+    if (q == 0) {
+      gbase[d] = pre;
+      lpre[d] = tot;
+    }
+    __syncthreads();
+    uint32_t v = 0, inc = 0;
+    if (t < 256) {
+      v = lpre[t];
+      inc = v;
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += y;
+      }
+      if (lane == 63) wsum[w] = inc;
+    }
+    __syncthreads();
+    if (t < 256) {
+      uint32_t p = inc - v;
+      for (int q2 = 0; q2 < w; ++q2) p += wsum[q2];
+      gbase[t] += p;
+    }
+  } else {
+    if (t < 256) gbase[t] = hist[(size_t)t * stride + blk];
+  }
+  for (int i = t; i < W * 256; i += THREADS) (&wcnt[0][0])[i] = 0;
   __syncthreads();
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int base = blockIdx.x * kRsTile + w * C;
+  const int base = blk * TILE + w * C;
   uint32_t key[R], val[R], pos[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -226,21 +297,23 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t* __res
   __syncthreads();
   // digit t: its total in the tile, each wavefront's offset inside it, and
   // the exclusive prefix of the totals over the digits
-  {
-    uint32_t c[W], tot = 0;
+  uint32_t tot = 0, inc = 0;
+  if (t < 256) {
 #pragma unroll
     for (int q = 0; q < W; ++q) {
-      c[q] = wcnt[q][t];
+      const uint32_t c = wcnt[q][t];
       wcnt[q][t] = tot;
-      tot += c[q];
+      tot += c;
     }
-    uint32_t inc = tot;
+    inc = tot;
     for (int off = 1; off < 64; off <<= 1) {
       const uint32_t y = __shfl_up(inc, off, 64);
       if (lane >= off) inc += y;
     }
     if (lane == 63) wsum[w] = inc;
-    __syncthreads();
+  }
+  __syncthreads();
+  if (t < 256) {
     uint32_t pre = inc - tot;
     for (int q = 0; q < w; ++q) pre += wsum[q];
     lpre[t] = pre;
@@ -256,8 +329,8 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint32_t* __res
     }
   }
   __syncthreads();
-  const int valid_here = min(kRsTile, n - (int)blockIdx.x * kRsTile);
-  for (int j = t; j < valid_here; j += kRsThreads) {
+  const int valid_here = min(TILE, n - blk * TILE);
+  for (int j = t; j < valid_here; j += THREADS) {
     const uint32_t k = skey[j];
     const uint32_t d = (k >> shift) & 255u;
     const uint32_t g = gbase[d] + ((uint32_t)j - lpre[d]);
@@ -1414,13 +1487,13 @@ __global__ void k_zero_counters(unsigned long long* p) {
 
 static int reserve_sort(aster_ctx* c) {
   const size_t n = (size_t)c->n;
-  const size_t nblocks = (n + kRsTile - 1) / kRsTile;
+  const size_t nblocks = (n + kRsTile - 1) / kRsTile;  // the most tiles any pass layout uses
   ASTER_HIP(c, c->codes.reserve(n));
   ASTER_HIP(c, c->idx.reserve(n));
   ASTER_HIP(c, c->codes2.reserve(n));
   ASTER_HIP(c, c->idx2.reserve(n));
   ASTER_HIP(c, c->xm4.reserve(n));
-  ASTER_HIP(c, c->hist.reserve(256 * nblocks));
+  ASTER_HIP(c, c->hist.reserve(256 * (size_t)rs_stride((int)nblocks)));
   ASTER_HIP(c, c->scanpart.reserve((n + kScanChunk - 1) / kScanChunk + 1));
   return ASTER_OK;
 }
@@ -1496,14 +1569,44 @@ int morton_sort(aster_ctx* c, bool user_box) {
   if (rc) return rc;
   if ((rc = morton_codes(c, user_box))) return rc;
   const int n = (int)c->n;
-  const int nblocks = (n + kRsTile - 1) / kRsTile;
   uint32_t *kin = c->codes.p, *vin = c->idx.p, *kout = c->codes2.p, *vout = c->idx2.p;
+  // The smaller tile (4096 or 8192 keys) that needs at most kRsRawMaxBlocks
+  // tiles; beyond 2^20 keys, 2048-key tiles and the scan. (2048-key tiles
+  // with the offsets formed in the scatter were slower at 2^18 keys: each
+  // block then sums twice the count matrix with half the threads.)
+  int threads = 0;
+  for (int th = 512; th <= 1024 && !threads; th *= 2)
+    if ((n + kRsPer * th - 1) / (kRsPer * th) <= kRsRawMaxBlocks) threads = th;
+  const bool raw = threads != 0;
+  if (!raw) threads = kRsThreads;
+  const int tile = kRsPer * threads;
+  const int nblocks = (n + tile - 1) / tile;
+  const int stride = raw ? rs_stride(nblocks) : nblocks;  // the scan runs over 256 * nblocks contiguous counts
   phase_begin(c, ASTER_PHASE_SORT);
   for (int shift = 0; shift < 32; shift += 8) {
-    hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(kRsThreads), 0, c->stream, kin, n, shift, c->hist.p, nblocks);
-    if ((rc = device_scan(c, c->hist.p, 256 * nblocks, nullptr))) return rc;
-    hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(kRsThreads), 0, c->stream, kin, vin, kout, vout, n, shift,
-                       c->hist.p, nblocks);
+    const dim3 grid(nblocks), block(threads);
+    switch (threads) {
+      case 256:
+        hipLaunchKernelGGL(k_rs_hist<256>, grid, block, 0, c->stream, kin, n, shift, c->hist.p, stride);
+        break;
+      case 512:
+        hipLaunchKernelGGL(k_rs_hist<512>, grid, block, 0, c->stream, kin, n, shift, c->hist.p, stride);
+        break;
+      default:
+        hipLaunchKernelGGL(k_rs_hist<1024>, grid, block, 0, c->stream, kin, n, shift, c->hist.p, stride);
+        break;
+    }
+    if (!raw) {
+      if ((rc = device_scan(c, c->hist.p, 256 * nblocks, nullptr))) return rc;
+      hipLaunchKernelGGL((k_rs_scatter<kRsThreads, false>), grid, block, 0, c->stream, kin, vin, kout, vout, n, shift,
+                         c->hist.p, nblocks, stride);
+    } else if (threads == 512) {
+      hipLaunchKernelGGL((k_rs_scatter<512, true>), grid, block, 0, c->stream, kin, vin, kout, vout, n, shift,
+                         c->hist.p, nblocks, stride);
+    } else {
+      hipLaunchKernelGGL((k_rs_scatter<1024, true>), grid, block, 0, c->stream, kin, vin, kout, vout, n, shift,
+                         c->hist.p, nblocks, stride);
+    }
     std::swap(kin, kout);
     std::swap(vin, vout);
   }

@@ -120,6 +120,7 @@ int import_state(aster_ctx* c, int64_t n, const double* d_pos, const double* d_v
   c->adj.ready = false;
   c->e0_valid = false;
   c->mass_total_valid = false;
+  c->box_parts = 0;
   return ASTER_OK;
 }
 
@@ -168,6 +169,7 @@ int grow_state(aster_ctx* c, int64_t add, int64_t* base) {
   c->adj.ready = false;
   c->e0_valid = false;
   c->mass_total_valid = false;
+  c->box_parts = 0;
   *base = n0;
   return ASTER_OK;
 }

@@ -72,6 +72,25 @@ def test_tree_bit_identical(ctx, orc, n, dup):
     assert np.array_equal(T["com"], R["com"])
 
 
+@pytest.mark.parametrize("n,dup", [(5000, True), (1 << 18, False), (300001, True), (1 << 20, True),
+                                   ((1 << 20) + 1, False), (3 << 20, True)])
+def test_sort_order(ctx, n, dup):
+    """The (code, body index) order of the radix sort against numpy, on each
+    tile layout: 4096- and 8192-key tiles with the offsets formed in the
+    scatter (up to 2^20 keys), 2048-key tiles with the scanned offsets
+    (above)."""
+    pos, vel, mass = mixed(n, seed=n + 7)
+    if dup:
+        pos[: n // 3] = pos[0] + 1e-9 * np.arange(n // 3)[:, None]
+    load(ctx, pos, vel, mass, precision=A.FP32, box=(0.0, 0.0, 0.0))
+    codes, _ = ctx.morton()
+    ctx.forces(A.BH_GROUP, out=False)
+    T = ctx.tree()
+    ref = np.lexsort((np.arange(n), codes))
+    assert np.array_equal(T["perm"], ref)
+    assert np.array_equal(T["codes"], codes[ref])
+
+
 @pytest.mark.parametrize("theta", [0.3, 0.5, 0.7])
 def test_bh_fp64_parity_bit_identical(ctx, orc, theta):
     pos, vel, mass = mixed(4096, seed=3)
@@ -242,6 +261,26 @@ def test_bh_leapfrog_steps(ctx, orc):
     rp, rv, _ = orc.integrate(pos, vel, mass, 5e-4, 3, integrator=2, method=1, soft=2.5e-5, box=np.array(box),
                               theta=0.5)
     assert np.array_equal(p, rp) and np.array_equal(v, rv)
+
+
+@pytest.mark.parametrize("integrator", [A.LEAPFROG, A.YOSHIDA4])
+def test_bh_steps_automatic_box(ctx, orc, integrator):
+    # With the automatic box each force evaluation takes the box of the
+    # positions the drift just wrote (the drift forms its first stage):
+    # bit-identical to the CPU path, which recomputes the box per evaluation.
+    pos, vel, mass = mixed(3000, seed=9)
+    load(ctx, pos, vel, mass, theta=0.5, precision=A.FP64, box=(0.0, 0.0, 0.0))
+    ctx.step(A.BH_PARITY, integrator, 5e-4, 4)
+    p, v = ctx.get_state()
+    rp, rv, _ = orc.integrate(pos, vel, mass, 5e-4, 4, integrator=2 if integrator == A.LEAPFROG else 5, method=1,
+                              soft=2.5e-5, theta=0.5)
+    assert np.array_equal(p, rp) and np.array_equal(v, rv)
+    # a state loaded afterwards gets its own box, not the drift's
+    pos2 = pos * 3.0
+    load(ctx, pos2, vel, mass, theta=0.5, precision=A.FP64, box=(0.0, 0.0, 0.0))
+    _, box = ctx.morton()
+    assert np.array_equal(box, orc.box_auto(pos2))
+    assert np.array_equal(ctx.forces(A.BH_PARITY), orc.bh(pos2, vel, mass, orc.box_auto(pos2), 0.5, soft=2.5e-5))
 
 
 def test_bh_group_duplicates_and_unsoftened(ctx):
