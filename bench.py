@@ -27,7 +27,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "pair interactions/s (all-pairs N=64K, BH N=1M); adjoint/forward step time ratio"
-FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
+FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak (packed FP32 FMAs)
+# One wave64 FP32 FMA per SIMD quad-cycle (1024 SIMDs, 2.4 GHz): the ceiling
+# without packed FP32 instructions, which this build does not emit.
+FP32_NONPACKED_TFLOPS = 78.6
 FP64_PEAK_TFLOPS = 78.6   # MI355X published FP64 vector peak
 # FP32 operations (an FMA counts 2) per source-target pair of the fused
 # adjoint kernel k_adj_fused: separation 3, r^2 6, rsqrt 1, inverse powers 3,
@@ -43,15 +46,16 @@ HBM_PEAK_GBPS = 8000.0    # MI355X published HBM3E bandwidth
 
 # Algorithmic HBM bytes per body of the bandwidth-bound phases of a BH step
 # (what each phase must read and write once):
-#  morton: box pass 24 B (FP64 x) + Morton pass 24 B x + 8 B m read, 4 B code
-#          + 4 B index + 32 B {x, m} record written;
+#  morton: Morton pass 24 B x + 8 B m read, 4 B code + 4 B index + 32 B
+#          {x, m} record written (the box's maxima come from the drift that
+#          wrote x; the drift reads x anyway);
 #  sort: 4 passes x (4 B histogram read + 8 B key/value read + 8 B written);
 #  build: 4 B code + 4 B permutation + 32 B record read, 48 B of leaf records
 #         (FP64 {x, m}, FP32 sorted body) and 24 B of internal-node links
 #         (left, right, parent, range, split) written, plus the group pass's
 #         12 B (start flag, climb, root);
 #  kickdrift: FP64 x, v, a read (72 B), x, v written (48 B).
-PHASE_BYTES_PER_BODY = {"morton": 24 + 24 + 8 + 4 + 4 + 32, "sort": 4 * (4 + 8 + 8), "build": 4 + 4 + 32 + 48 + 24 + 12,
+PHASE_BYTES_PER_BODY = {"morton": 24 + 8 + 4 + 4 + 32, "sort": 4 * (4 + 8 + 8), "build": 4 + 4 + 32 + 48 + 24 + 12,
                         "kickdrift": 72 + 48}
 
 
@@ -243,6 +247,7 @@ def bench_allpairs(A, ctx, evals, n=65536):
             "force_error_vs_fp64": errs,
             "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": achieved / FP32_PEAK_TFLOPS,
+                         "frac_of_nonpacked_ceiling": achieved / FP32_NONPACKED_TFLOPS,
                          "traffic": traffic_per_launch("k_allpairs_f32_mixed" if vop == 15 else "k_allpairs_f32_plain"),
                          "kernel": "k_allpairs_f32",
                          "flop_per_interaction": 20, "valu_instructions_per_interaction": vop + 1},
@@ -377,6 +382,7 @@ def bench_adjoint(A, ctx, n=65536, T=512, ckpt=64, dt=1e-3):
             "ratio": ratio, "loss": loss, "grad_norms": [float(np.linalg.norm(g)) for g in (gx, gv, gm)],
             "roofline": {"bound": "valu", "kernel": "k_adj_fused", "kernel_ms": k_ms, "achieved": achieved,
                          "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                         "frac_of_nonpacked_ceiling": achieved / FP32_NONPACKED_TFLOPS,
                          "traffic": traffic_per_launch("k_adj_fused"),
                          "flop_per_pair": ADJ_FLOP_PER_PAIR}}
 
@@ -682,6 +688,7 @@ def main():
                    "n": bh["n"], "theta": bh["theta"], "method": args.method, "dt": bh["dt"]},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic_per_launch("k_bh_group"),
+                     "frac_of_nonpacked_ceiling": achieved / FP32_NONPACKED_TFLOPS,
                      "kernel": "traversal",
                      "kernel_ms": trav_ms, "flops": "20 per interaction + 8 per node visit"},
         "cpu_baseline": cpu,
