@@ -1299,16 +1299,30 @@ __device__ __forceinline__ void group_piece_sum(const float4* __restrict__ p, in
   }
 }
 
-template <bool MASK>
+// Post-Newtonian terms of the group traversal (PN): with every list flush each
+// body's lane also adds, in FP64, the pairwise 1PN / 2PN / 2.5PN terms
+// (compute_gravitational's PN part, tool_chain_impl.cpp:126-136) from each
+// list entry, taking the node's FP64 centre of mass, mass and mass-weighted
+// velocity as get_node_body does (barnes_hut.cpp:202-261; nodes at distance 0
+// skipped, :235), times 1/m_i. The list keeps its node ids in LDS for this.
+struct GroupPn {
+  const double4* nmc;   // FP64 {centre of mass, mass} per node (leaf k: body perm[k])
+  const double4* nvel;  // FP64 mass-weighted velocity per node
+  double G, ic, soft;
+  int pnf;
+};
+
+template <bool MASK, bool PN>
 __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restrict__ gstart,
                                                  const uint32_t* __restrict__ gcount, const float4* __restrict__ gsh,
                                                  const float4* __restrict__ kids, const uint32_t* __restrict__ perm,
                                                  float eps2, float msc, double G,
                                                  double* __restrict__ a, unsigned long long* __restrict__ counters,
-                                                 uint32_t* __restrict__ flags, int stack_cap) {
+                                                 uint32_t* __restrict__ flags, int stack_cap, GroupPn pnp) {
   extern __shared__ float4 grp_lds[];
   float4* lst = grp_lds;                                              // kGrpList + kGrpListPad entries
   int* stk = reinterpret_cast<int*>(grp_lds + kGrpList + kGrpListPad);  // stack_cap entries
+  int* lid = stk + stack_cap;  // PN: node id of each list entry (kGrpList entries)
   const int lane = threadIdx.x;
   const int ngroups = (int)*gcount;
   const int per_xcd = (ngroups + 7) >> 3;
@@ -1324,6 +1338,13 @@ __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restric
     const int k = lo + lane;
     const bool live = k <= hi;
     const float4 me = gsh[live ? k : hi];
+    // PN: this body's FP64 position, mass and velocity (leaf k's records)
+    double4 pme = make_double4(0.0, 0.0, 0.0, 1.0), vme = make_double4(0.0, 0.0, 0.0, 0.0);
+    double pax = 0.0, pay = 0.0, paz = 0.0;
+    if (PN) {
+      pme = pnp.nmc[live ? k : hi];
+      vme = pnp.nvel[live ? k : hi];
+    }
     // this lane's piece in each pass
     const int npass = __builtin_amdgcn_readfirstlane((int)kPieces.passes[gsz]);
     const int npc = __builtin_amdgcn_readfirstlane((int)kPieces.pieces[gsz]);
@@ -1391,6 +1412,26 @@ __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restric
         kahan_add32(sy, cy, ty);
         kahan_add32(sz, cz, tz);
       }
+      if (PN && live) {
+#pragma clang fp contract(off)
+        const double inv_m = 1.0 / pme.w;
+        for (int e = 0; e < m; ++e) {
+          const int node = __builtin_amdgcn_readfirstlane(lid[e]);
+          const double4 nc = pnp.nmc[node];
+          const double dx = nc.x - pme.x, dy = nc.y - pme.y, dz = nc.z - pme.z;
+          const double d2 = dx * dx + dy * dy + dz * dz;
+          if (d2 == 0.0) continue;
+          const double4 nv = pnp.nvel[node];
+          const double r2 = d2 + pnp.soft;
+          const double inv_r = 1.0 / sqrt(r2);
+          double fx = 0.0, fy = 0.0, fz = 0.0;
+          pn_add64(pnp.pnf, pme.w, nc.w, vme.x, vme.y, vme.z, nv.x, nv.y, nv.z, inv_r, -(dx * inv_r),
+                   -(dy * inv_r), -(dz * inv_r), pnp.G, pnp.ic, fx, fy, fz);
+          pax += fx * inv_m;
+          pay += fy * inv_m;
+          paz += fz * inv_m;
+        }
+      }
       wave_sync();  // the partial sums are read before the list is refilled
       listed += (unsigned long long)cnt;
       cnt = 0;
@@ -1399,7 +1440,10 @@ __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restric
       const int root = n, rs = 2 * (n - 1);
       const float4 r = kids[2 * rs];
       if (dmin2(r) > kids[2 * rs + 1].y) {
-        if (lane == 0) lst[0] = make_float4(r.x, r.y, r.z, r.w * msc);
+        if (lane == 0) {
+          lst[0] = make_float4(r.x, r.y, r.z, r.w * msc);
+          if (PN) lid[0] = root;
+        }
         cnt = 1;
       } else {
         if (lane == 0) stk[0] = root;
@@ -1448,7 +1492,11 @@ __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restric
         overflow = true;
       }
       // list entries carry the scaled mass (one multiply per entry, not per pair)
-      if (acc) lst[cnt + __popcll(am & lt)] = make_float4(r.x, r.y, r.z, r.w * msc);
+      if (acc) {
+        const int e = cnt + (int)__popcll(am & lt);
+        lst[e] = make_float4(r.x, r.y, r.z, r.w * msc);
+        if (PN) lid[e] = node;
+      }
       if (opn && !over) stk[sp + __popcll(om & lt)] = node;
       cnt += (int)__popcll(am);
       sp = over ? 0 : sp + nopen;
@@ -1463,9 +1511,9 @@ __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restric
     wave_sync();
     if (live) {
       const int b = (int)perm[k];
-      a[b] = G * ((double)sx - (double)cx);
-      a[n + b] = G * ((double)sy - (double)cy);
-      a[2 * n + b] = G * ((double)sz - (double)cz);
+      a[b] = G * ((double)sx - (double)cx) + pax;
+      a[n + b] = G * ((double)sy - (double)cy) + pay;
+      a[2 * n + b] = G * ((double)sz - (double)cz) + paz;
     }
     // every body's own leaf is in its group's list (distance 0, no force)
     const unsigned long long nlive = (unsigned long long)(hi - lo + 1);
@@ -1798,10 +1846,13 @@ static int bh_forces_all(aster_ctx* c, int method) {
       const int blocks = ((n / 32 + 1) + 7) & ~7;
       const dim3 grid(blocks), block(64);
       const int cap = grp_stack_cap(c->tree_levels);
-      const size_t lds = (kGrpList + kGrpListPad) * sizeof(float4) + (size_t)cap * sizeof(int);
-      auto kern = mask ? k_bh_group<true> : k_bh_group<false>;
+      const size_t lds = (kGrpList + kGrpListPad) * sizeof(float4) + (size_t)cap * sizeof(int) +
+                         (pn ? kGrpList * sizeof(int) : 0);
+      const GroupPn gp{c->nmc.p, c->nvel.p, c->P.G, ic, c->P.softening, pnf};
+      auto kern = pn ? (mask ? k_bh_group<true, true> : k_bh_group<false, true>)
+                     : (mask ? k_bh_group<true, false> : k_bh_group<false, false>);
       hipLaunchKernelGGL(kern, grid, block, lds, c->stream, n, c->gstart.p, c->gcount.p, c->gsh.p, c->kids.p,
-                         c->idx.p, eps2, sc.msc, sc.gsc, c->a.p, c->bh_counters.p, c->flags.p, cap);
+                         c->idx.p, eps2, sc.msc, sc.gsc, c->a.p, c->bh_counters.p, c->flags.p, cap, gp);
     } else if (mask) {
       hipLaunchKernelGGL((k_bh_f32<true>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->gsh.p, c->nrec.p,
                          c->nlink.p, eps2, sc.msc, sc.gsc, c->a.p, c->bh_counters.p, c->flags.p);
@@ -1868,8 +1919,9 @@ int bh_forces(aster_ctx* c, int method) {
   const int pnf = c->P.pn_flags & ASTER_PN_TERMS;
   const bool f64 = method == ASTER_BH_PARITY && c->P.precision == ASTER_FP64;
   if (pnf & ASTER_EIH) return fail(c, ASTER_ERR_INVALID, "ASTER_EIH needs ASTER_ALLPAIRS_FP64");
-  if (pnf && !f64)
-    return fail(c, ASTER_ERR_INVALID, "Barnes-Hut with post-Newtonian terms needs ASTER_BH_PARITY with precision ASTER_FP64");
+  if (pnf && !f64 && method != ASTER_BH_GROUP)
+    return fail(c, ASTER_ERR_INVALID,
+                "Barnes-Hut with post-Newtonian terms needs ASTER_BH_GROUP, or ASTER_BH_PARITY with precision ASTER_FP64");
   if (!(c->P.theta >= 0.0)) return fail(c, ASTER_ERR_INVALID, "theta must be >= 0");
   const int lo = c->P.lower < 0 ? 0 : c->P.lower;
   const int hi = (c->P.upper < 0 || c->P.upper > n) ? n : c->P.upper;

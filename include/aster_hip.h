@@ -75,9 +75,13 @@ enum aster_method {
    * accept when d > l/theta + delta (l = longest box side, delta = distance
    * of the centre of mass from the box centre). Per-body stackless traversal. */
   ASTER_BH_GEOMETRIC = 4,
-  /* As ASTER_BH_GEOMETRIC, but each wavefront of 64 Morton-consecutive bodies
-   * traverses once against the group's bounding box and evaluates the shared
-   * interaction list cooperatively (FP32). */
+  /* As ASTER_BH_GEOMETRIC, but the bodies of each subtree of at most 64
+   * leaves (a group, Morton-consecutive) share one traversal against the
+   * group's bounding box, and the wavefront evaluates the shared interaction
+   * list cooperatively (FP32 sums, Kahan-compensated across list flushes).
+   * The post-Newtonian terms of pn_flags are added in FP64 over the same
+   * lists, with each node's FP64 centre of mass, mass and mass-weighted
+   * velocity as in Barnes_Hut::get_node_body (barnes_hut.cpp:202-261). */
   ASTER_BH_GROUP = 5
 };
 

@@ -164,16 +164,10 @@ inline V3 pn25_term(double m1, double m2, V3 v1, V3 v2, double inv_r, V3 n, doub
 // body 2 (G m1 m2 / r^2 along d = p2 - p1, Plummer-softened with
 // softening = eps^2), plus the pairwise 1PN, 2PN and 2.5PN terms the flags
 // select, each times 1/c^k and m1.
-inline V3 gravitational(double m1, double m2, V3 v1, V3 v2, V3 p1, V3 p2, const Law& L) {
-  V3 d{p2.x - p1.x, p2.y - p1.y, p2.z - p1.z};
-  double r2 = d.x * d.x + d.y * d.y + d.z * d.z + L.soft;
-  double inv_r = 1.0 / std::sqrt(r2);
-  double inv_r2 = inv_r * inv_r;
-  V3 norm{d.x * inv_r, d.y * inv_r, d.z * inv_r};
-  double scal = L.G * m2 * m1 * inv_r2;
-  V3 f{norm.x * scal, norm.y * scal, norm.z * scal};
+// The post-Newtonian terms of compute_gravitational (tool_chain_impl.cpp:126-136)
+// added to f in the reference's order; mn is the caller's -norm.
+inline void add_pn(V3& f, double m1, double m2, V3 v1, V3 v2, double inv_r, V3 mn, const Law& L) {
   const double inv_c = 1.0 / L.c;
-  const V3 mn{-norm.x, -norm.y, -norm.z};
   if (L.pn & kPN1) {
     double ic2 = inv_c * inv_c;
     V3 p = pn1_term(m1, m2, v1, v2, inv_r, mn, L.G);
@@ -198,6 +192,28 @@ inline V3 gravitational(double m1, double m2, V3 v1, V3 v2, V3 p1, V3 p2, const 
     f.y = f.y + a25.y * m1;
     f.z = f.z + a25.z * m1;
   }
+}
+
+inline V3 gravitational(double m1, double m2, V3 v1, V3 v2, V3 p1, V3 p2, const Law& L) {
+  V3 d{p2.x - p1.x, p2.y - p1.y, p2.z - p1.z};
+  double r2 = d.x * d.x + d.y * d.y + d.z * d.z + L.soft;
+  double inv_r = 1.0 / std::sqrt(r2);
+  double inv_r2 = inv_r * inv_r;
+  V3 norm{d.x * inv_r, d.y * inv_r, d.z * inv_r};
+  double scal = L.G * m2 * m1 * inv_r2;
+  V3 f{norm.x * scal, norm.y * scal, norm.z * scal};
+  add_pn(f, m1, m2, v1, v2, inv_r, V3{-norm.x, -norm.y, -norm.z}, L);
+  return f;
+}
+
+// compute_gravitational's post-Newtonian part alone (its terms added to zero).
+inline V3 pn_part(double m1, double m2, V3 v1, V3 v2, V3 p1, V3 p2, const Law& L) {
+  V3 d{p2.x - p1.x, p2.y - p1.y, p2.z - p1.z};
+  double r2 = d.x * d.x + d.y * d.y + d.z * d.z + L.soft;
+  double inv_r = 1.0 / std::sqrt(r2);
+  V3 norm{d.x * inv_r, d.y * inv_r, d.z * inv_r};
+  V3 f{0.0, 0.0, 0.0};
+  add_pn(f, m1, m2, v1, v2, inv_r, V3{-norm.x, -norm.y, -norm.z}, L);
   return f;
 }
 
@@ -751,6 +767,64 @@ int orc_bh_group(int64_t n, const double* pos, const double* mass, const double*
   });
   if (interactions) *interactions = std::accumulate(g_inter.begin(), g_inter.end(), (int64_t)0);
   if (visits) *visits = std::accumulate(g_vis.begin(), g_vis.end(), (int64_t)0);
+  return 0;
+}
+
+// The post-Newtonian part of the group traversal's accelerations: the
+// pairwise 1PN / 2PN / 2.5PN terms `pn` selects (compute_gravitational's PN
+// part, tool_chain_impl.cpp:126-136) between each body and every node of its
+// group's list (the lists of orc_bh_group), with the node's FP64 centre of
+// mass, mass and mass-weighted velocity (get_node_body,
+// barnes_hut.cpp:202-261), skipping nodes at distance 0 (:235), times 1/m_i.
+// FP64 sums in node order. The library adds this to the FP32 Newtonian sums.
+int orc_bh_group_pn(int64_t n, const double* pos, const double* vel, const double* mass, const double* box,
+                    double theta, double G, double c, double soft, int pn, double* acc, int nthreads) {
+  for (int64_t i = 0; i < 3 * n; ++i) acc[i] = 0.0;
+  if (n < 2) return 0;
+  BuiltTree B;
+  make_tree(n, pos, vel, mass, box, B, nthreads);
+  GroupTree GT;
+  group_tree(n, B.T, theta, GT);
+  std::vector<int64_t> roots;
+  for (int64_t k = 0; k < 2 * n - 1; ++k)
+    if (GT.size[k] <= 64 && (GT.parent[k] < 0 || GT.size[GT.parent[k]] > 64)) roots.push_back(k);
+  auto first_leaf = [&](int64_t k) { while (k >= n) k = B.T.left[k]; return k; };
+  const Law L{G, c, soft, pn & (kPN1 | kPN2 | kPN25)};
+  parallel_for((int64_t)roots.size(), nthreads, [&](int64_t g) {
+    const int64_t root = roots[g], lo = first_leaf(root), cnt = GT.size[root];
+    const float* blo = &GT.lo[3 * root];
+    const float* bhi = &GT.hi[3 * root];
+    std::vector<int64_t> list, stack{n};
+    while (!stack.empty()) {
+      const int64_t k = stack.back();
+      stack.pop_back();
+      if (k < n || box_d2_f(blo, bhi, &GT.rec[4 * k]) > GT.crit2[k]) {
+        list.push_back(k);
+      } else {
+        stack.push_back(B.T.left[k]);
+        stack.push_back(B.T.right[k]);
+      }
+    }
+    std::sort(list.begin(), list.end());
+    for (int64_t s = lo; s < lo + cnt; ++s) {
+      const uint32_t b = B.perm[s];
+      const V3 pi = load3(pos, b), vi = load3(vel, b);
+      const double mi = mass[b];
+      const double inv_m = 1.0 / mi;
+      V3 a{0.0, 0.0, 0.0};
+      for (int64_t k : list) {
+        const V3 com = load3(B.T.com.data(), k), vn = load3(B.T.vel.data(), k);
+        const double mn = B.T.mass[k];
+        const V3 d{pi.x - com.x, pi.y - com.y, pi.z - com.z};
+        if (d.x * d.x + d.y * d.y + d.z * d.z == 0.0) continue;
+        const V3 f = pn_part(mi, mn, vi, vn, pi, com, L);
+        a.x += f.x * inv_m;
+        a.y += f.y * inv_m;
+        a.z += f.z * inv_m;
+      }
+      store3(acc, b, a);
+    }
+  });
   return 0;
 }
 

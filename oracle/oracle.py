@@ -33,6 +33,7 @@ _SIG = {
     "orc_bh": [_I64, _D, _D, _D, _D, _dbl, _dbl, _dbl, _dbl, ctypes.c_int, _D, _PI64, ctypes.c_int],
     "orc_bh_group": [_I64, _D, _D, _D, _dbl, _dbl, _dbl, _D, _PI64, _PI64, ctypes.c_int],
     "orc_bh_group_bruteforce_count": [_I64, _D, _D, _D, _dbl],
+    "orc_bh_group_pn": [_I64, _D, _D, _D, _D, _dbl, _dbl, _dbl, _dbl, ctypes.c_int, _D, ctypes.c_int],
     "orc_energy": [_I64, _D, _D, _D, _dbl, _dbl, _D, _D, _D],
     "orc_yoshida_weights": [_D],
     "orc_integrate": [_I64, _D, _D, _D, _dbl, _dbl, _dbl, ctypes.c_int, ctypes.c_int, _D, _dbl, ctypes.c_int,
@@ -189,6 +190,18 @@ def bh_group(pos, mass, box, theta, G=1.0, soft=0.0, threads=0):
     lib().orc_bh_group(n, _dp(pos), _dp(mass), _dp(box), theta, G, soft, _dp(acc), ctypes.byref(inter),
                        ctypes.byref(vis), threads)
     return acc, inter.value, vis.value
+
+
+def bh_group_pn(pos, vel, mass, box, theta, G=1.0, c=1e4, soft=0.0, pn=0x11, threads=0):
+    """The post-Newtonian part of the group traversal's accelerations (n, 3):
+    the terms `pn` selects between each body and its group's list, with the
+    nodes' FP64 centres of mass, masses and mass-weighted velocities."""
+    pos, vel, mass, box = _c(pos), _c(vel), _c(mass), _c(box)
+    n = len(mass)
+    acc = np.empty((n, 3))
+    assert lib().orc_bh_group_pn(n, _dp(pos), _dp(vel), _dp(mass), _dp(box), theta, G, c, soft, pn, _dp(acc),
+                                 threads) == 0
+    return acc
 
 
 def bh_group_bruteforce_count(pos, mass, box, theta):

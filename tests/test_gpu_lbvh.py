@@ -121,7 +121,51 @@ def test_bh_fp64_parity_pn2_pn25_bit_identical(ctx, orc):
     assert np.array_equal(acc, ref)
     load(ctx, pos, vel, mass, theta=0.4, precision=A.FP32, c=30.0, pn_flags=flags)
     with pytest.raises(A.AsterError):
-        ctx.forces(A.BH_GROUP)  # post-Newtonian terms need the FP64 parity traversal
+        ctx.forces(A.BH_GEOMETRIC)  # the per-body FP32 walks have no post-Newtonian terms
+
+
+@pytest.mark.parametrize("n,theta", [(20000, 0.5), (3000, 0.0), (130, 0.7)])
+def test_bh_group_pn_matches_oracle(ctx, orc, n, theta):
+    """Group traversal with the 1PN, 2PN and 2.5PN terms: the FP32 Newtonian
+    sums as without them, plus the FP64 post-Newtonian part over the same
+    lists (node centres of mass, masses and mass-weighted velocities), which
+    must equal the oracle's to FP64 summation-order rounding."""
+    flags = A.NEWTON | A.PN1 | A.PN2 | A.PN25
+    pos, vel, mass = mixed(n, seed=n + 3)
+    box = orc.box_auto(pos)
+    load(ctx, pos, vel, mass, theta=theta, precision=A.FP32, c=30.0, pn_flags=A.NEWTON)
+    newton = ctx.forces(A.BH_GROUP)
+    load(ctx, pos, vel, mass, theta=theta, precision=A.FP32, c=30.0, pn_flags=flags)
+    acc = ctx.forces(A.BH_GROUP)
+    ref_pn = orc.bh_group_pn(pos, vel, mass, box, theta, c=30.0, soft=2.5e-5, pn=flags)
+    scale = np.linalg.norm(ref_pn, axis=1)
+    err = np.linalg.norm((acc - newton) - ref_pn, axis=1)
+    assert np.all(err <= 1e-9 * np.maximum(scale, np.median(scale)) + 1e-15 * np.abs(acc).max())
+    # and the whole force against the oracle's FP64 sums of the same lists
+    ref_n, _, _ = orc.bh_group(pos, mass, box, theta, soft=2.5e-5)
+    ref = ref_n + ref_pn
+    s = np.linalg.norm(ref, axis=1)
+    e = np.linalg.norm(acc - ref, axis=1) / np.maximum(s, np.median(s))
+    assert np.median(e) < 1e-6 and e.max() < 1e-4
+
+
+def test_bh_group_pn_leapfrog_steps(ctx, orc):
+    """Leapfrog with the velocity-dependent 1PN law over the group traversal
+    with every node opened (theta = 0) against update_leapfrog over FP64
+    all-pairs: the positions agree to the FP32 rounding of the Newtonian sums
+    (bars as for the Newtonian steps)."""
+    n, dt, steps = 3000, 5e-4, 4
+    pos, vel, mass = mixed(n, seed=33)
+    load(ctx, pos, vel, mass, theta=0.0, precision=A.FP32, c=30.0, pn_flags=A.NEWTON | A.PN1, box=(0.0, 0.0, 0.0))
+    ctx.step(A.BH_GROUP, A.LEAPFROG, dt, steps)
+    p, v = ctx.get_state()
+    rp, rv, ra = orc.integrate(pos, vel, mass, dt, steps, integrator=2, method=0, c=30.0, soft=2.5e-5, pn=0x11)
+    amax = np.abs(ra).max()
+    ex = np.abs(p - rp).max(1) / (steps ** 2 * dt ** 2 * amax)
+    ev = np.abs(v - rv).max(1) / (steps * dt * amax)
+    assert np.all(np.isfinite(p)) and np.all(np.isfinite(v))
+    assert np.quantile(ex, 0.999) <= 1e-5 and ex.max() <= 1e-3
+    assert np.quantile(ev, 0.999) <= 1e-5 and ev.max() <= 1e-3
 
 
 def test_bh_fp64_parity_node_velocities_in_moment_bands(ctx, orc):

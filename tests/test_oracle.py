@@ -831,3 +831,49 @@ def test_traj_grad_finite_differences(orc):
         fd = (orc.traj_loss(pos, vel, m1, dt, steps, snaps, target, soft=soft)
               - orc.traj_loss(pos, vel, m2, dt, steps, snaps, target, soft=soft)) / (2 * h)
         assert gm[k] == pytest.approx(fd, rel=1e-6, abs=1e-10)
+
+
+def test_bh_group_pn_at_theta_zero_is_allpairs_pn_part(orc):
+    # every node opened: the post-Newtonian part over the group lists is the
+    # all-pairs force with the PN terms minus the Newtonian force (up to the
+    # rounding of that difference and of the sum order)
+    import aster_amd
+    pos, vel, mass = aster_amd.ic_mixed(300, 41)
+    flags = 0x1 | 0x10 | 0x100 | 0x1000
+    c, soft = 30.0, 2.5e-5
+    ref = orc.allpairs(pos, vel, mass, c=c, soft=soft, pn=flags) - orc.allpairs(pos, vel, mass, c=c, soft=soft, pn=1)
+    got = orc.bh_group_pn(pos, vel, mass, orc.box_auto(pos), 0.0, c=c, soft=soft, pn=flags)
+    scale = np.linalg.norm(ref, axis=1)
+    assert np.all(np.linalg.norm(got - ref, axis=1) <= 1e-9 * scale + 1e-14 * np.abs(ref).max())
+    # the Newtonian law alone has no PN part
+    assert not np.any(orc.bh_group_pn(pos, vel, mass, orc.box_auto(pos), 0.5, c=c, soft=soft, pn=1))
+
+
+def test_bh_group_pn_takes_node_mass_com_and_velocity(orc):
+    # Two clusters of 64 bodies far apart: each is one traversal group whose
+    # list holds its own 64 leaves and the other cluster as a single node. The
+    # light cluster's PN part is then the Blanchet 1PN sum over its own bodies
+    # plus one term from a body of the heavy cluster's total mass at its centre
+    # of mass moving with its mass-weighted velocity. That term dominates, and
+    # the heavy cluster's bodies taken one by one would differ from it by
+    # ~(0.3 / 170)^2 ~ 3e-6.
+    r = rng(17)
+    pos = np.concatenate([r.normal(size=(64, 3)) * 0.3 - 50.0, r.normal(size=(64, 3)) * 0.3 + 50.0])
+    vel = np.concatenate([r.normal(size=(64, 3)) * 0.3 + 0.5, r.normal(size=(64, 3)) * 0.3 - 0.2])
+    mass = np.concatenate([r.uniform(0.5, 1.5, size=64) * 1e-6, r.uniform(0.5, 1.5, size=64)])
+    c = 30.0
+    got = orc.bh_group_pn(pos, vel, mass, orc.box_auto(pos), 0.5, G=G, c=c, soft=0.0, pn=0x11)
+    heavy = slice(64, 128)
+    M = mass[heavy].sum()
+    X = (mass[heavy, None] * pos[heavy]).sum(0) / M
+    V = (mass[heavy, None] * vel[heavy]).sum(0) / M
+    for i in range(64):
+        e = blanchet_1pn(mass[i], M, pos[i], X, vel[i], V, G, c)[1]
+        for j in range(64):
+            if j != i:
+                e = e + blanchet_1pn(mass[i], mass[j], pos[i], pos[j], vel[i], vel[j], G, c)[1]
+        np.testing.assert_allclose(got[i], e, rtol=1e-9)
+        # bodies one by one instead of the node: measurably different
+        leaves = sum(blanchet_1pn(mass[i], mass[j], pos[i], pos[j], vel[i], vel[j], G, c)[1] for j in range(64, 128))
+        assert np.linalg.norm(leaves - (e - sum(blanchet_1pn(mass[i], mass[j], pos[i], pos[j], vel[i], vel[j], G, c)[1]
+                                                 for j in range(64) if j != i))) > 1e-7 * np.linalg.norm(e)
