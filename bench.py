@@ -164,8 +164,25 @@ def bench_bh(A, ctx, steps, warmup, method, n=1 << 20, theta=0.5, dt=5e-4, refer
             sweep[str(th)] = e
             log(f"[bench] theta={th}: {e}")
         errs["theta_sweep"] = sweep
+        # the same traversal with the pairwise 1PN terms (added in FP64 over
+        # the same lists): force evaluation time against the Newtonian one
         params(A, ctx, G=1.0, softening=0.005 ** 2, theta=theta, precision=A.FP32)
         ctx.set_state(pos, vel, mass)
+        pn_ms = {}
+        for name, flags in (("newton", A.NEWTON), ("newton_pn1", A.NEWTON | A.PN1)):
+            params(A, ctx, G=1.0, softening=0.005 ** 2, theta=theta, precision=A.FP32, c=1e4, pn_flags=flags)
+            ctx.forces(method, out=False)
+            ctx.sync()
+            ctx.profile(True)
+            for _ in range(3):
+                ctx.forces(method, out=False)
+            ph = ctx.phase_times()
+            ctx.profile(False)
+            pn_ms[name] = ph["traverse"][0] / ph["traverse"][1]
+        ctx.bh_stats()
+        errs["pn1_traverse_ms"] = pn_ms
+        log(f"[bench] traversal with the 1PN terms: {pn_ms}")
+        params(A, ctx, G=1.0, softening=0.005 ** 2, theta=theta, precision=A.FP32)
     # warmup, then K timed leapfrog steps (no per-phase events inside the
     # timed region: they cost ~2.5% of a step)
     ctx.step(method, A.LEAPFROG, dt, warmup)
