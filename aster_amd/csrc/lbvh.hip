@@ -1033,15 +1033,53 @@ __global__ __launch_bounds__(256) void k_links(int n, const int32_t* __restrict_
 // --------------------------------------------------------- traversals ------
 
 // FP32 per-body stackless traversal (lanes = consecutive sorted bodies).
-template <bool MASK>
+// Post-Newtonian terms of the FP32 traversals: for every node a body takes,
+// the pairwise 1PN / 2PN / 2.5PN terms of compute_gravitational
+// (tool_chain_impl.cpp:126-136) in FP64, with the node's FP64 centre of mass,
+// mass and mass-weighted velocity as get_node_body takes them
+// (barnes_hut.cpp:202-261; nodes at distance 0 skipped, :235), times 1/m_i,
+// added to the FP32 Newtonian sums.
+struct NodePn {
+  const double4* nmc;   // FP64 {centre of mass, mass} per node (leaf k: body perm[k])
+  const double4* nvel;  // FP64 mass-weighted velocity per node
+  double G, ic, soft;
+  int pnf;
+};
+
+// The terms from node `node` on the body of leaf record pme / velocity vme.
+__device__ __forceinline__ void node_pn(const NodePn& p, int node, const double4& pme, const double4& vme,
+                                        double inv_m, double& ax, double& ay, double& az) {
+#pragma clang fp contract(off)
+  const double4 nc = p.nmc[node];
+  const double dx = nc.x - pme.x, dy = nc.y - pme.y, dz = nc.z - pme.z;
+  const double d2 = dx * dx + dy * dy + dz * dz;
+  if (d2 == 0.0) return;
+  const double4 nv = p.nvel[node];
+  const double inv_r = 1.0 / sqrt(d2 + p.soft);
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  pn_add64(p.pnf, pme.w, nc.w, vme.x, vme.y, vme.z, nv.x, nv.y, nv.z, inv_r, -(dx * inv_r), -(dy * inv_r),
+           -(dz * inv_r), p.G, p.ic, fx, fy, fz);
+  ax += fx * inv_m;
+  ay += fy * inv_m;
+  az += fz * inv_m;
+}
+
+template <bool MASK, bool PN>
 __global__ __launch_bounds__(256) void k_bh_f32(int n, const float4* __restrict__ gsh, const float4* __restrict__ nrec,
                                                 const int4* __restrict__ nlink, float eps2, float msc, double G,
                                                 double* __restrict__ a, unsigned long long* __restrict__ counters,
-                                                uint32_t* __restrict__ flags) {
+                                                uint32_t* __restrict__ flags, NodePn pnp) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = k < n;
   const float4 me = gsh[live ? k : n - 1];
   float ax = 0.f, ay = 0.f, az = 0.f;
+  double4 pme = make_double4(0.0, 0.0, 0.0, 1.0), vme = make_double4(0.0, 0.0, 0.0, 0.0);
+  double pax = 0.0, pay = 0.0, paz = 0.0, inv_m = 1.0;
+  if (PN) {
+    pme = pnp.nmc[live ? k : n - 1];
+    vme = pnp.nvel[live ? k : n - 1];
+    inv_m = 1.0 / pme.w;
+  }
   uint32_t inter = 0, visits = 0;
   int node = live ? n : -1;
   const uint32_t limit = 4u * (uint32_t)n + 16u;
@@ -1067,6 +1105,7 @@ __global__ __launch_bounds__(256) void k_bh_f32(int n, const float4* __restrict_
       ax = fmaf(f, dx, ax);
       ay = fmaf(f, dy, ay);
       az = fmaf(f, dz, az);
+      if (PN) node_pn(pnp, node, pme, vme, inv_m, pax, pay, paz);
       ++inter;
       node = L.y;
     } else {
@@ -1075,9 +1114,9 @@ __global__ __launch_bounds__(256) void k_bh_f32(int n, const float4* __restrict_
   }
   if (live) {
     const int b = nlink[k].w;
-    a[b] = G * (double)ax;
-    a[n + b] = G * (double)ay;
-    a[2 * n + b] = G * (double)az;
+    a[b] = G * (double)ax + pax;
+    a[n + b] = G * (double)ay + pay;
+    a[2 * n + b] = G * (double)az + paz;
   }
   unsigned long long ci = inter, cv = visits;
   for (int off = 32; off > 0; off >>= 1) {
@@ -1300,25 +1339,15 @@ __device__ __forceinline__ void group_piece_sum(const float4* __restrict__ p, in
 }
 
 // Post-Newtonian terms of the group traversal (PN): with every list flush each
-// body's lane also adds, in FP64, the pairwise 1PN / 2PN / 2.5PN terms
-// (compute_gravitational's PN part, tool_chain_impl.cpp:126-136) from each
-// list entry, taking the node's FP64 centre of mass, mass and mass-weighted
-// velocity as get_node_body does (barnes_hut.cpp:202-261; nodes at distance 0
-// skipped, :235), times 1/m_i. The list keeps its node ids in LDS for this.
-struct GroupPn {
-  const double4* nmc;   // FP64 {centre of mass, mass} per node (leaf k: body perm[k])
-  const double4* nvel;  // FP64 mass-weighted velocity per node
-  double G, ic, soft;
-  int pnf;
-};
-
+// body's lane also adds, in FP64, the terms of each list entry (NodePn). The
+// list keeps its node ids in LDS for this.
 template <bool MASK, bool PN>
 __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restrict__ gstart,
                                                  const uint32_t* __restrict__ gcount, const float4* __restrict__ gsh,
                                                  const float4* __restrict__ kids, const uint32_t* __restrict__ perm,
                                                  float eps2, float msc, double G,
                                                  double* __restrict__ a, unsigned long long* __restrict__ counters,
-                                                 uint32_t* __restrict__ flags, int stack_cap, GroupPn pnp) {
+                                                 uint32_t* __restrict__ flags, int stack_cap, NodePn pnp) {
   extern __shared__ float4 grp_lds[];
   float4* lst = grp_lds;                                              // kGrpList + kGrpListPad entries
   int* stk = reinterpret_cast<int*>(grp_lds + kGrpList + kGrpListPad);  // stack_cap entries
@@ -1413,24 +1442,9 @@ __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restric
         kahan_add32(sz, cz, tz);
       }
       if (PN && live) {
-#pragma clang fp contract(off)
         const double inv_m = 1.0 / pme.w;
-        for (int e = 0; e < m; ++e) {
-          const int node = __builtin_amdgcn_readfirstlane(lid[e]);
-          const double4 nc = pnp.nmc[node];
-          const double dx = nc.x - pme.x, dy = nc.y - pme.y, dz = nc.z - pme.z;
-          const double d2 = dx * dx + dy * dy + dz * dz;
-          if (d2 == 0.0) continue;
-          const double4 nv = pnp.nvel[node];
-          const double r2 = d2 + pnp.soft;
-          const double inv_r = 1.0 / sqrt(r2);
-          double fx = 0.0, fy = 0.0, fz = 0.0;
-          pn_add64(pnp.pnf, pme.w, nc.w, vme.x, vme.y, vme.z, nv.x, nv.y, nv.z, inv_r, -(dx * inv_r),
-                   -(dy * inv_r), -(dz * inv_r), pnp.G, pnp.ic, fx, fy, fz);
-          pax += fx * inv_m;
-          pay += fy * inv_m;
-          paz += fz * inv_m;
-        }
+        for (int e = 0; e < m; ++e)
+          node_pn(pnp, __builtin_amdgcn_readfirstlane(lid[e]), pme, vme, inv_m, pax, pay, paz);
       }
       wave_sync();  // the partial sums are read before the list is refilled
       listed += (unsigned long long)cnt;
@@ -1848,17 +1862,17 @@ static int bh_forces_all(aster_ctx* c, int method) {
       const int cap = grp_stack_cap(c->tree_levels);
       const size_t lds = (kGrpList + kGrpListPad) * sizeof(float4) + (size_t)cap * sizeof(int) +
                          (pn ? kGrpList * sizeof(int) : 0);
-      const GroupPn gp{c->nmc.p, c->nvel.p, c->P.G, ic, c->P.softening, pnf};
+      const NodePn gp{c->nmc.p, c->nvel.p, c->P.G, ic, c->P.softening, pnf};
       auto kern = pn ? (mask ? k_bh_group<true, true> : k_bh_group<false, true>)
                      : (mask ? k_bh_group<true, false> : k_bh_group<false, false>);
       hipLaunchKernelGGL(kern, grid, block, lds, c->stream, n, c->gstart.p, c->gcount.p, c->gsh.p, c->kids.p,
                          c->idx.p, eps2, sc.msc, sc.gsc, c->a.p, c->bh_counters.p, c->flags.p, cap, gp);
-    } else if (mask) {
-      hipLaunchKernelGGL((k_bh_f32<true>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->gsh.p, c->nrec.p,
-                         c->nlink.p, eps2, sc.msc, sc.gsc, c->a.p, c->bh_counters.p, c->flags.p);
     } else {
-      hipLaunchKernelGGL((k_bh_f32<false>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->gsh.p, c->nrec.p,
-                         c->nlink.p, eps2, sc.msc, sc.gsc, c->a.p, c->bh_counters.p, c->flags.p);
+      const NodePn gp{c->nmc.p, c->nvel.p, c->P.G, ic, c->P.softening, pnf};
+      auto kern = pn ? (mask ? k_bh_f32<true, true> : k_bh_f32<false, true>)
+                     : (mask ? k_bh_f32<true, false> : k_bh_f32<false, false>);
+      hipLaunchKernelGGL(kern, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->gsh.p, c->nrec.p, c->nlink.p,
+                         eps2, sc.msc, sc.gsc, c->a.p, c->bh_counters.p, c->flags.p, gp);
     }
   }
   ASTER_HIP(c, hipGetLastError());
@@ -1919,9 +1933,6 @@ int bh_forces(aster_ctx* c, int method) {
   const int pnf = c->P.pn_flags & ASTER_PN_TERMS;
   const bool f64 = method == ASTER_BH_PARITY && c->P.precision == ASTER_FP64;
   if (pnf & ASTER_EIH) return fail(c, ASTER_ERR_INVALID, "ASTER_EIH needs ASTER_ALLPAIRS_FP64");
-  if (pnf && !f64 && method != ASTER_BH_GROUP)
-    return fail(c, ASTER_ERR_INVALID,
-                "Barnes-Hut with post-Newtonian terms needs ASTER_BH_GROUP, or ASTER_BH_PARITY with precision ASTER_FP64");
   if (!(c->P.theta >= 0.0)) return fail(c, ASTER_ERR_INVALID, "theta must be >= 0");
   const int lo = c->P.lower < 0 ? 0 : c->P.lower;
   const int hi = (c->P.upper < 0 || c->P.upper > n) ? n : c->P.upper;

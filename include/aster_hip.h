@@ -73,7 +73,10 @@ enum aster_method {
   ASTER_BH_PARITY = 3,
   /* LBVH Barnes-Hut, FP32, geometric opening test on node bounding boxes:
    * accept when d > l/theta + delta (l = longest box side, delta = distance
-   * of the centre of mass from the box centre). Per-body stackless traversal. */
+   * of the centre of mass from the box centre). Per-body stackless traversal.
+   * On every FP32 traversal the post-Newtonian terms of pn_flags are added in
+   * FP64 for each node taken, with its FP64 centre of mass, mass and
+   * mass-weighted velocity as in Barnes_Hut::get_node_body. */
   ASTER_BH_GEOMETRIC = 4,
   /* As ASTER_BH_GEOMETRIC, but the bodies of each subtree of at most 64
    * leaves (a group, Morton-consecutive) share one traversal against the

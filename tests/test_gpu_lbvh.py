@@ -119,9 +119,40 @@ def test_bh_fp64_parity_pn2_pn25_bit_identical(ctx, orc):
     acc = ctx.forces(A.BH_PARITY)
     ref = orc.bh(pos, vel, mass, orc.box_auto(pos), 0.4, c=30.0, soft=2.5e-5, pn=flags)
     assert np.array_equal(acc, ref)
-    load(ctx, pos, vel, mass, theta=0.4, precision=A.FP32, c=30.0, pn_flags=flags)
+    load(ctx, pos, vel, mass, theta=0.4, precision=A.FP32, c=30.0, pn_flags=A.NEWTON | A.EIH)
     with pytest.raises(A.AsterError):
-        ctx.forces(A.BH_GEOMETRIC)  # the per-body FP32 walks have no post-Newtonian terms
+        ctx.forces(A.BH_GEOMETRIC)  # the N-body EIH law runs on FP64 all-pairs only
+
+
+@pytest.mark.parametrize("method", [A.BH_GEOMETRIC, A.BH_PARITY])
+def test_bh_per_body_fp32_pn(ctx, orc, method):
+    """The per-body FP32 walks add the post-Newtonian terms in FP64 for every
+    node they take. With every node opened (theta = 0) the post-Newtonian part
+    is the all-pairs one (to FP64 summation-order rounding); at theta = 0.5 the
+    FP32 reference-MAC walk takes the nodes of the FP64 oracle walk, so most
+    bodies get its post-Newtonian part to the same rounding."""
+    flags = A.NEWTON | A.PN1 | A.PN2 | A.PN25
+    pos, vel, mass = mixed(3000, seed=44)
+
+    def pn_part(theta):
+        load(ctx, pos, vel, mass, theta=theta, precision=A.FP32, c=30.0, pn_flags=A.NEWTON)
+        newton = ctx.forces(method)
+        load(ctx, pos, vel, mass, theta=theta, precision=A.FP32, c=30.0, pn_flags=flags)
+        return ctx.forces(method) - newton
+
+    got = pn_part(0.0)
+    ref = orc.bh_group_pn(pos, vel, mass, orc.box_auto(pos), 0.0, c=30.0, soft=2.5e-5, pn=flags)
+    scale = np.linalg.norm(ref, axis=1)
+    assert np.all(np.linalg.norm(got - ref, axis=1) <= 1e-9 * np.maximum(scale, np.median(scale)))
+    if method == A.BH_PARITY:
+        got = pn_part(0.5)
+        box = orc.box_auto(pos)
+        ref = (orc.bh(pos, vel, mass, box, 0.5, c=30.0, soft=2.5e-5, pn=flags)
+               - orc.bh(pos, vel, mass, box, 0.5, c=30.0, soft=2.5e-5, pn=A.NEWTON))
+        scale = np.linalg.norm(ref, axis=1)
+        e = np.linalg.norm(got - ref, axis=1) / np.maximum(scale, np.median(scale))
+        print(f"\nparity MAC FP32 walk: PN part vs oracle median {np.median(e):.2e} p90 {np.quantile(e, .9):.2e}")
+        assert np.median(e) < 1e-9 and np.quantile(e, 0.9) < 1e-9
 
 
 @pytest.mark.parametrize("n,theta", [(20000, 0.5), (3000, 0.0), (130, 0.7)])
