@@ -1046,15 +1046,15 @@ struct NodePn {
   int pnf;
 };
 
-// The terms from node `node` on the body of leaf record pme / velocity vme.
-__device__ __forceinline__ void node_pn(const NodePn& p, int node, const double4& pme, const double4& vme,
-                                        double inv_m, double& ax, double& ay, double& az) {
+// The terms from a node (record nc, velocity nv) on the body of leaf record
+// pme / velocity vme.
+__device__ __forceinline__ void node_pn_rec(const NodePn& p, const double4& nc, const double4& nv,
+                                            const double4& pme, const double4& vme, double inv_m, double& ax,
+                                            double& ay, double& az) {
 #pragma clang fp contract(off)
-  const double4 nc = p.nmc[node];
   const double dx = nc.x - pme.x, dy = nc.y - pme.y, dz = nc.z - pme.z;
   const double d2 = dx * dx + dy * dy + dz * dz;
   if (d2 == 0.0) return;
-  const double4 nv = p.nvel[node];
   const double inv_r = 1.0 / sqrt(d2 + p.soft);
   double fx = 0.0, fy = 0.0, fz = 0.0;
   pn_add64(p.pnf, pme.w, nc.w, vme.x, vme.y, vme.z, nv.x, nv.y, nv.z, inv_r, -(dx * inv_r), -(dy * inv_r),
@@ -1062,6 +1062,11 @@ __device__ __forceinline__ void node_pn(const NodePn& p, int node, const double4
   ax += fx * inv_m;
   ay += fy * inv_m;
   az += fz * inv_m;
+}
+
+__device__ __forceinline__ void node_pn(const NodePn& p, int node, const double4& pme, const double4& vme,
+                                        double inv_m, double& ax, double& ay, double& az) {
+  node_pn_rec(p, p.nmc[node], p.nvel[node], pme, vme, inv_m, ax, ay, az);
 }
 
 template <bool MASK, bool PN>
@@ -1352,6 +1357,9 @@ __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restric
   float4* lst = grp_lds;                                              // kGrpList + kGrpListPad entries
   int* stk = reinterpret_cast<int*>(grp_lds + kGrpList + kGrpListPad);  // stack_cap entries
   int* lid = stk + stack_cap;  // PN: node id of each list entry (kGrpList entries)
+  // PN: the entries' FP64 node records, gathered once per flush (kGrpList each)
+  double4* pnc = reinterpret_cast<double4*>(grp_lds + kGrpList + kGrpListPad + (stack_cap + kGrpList + 3) / 4);
+  double4* pnv = pnc + kGrpList;
   const int lane = threadIdx.x;
   const int ngroups = (int)*gcount;
   const int per_xcd = (ngroups + 7) >> 3;
@@ -1441,10 +1449,19 @@ __global__ __launch_bounds__(64) void k_bh_group(int n, const int32_t* __restric
         kahan_add32(sy, cy, ty);
         kahan_add32(sz, cz, tz);
       }
-      if (PN && live) {
-        const double inv_m = 1.0 / pme.w;
-        for (int e = 0; e < m; ++e)
-          node_pn(pnp, __builtin_amdgcn_readfirstlane(lid[e]), pme, vme, inv_m, pax, pay, paz);
+      if (PN) {
+        // gather the list's node records into LDS (one round per 64 entries),
+        // then every body's lane reads them as broadcasts
+        for (int e = lane; e < m; e += 64) {
+          const int node = lid[e];
+          pnc[e] = pnp.nmc[node];
+          pnv[e] = pnp.nvel[node];
+        }
+        wave_sync();
+        if (live) {
+          const double inv_m = 1.0 / pme.w;
+          for (int e = 0; e < m; ++e) node_pn_rec(pnp, pnc[e], pnv[e], pme, vme, inv_m, pax, pay, paz);
+        }
       }
       wave_sync();  // the partial sums are read before the list is refilled
       listed += (unsigned long long)cnt;
@@ -1860,8 +1877,10 @@ static int bh_forces_all(aster_ctx* c, int method) {
       const int blocks = ((n / 32 + 1) + 7) & ~7;
       const dim3 grid(blocks), block(64);
       const int cap = grp_stack_cap(c->tree_levels);
-      const size_t lds = (kGrpList + kGrpListPad) * sizeof(float4) + (size_t)cap * sizeof(int) +
-                         (pn ? kGrpList * sizeof(int) : 0);
+      // PN: node ids, then (16-byte aligned) FP64 records and velocities of the list
+      const size_t lds = (kGrpList + kGrpListPad) * sizeof(float4) +
+                         (pn ? (size_t)(cap + kGrpList + 3) / 4 * sizeof(float4) + 2 * kGrpList * sizeof(double4)
+                             : (size_t)cap * sizeof(int));
       const NodePn gp{c->nmc.p, c->nvel.p, c->P.G, ic, c->P.softening, pnf};
       auto kern = pn ? (mask ? k_bh_group<true, true> : k_bh_group<false, true>)
                      : (mask ? k_bh_group<true, false> : k_bh_group<false, false>);
