@@ -59,6 +59,9 @@ __device__ __forceinline__ void batch_accel(int lane, int base, int cnt, int max
                                             double mi, double G, double ic2, double ic, int pnf, double soft,
                                             double* A) {
   double ax = 0.0, ay = 0.0, az = 0.0;
+  // fast mode, 1PN: this body's |v1|^2 and 5 G m1 (get_pn1's per-body parts)
+  const double v11 = V[0] * V[0] + V[1] * V[1] + V[2] * V[2];
+  const double g5m1 = 5.0 * G * mi;
   for (int j = 0; j < maxcnt; ++j) {
     const int src = min(base + j, 63);
     const bool valid = live && j < cnt && src != lane;
@@ -83,17 +86,31 @@ __device__ __forceinline__ void batch_accel(int lane, int base, int cnt, int max
       const double dx = x2 - X[0], dy = y2 - X[1], dz = z2 - X[2];
       const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, soft)));
       const double inv = rsqrt64(valid ? r2 : 1.0);
-      const double f = valid ? G * m2 * (inv * inv * inv) : 0.0;
+      const double inv2 = inv * inv;
+      const double gm2 = G * m2;
+      const double f = valid ? gm2 * (inv2 * inv) : 0.0;
       ax = fma(f, dx, ax);
       ay = fma(f, dy, ay);
       az = fma(f, dz, az);
       if (PN) {
-        double px, py, pz;
-        pn1_term64(mi, m2, V[0], V[1], V[2], v2x, v2y, v2z, inv, -dx * inv, -dy * inv, -dz * inv, G, px, py, pz);
-        const double s = valid ? ic2 : 0.0;
-        ax = fma(px, s, ax);
-        ay = fma(py, s, ay);
-        az = fma(pz, s, az);
+        // get_pn1 (tool_chain_impl.cpp:110-118) with n = (x1 - x2)/r, the same
+        // terms grouped for fused multiply-adds:
+        //   G m2/r^2 [n (-v1^2 - 2 v2^2 + 4 v1.v2 + 3/2 (n.v2)^2 + (5 G m1 + 4 G m2)/r)
+        //             + (v1 - v2)(4 n.v1 - 3 n.v2)]
+        const double nx = -dx * inv, ny = -dy * inv, nz = -dz * inv;
+        const double nv1 = fma(nx, V[0], fma(ny, V[1], nz * V[2]));
+        const double nv2 = fma(nx, v2x, fma(ny, v2y, nz * v2z));
+        const double v12 = fma(V[0], v2x, fma(V[1], v2y, V[2] * v2z));
+        const double v22 = fma(v2x, v2x, fma(v2y, v2y, v2z * v2z));
+        double sc = fma(4.0, v12, -v11);
+        sc = fma(-2.0, v22, sc);
+        sc = fma(1.5 * nv2, nv2, sc);
+        sc = fma(fma(4.0 * G, m2, g5m1), inv, sc);
+        const double t = fma(4.0, nv1, -3.0 * nv2);
+        const double k = valid ? gm2 * inv2 * ic2 : 0.0;
+        ax = fma(fma(nx, sc, (V[0] - v2x) * t), k, ax);
+        ay = fma(fma(ny, sc, (V[1] - v2y) * t), k, ay);
+        az = fma(fma(nz, sc, (V[2] - v2z) * t), k, az);
       }
     }
   }
@@ -194,8 +211,18 @@ __device__ __forceinline__ void kick_reg(double* V, const double* A, double dt) 
 
 template <bool PN, bool EXACT>
 __global__ __launch_bounds__(256) void k_batch(BatchArgs P) {
-  __shared__ double sx[kBatchMaxWpb][64], sy[kBatchMaxWpb][64], sz[kBatchMaxWpb][64], sm[kBatchMaxWpb][64];
-  __shared__ double svx[PN ? kBatchMaxWpb : 1][64], svy[PN ? kBatchMaxWpb : 1][64], svz[PN ? kBatchMaxWpb : 1][64];
+  // positions, masses (and velocities) of each wavefront's bodies, one plane
+  // per quantity in a single array, so a source's values sit at one address
+  // plus constant plane offsets
+  constexpr int kPlanes = PN ? 7 : 4;
+  __shared__ double planes[kPlanes][kBatchMaxWpb][64];
+  auto sx = planes[0];
+  auto sy = planes[1];
+  auto sz = planes[2];
+  auto sm = planes[3];
+  auto svx = planes[PN ? 4 : 0];
+  auto svy = planes[PN ? 5 : 0];
+  auto svz = planes[PN ? 6 : 0];
   constexpr int kE = (PN && EXACT) ? kBatchMaxWpb : 1;  // EIH potentials and Newtonian accelerations
   __shared__ double sphi[kE][64], sanx[kE][64], sany[kE][64], sanz[kE][64];
   const int wl = threadIdx.x >> 6, lane = threadIdx.x & 63;
