@@ -187,10 +187,13 @@ __global__ __launch_bounds__(256) void k_allpairs_f64(int n, int lower, int uppe
     v1z = v[2 * n + ii];
   }
   double ax = 0.0, ay = 0.0, az = 0.0;
+  // Four sources per iteration: their pair forces are independent, and the
+  // sums still take them one by one in j order (the self pair is skipped by a
+  // select, which adds nothing, as the reference's `continue` does).
+#pragma unroll 4
   for (int j = lower; j < upper; ++j) {
     const double p2x = x[j], p2y = x[n + j], p2z = x[2 * n + j];
     const double m2 = m[j];
-    if (j == ii) continue;
     const double dx = p2x - p1x, dy = p2y - p1y, dz = p2z - p1z;
     const double r2 = dx * dx + dy * dy + dz * dz + soft;
     const double inv_r = 1.0 / sqrt(r2);
@@ -207,9 +210,10 @@ __global__ __launch_bounds__(256) void k_allpairs_f64(int n, int lower, int uppe
       const double v2x = v[j], v2y = v[n + j], v2z = v[2 * n + j];
       pn_add64(pnf, m1, m2, v1x, v1y, v1z, v2x, v2y, v2z, inv_r, -nx, -ny, -nz, G, ic, fx, fy, fz);
     }
-    ax += fx / m1;
-    ay += fy / m1;
-    az += fz / m1;
+    const double nax = ax + fx / m1, nay = ay + fy / m1, naz = az + fz / m1;
+    ax = j == ii ? ax : nax;
+    ay = j == ii ? ay : nay;
+    az = j == ii ? az : naz;
   }
   if (i < n) {
     a[i] = ax;

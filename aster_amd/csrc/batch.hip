@@ -62,10 +62,9 @@ __device__ __forceinline__ void batch_accel(int lane, int base, int cnt, int max
   // fast mode, 1PN: this body's |v1|^2 and 5 G m1 (get_pn1's per-body parts)
   const double v11 = V[0] * V[0] + V[1] * V[1] + V[2] * V[2];
   const double g5m1 = 5.0 * G * mi;
-  // fast mode: four sources per iteration (independent chains; 9% faster);
-  // the exact mode keeps one, in j order either way
-  constexpr int kUnroll = EXACT ? 1 : 4;
-#pragma unroll kUnroll
+  // Four sources per iteration: independent pair forces, sums still taken
+  // one by one in j order (the exact mode's results are unchanged)
+#pragma unroll 4
   for (int j = 0; j < maxcnt; ++j) {
     const int src = min(base + j, 63);
     const bool valid = live && j < cnt && src != lane;
@@ -80,11 +79,12 @@ __device__ __forceinline__ void batch_accel(int lane, int base, int cnt, int max
       double fx, fy, fz;
       grav64(mi, m2, V[0], V[1], V[2], v2x, v2y, v2z, X[0], X[1], X[2], x2, y2, z2, G, ic, soft, PN ? pnf : 0, fx, fy,
              fz);
-      if (valid) {
+      {
 #pragma clang fp contract(off)
-        ax += fx / mi;
-        ay += fy / mi;
-        az += fz / mi;
+        const double nax = ax + fx / mi, nay = ay + fy / mi, naz = az + fz / mi;
+        ax = valid ? nax : ax;
+        ay = valid ? nay : ay;
+        az = valid ? naz : az;
       }
     } else {
       const double dx = x2 - X[0], dy = y2 - X[1], dz = z2 - X[2];
