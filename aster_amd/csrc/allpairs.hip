@@ -13,9 +13,10 @@
 //
 // FP64 path: the reference's operation order (Parallelized::compute_forces,
 // src/utils/builder_endpoint.cpp:729-757, with compute_gravitational,
-// src/physics/tool_chain_impl.cpp:120-137), one target per lane, sequential
-// sources, IEEE division and square root, no contraction: bit-identical to the
-// CPU path.
+// src/physics/tool_chain_impl.cpp:120-137), each target's sources summed in j
+// order, IEEE division and square root, no contraction: bit-identical to the
+// CPU path. The pair contributions of a chunk of sources are formed by a whole
+// block and summed per target afterwards.
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -167,9 +168,94 @@ __global__ void k_combine(int n, int S, const double* __restrict__ part, const d
 
 // ---------------------------------------------------------------- FP64 ----
 
-// MODE 0: Newton (+ the post-Newtonian terms pnf selects if PN); MODE 1: only
-// the post-Newtonian terms (added to an FP32 Newtonian result). x, v: SoA
-// planes [3][n]. ic = 1/c.
+// FP64 accelerations in the reference's arithmetic and order: for target i,
+// sum over sources j in [lower, upper), j != i, in j order, of f/m_i with f =
+// compute_gravitational (Parallelized::compute_forces,
+// src/utils/builder_endpoint.cpp:729-757; tool_chain_impl.cpp:120-137), IEEE
+// division and square root, no contraction. MODE 0: Newton (+ the
+// post-Newtonian terms pnf selects if PN); MODE 1: only the post-Newtonian
+// terms (added to an FP32 Newtonian result). x, v: SoA planes [3][n]; ic = 1/c.
+// A block owns kF64Targets targets and walks the sources in chunks of
+// kF64Chunk: all 256 threads form the chunk's pair contributions f/m_i
+// (4 each) into LDS, then each target's thread adds its row in j order (the
+// self pair and sources past upper are skipped by a select). Every target sums
+// the same doubles in the same order as a sequential loop would, so the
+// result is bit-identical; a 4096-body problem runs 256 blocks instead of 16
+// wavefronts of long dependent chains.
+constexpr int kF64Targets = 16;
+constexpr int kF64Chunk = 64;
+
+template <bool PN, int MODE>
+__global__ __launch_bounds__(256) void k_allpairs_f64_tiled(int n, int lower, int upper,
+                                                            const double* __restrict__ x,
+                                                            const double* __restrict__ v,
+                                                            const double* __restrict__ m, double G, double ic,
+                                                            int pnf, double soft, double* __restrict__ a) {
+#pragma clang fp contract(off)
+  __shared__ double cf[3][kF64Targets][kF64Chunk + 1];
+  const int t = threadIdx.x;
+  const int ti = t % kF64Targets, sj = t / kF64Targets;  // this thread's target and first source slot
+  const int i = blockIdx.x * kF64Targets + ti;
+  const int ii = i < n ? i : n - 1;
+  const double p1x = x[ii], p1y = x[n + ii], p1z = x[2 * n + ii];
+  const double m1 = m[ii];
+  double v1x = 0.0, v1y = 0.0, v1z = 0.0;
+  if (PN) {
+    v1x = v[ii];
+    v1y = v[n + ii];
+    v1z = v[2 * n + ii];
+  }
+  double ax = 0.0, ay = 0.0, az = 0.0;  // the sums (threads t < kF64Targets)
+  for (int jb = lower; jb < upper; jb += kF64Chunk) {
+#pragma unroll
+    for (int q = 0; q < kF64Chunk * kF64Targets / 256; ++q) {
+      const int s = sj + q * (256 / kF64Targets);
+      const int j = min(jb + s, upper - 1);
+      const double p2x = x[j], p2y = x[n + j], p2z = x[2 * n + j];
+      const double m2 = m[j];
+      const double dx = p2x - p1x, dy = p2y - p1y, dz = p2z - p1z;
+      const double r2 = dx * dx + dy * dy + dz * dz + soft;
+      const double inv_r = 1.0 / sqrt(r2);
+      const double inv_r2 = inv_r * inv_r;
+      const double nx = dx * inv_r, ny = dy * inv_r, nz = dz * inv_r;
+      double fx = 0.0, fy = 0.0, fz = 0.0;
+      if (MODE == 0) {
+        const double scal = G * m2 * m1 * inv_r2;
+        fx = nx * scal;
+        fy = ny * scal;
+        fz = nz * scal;
+      }
+      if (PN) {
+        const double v2x = v[j], v2y = v[n + j], v2z = v[2 * n + j];
+        pn_add64(pnf, m1, m2, v1x, v1y, v1z, v2x, v2y, v2z, inv_r, -nx, -ny, -nz, G, ic, fx, fy, fz);
+      }
+      cf[0][ti][s] = fx / m1;
+      cf[1][ti][s] = fy / m1;
+      cf[2][ti][s] = fz / m1;
+    }
+    __syncthreads();
+    if (t < kF64Targets) {
+      const int cnt = min(kF64Chunk, upper - jb);
+      for (int s = 0; s < kF64Chunk; ++s) {
+        const bool take = s < cnt && jb + s != ii;
+        const double nax = ax + cf[0][t][s], nay = ay + cf[1][t][s], naz = az + cf[2][t][s];
+        ax = take ? nax : ax;
+        ay = take ? nay : ay;
+        az = take ? naz : az;
+      }
+    }
+    __syncthreads();  // the next chunk rewrites the contributions
+  }
+  if (t < kF64Targets && i < n) {
+    a[i] = ax;
+    a[n + i] = ay;
+    a[2 * n + i] = az;
+  }
+}
+
+// One target per thread, the sources in j order (the same sums as
+// k_allpairs_f64_tiled, which is faster when there are few targets): four
+// sources per iteration, whose pair forces are independent, added one by one.
 template <bool PN, int MODE>
 __global__ __launch_bounds__(256) void k_allpairs_f64(int n, int lower, int upper, const double* __restrict__ x,
                                                       const double* __restrict__ v, const double* __restrict__ m,
@@ -187,9 +273,6 @@ __global__ __launch_bounds__(256) void k_allpairs_f64(int n, int lower, int uppe
     v1z = v[2 * n + ii];
   }
   double ax = 0.0, ay = 0.0, az = 0.0;
-  // Four sources per iteration: their pair forces are independent, and the
-  // sums still take them one by one in j order (the self pair is skipped by a
-  // select, which adds nothing, as the reference's `continue` does).
 #pragma unroll 4
   for (int j = lower; j < upper; ++j) {
     const double p2x = x[j], p2y = x[n + j], p2z = x[2 * n + j];
@@ -311,6 +394,44 @@ __global__ __launch_bounds__(256) void k_eih_pass2(int n, const double* __restri
 
 // --------------------------------------------------------------- launch ----
 
+// Bit-exact FP64 sums: the tiled kernel below kF64TiledMax targets (few
+// wavefronts of one target per lane would leave most SIMDs idle), one target
+// per lane above.
+#ifndef ASTER_F64_TILED_MAX
+#define ASTER_F64_TILED_MAX 32768
+#endif
+constexpr int kF64TiledMax = ASTER_F64_TILED_MAX;
+
+static int launch_allpairs_f64(aster_ctx* c, bool pn, int mode, int n, int lower, int upper, double ic, int pnf,
+                               double* out) {
+  const bool tiled = n <= kF64TiledMax;
+  const dim3 grid(tiled ? (n + kF64Targets - 1) / kF64Targets : (n + 255) / 256), block(256);
+  const double G = c->P.G, soft = c->P.softening;
+  if (tiled) {
+    if (mode == 1)
+      hipLaunchKernelGGL((k_allpairs_f64_tiled<true, 1>), grid, block, 0, c->stream, n, lower, upper, c->x.p,
+                         c->v.p, c->m.p, G, ic, pnf, soft, out);
+    else if (pn)
+      hipLaunchKernelGGL((k_allpairs_f64_tiled<true, 0>), grid, block, 0, c->stream, n, lower, upper, c->x.p,
+                         c->v.p, c->m.p, G, ic, pnf, soft, out);
+    else
+      hipLaunchKernelGGL((k_allpairs_f64_tiled<false, 0>), grid, block, 0, c->stream, n, lower, upper, c->x.p,
+                         c->v.p, c->m.p, G, ic, pnf, soft, out);
+  } else {
+    if (mode == 1)
+      hipLaunchKernelGGL((k_allpairs_f64<true, 1>), grid, block, 0, c->stream, n, lower, upper, c->x.p, c->v.p,
+                         c->m.p, G, ic, pnf, soft, out);
+    else if (pn)
+      hipLaunchKernelGGL((k_allpairs_f64<true, 0>), grid, block, 0, c->stream, n, lower, upper, c->x.p, c->v.p,
+                         c->m.p, G, ic, pnf, soft, out);
+    else
+      hipLaunchKernelGGL((k_allpairs_f64<false, 0>), grid, block, 0, c->stream, n, lower, upper, c->x.p, c->v.p,
+                         c->m.p, G, ic, pnf, soft, out);
+  }
+  ASTER_HIP(c, hipGetLastError());
+  return ASTER_OK;
+}
+
 int default_split(int64_t npad) {
   // Aim for about 8 waves per SIMD (256 CUs x 4 SIMDs) while keeping each
   // slice at least 1024 sources.
@@ -379,14 +500,8 @@ int launch_allpairs(aster_ctx* c, int method) {
     return ASTER_OK;
   }
   if (method == ASTER_ALLPAIRS_FP64) {
-    dim3 grid((n + 255) / 256);
     phase_begin(c, ASTER_PHASE_ALLPAIRS);
-    if (pn)
-      hipLaunchKernelGGL((k_allpairs_f64<true, 0>), grid, dim3(256), 0, c->stream, n, lower, upper, c->x.p, c->v.p,
-                         c->m.p, c->P.G, ic, pnf, c->P.softening, c->a.p);
-    else
-      hipLaunchKernelGGL((k_allpairs_f64<false, 0>), grid, dim3(256), 0, c->stream, n, lower, upper, c->x.p, c->v.p,
-                         c->m.p, c->P.G, ic, pnf, c->P.softening, c->a.p);
+    if (const int e = launch_allpairs_f64(c, pn, 0, n, lower, upper, ic, pnf, c->a.p)) return e;
     ASTER_HIP(c, hipGetLastError());
     phase_end(c, ASTER_PHASE_ALLPAIRS);
     return ASTER_OK;
@@ -444,8 +559,7 @@ int launch_allpairs(aster_ctx* c, int method) {
   const double* pnp = nullptr;
   if (pn) {
     ASTER_HIP(c, c->apn.reserve((size_t)3 * n));
-    hipLaunchKernelGGL((k_allpairs_f64<true, 1>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n, lower, upper,
-                       c->x.p, c->v.p, c->m.p, c->P.G, ic, pnf, c->P.softening, c->apn.p);
+    if ((rc = launch_allpairs_f64(c, true, 1, n, lower, upper, ic, pnf, c->apn.p))) return rc;
     ASTER_HIP(c, hipGetLastError());
     pnp = c->apn.p;
   }
