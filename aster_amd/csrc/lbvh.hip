@@ -1055,6 +1055,29 @@ __device__ __forceinline__ void node_pn_rec(const NodePn& p, const double4& nc, 
   const double dx = nc.x - pme.x, dy = nc.y - pme.y, dz = nc.z - pme.z;
   const double d2 = dx * dx + dy * dy + dz * dz;
   if (d2 == 0.0) return;
+  if (p.pnf == ASTER_PN1) {
+    // the pairwise 1PN term alone: get_pn1 (tool_chain_impl.cpp:110-118) as an
+    // acceleration, n = (x_i - x_node)/r, grouped for fused multiply-adds:
+    //   G m2/r^2 [n (-v1^2 - 2 v2^2 + 4 v1.v2 + 3/2 (n.v2)^2 + (5 G m1 + 4 G m2)/r)
+    //             + (v1 - v2)(4 n.v1 - 3 n.v2)] / c^2
+    const double inv = rsqrt64(d2 + p.soft);
+    const double nx = -dx * inv, ny = -dy * inv, nz = -dz * inv;
+    const double nv1 = fma(nx, vme.x, fma(ny, vme.y, nz * vme.z));
+    const double nv2 = fma(nx, nv.x, fma(ny, nv.y, nz * nv.z));
+    const double v11 = fma(vme.x, vme.x, fma(vme.y, vme.y, vme.z * vme.z));
+    const double v12 = fma(vme.x, nv.x, fma(vme.y, nv.y, vme.z * nv.z));
+    const double v22 = fma(nv.x, nv.x, fma(nv.y, nv.y, nv.z * nv.z));
+    double sc = fma(4.0, v12, -v11);
+    sc = fma(-2.0, v22, sc);
+    sc = fma(1.5 * nv2, nv2, sc);
+    sc = fma(fma(4.0 * p.G, nc.w, 5.0 * p.G * pme.w), inv, sc);
+    const double t = fma(4.0, nv1, -3.0 * nv2);
+    const double k = p.G * nc.w * (inv * inv) * (p.ic * p.ic);
+    ax = fma(fma(nx, sc, (vme.x - nv.x) * t), k, ax);
+    ay = fma(fma(ny, sc, (vme.y - nv.y) * t), k, ay);
+    az = fma(fma(nz, sc, (vme.z - nv.z) * t), k, az);
+    return;
+  }
   const double inv_r = 1.0 / sqrt(d2 + p.soft);
   double fx = 0.0, fy = 0.0, fz = 0.0;
   pn_add64(p.pnf, pme.w, nc.w, vme.x, vme.y, vme.z, nv.x, nv.y, nv.z, inv_r, -(dx * inv_r), -(dy * inv_r),

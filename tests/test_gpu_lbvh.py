@@ -125,13 +125,13 @@ def test_bh_fp64_parity_pn2_pn25_bit_identical(ctx, orc):
 
 
 @pytest.mark.parametrize("method", [A.BH_GEOMETRIC, A.BH_PARITY])
-def test_bh_per_body_fp32_pn(ctx, orc, method):
+@pytest.mark.parametrize("flags", [A.NEWTON | A.PN1 | A.PN2 | A.PN25, A.NEWTON | A.PN1])
+def test_bh_per_body_fp32_pn(ctx, orc, method, flags):
     """The per-body FP32 walks add the post-Newtonian terms in FP64 for every
     node they take. With every node opened (theta = 0) the post-Newtonian part
     is the all-pairs one (to FP64 summation-order rounding); at theta = 0.5 the
     FP32 reference-MAC walk takes the nodes of the FP64 oracle walk, so most
     bodies get its post-Newtonian part to the same rounding."""
-    flags = A.NEWTON | A.PN1 | A.PN2 | A.PN25
     pos, vel, mass = mixed(3000, seed=44)
 
     def pn_part(theta):
@@ -155,13 +155,16 @@ def test_bh_per_body_fp32_pn(ctx, orc, method):
         assert np.median(e) < 1e-9 and np.quantile(e, 0.9) < 1e-9
 
 
-@pytest.mark.parametrize("n,theta", [(20000, 0.5), (3000, 0.0), (130, 0.7)])
-def test_bh_group_pn_matches_oracle(ctx, orc, n, theta):
-    """Group traversal with the 1PN, 2PN and 2.5PN terms: the FP32 Newtonian
-    sums as without them, plus the FP64 post-Newtonian part over the same
-    lists (node centres of mass, masses and mass-weighted velocities), which
-    must equal the oracle's to FP64 summation-order rounding."""
-    flags = A.NEWTON | A.PN1 | A.PN2 | A.PN25
+@pytest.mark.parametrize("n,theta,flags", [(20000, 0.5, A.NEWTON | A.PN1 | A.PN2 | A.PN25),
+                                           (3000, 0.0, A.NEWTON | A.PN1 | A.PN2 | A.PN25),
+                                           (130, 0.7, A.NEWTON | A.PN1 | A.PN2 | A.PN25),
+                                           (20000, 0.5, A.NEWTON | A.PN1), (3000, 0.0, A.NEWTON | A.PN1)])
+def test_bh_group_pn_matches_oracle(ctx, orc, n, theta, flags):
+    """Group traversal with the post-Newtonian terms: the FP32 Newtonian sums
+    as without them, plus the FP64 post-Newtonian part over the same lists
+    (node centres of mass, masses and mass-weighted velocities), which must
+    equal the oracle's to FP64 rounding (the 1PN term alone is evaluated
+    grouped for fused multiply-adds, the full set in the reference's order)."""
     pos, vel, mass = mixed(n, seed=n + 3)
     box = orc.box_auto(pos)
     load(ctx, pos, vel, mass, theta=theta, precision=A.FP32, c=30.0, pn_flags=A.NEWTON)
