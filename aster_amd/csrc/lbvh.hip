@@ -99,15 +99,19 @@ __global__ __launch_bounds__(THREADS) void k_rs_hist(const uint32_t* __restrict_
   static_assert(THREADS % 256 == 0, "one counter per thread and copy");
   __shared__ uint32_t cnt[H][256];
   const int t = threadIdx.x;
+  const int base = blockIdx.x * TILE;
+  // every load of the thread is issued before the first is used (clamped
+  // addresses, no branch around a load), so the tile costs one memory
+  // round trip rather than kRsPer
+  uint32_t key[kRsPer];
+#pragma unroll
+  for (int r = 0; r < kRsPer; ++r) key[r] = keys[min(base + r * THREADS + t, n - 1)];
   cnt[t >> 8][t & 255] = 0;
   __syncthreads();
   uint32_t* mine = cnt[(t >> 6) % H];  // wavefront w counts into copy w mod H
-  const int base = blockIdx.x * TILE;
 #pragma unroll
-  for (int r = 0; r < kRsPer; ++r) {
-    const int p = base + r * THREADS + t;
-    if (p < n) atomicAdd(&mine[(keys[p] >> shift) & 255u], 1u);
-  }
+  for (int r = 0; r < kRsPer; ++r)
+    if (base + r * THREADS + t < n) atomicAdd(&mine[(key[r] >> shift) & 255u], 1u);
   __syncthreads();
   if (t < 256) {
     uint32_t s = 0;
@@ -137,12 +141,13 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __
                                                               uint32_t* __restrict__ part) {
   __shared__ uint32_t ws[4];
   const int base = blockIdx.x * kScanChunk;
+  // all kScanPer loads issued before the first is used (clamped addresses)
+  uint32_t v[kScanPer];
+#pragma unroll
+  for (int r = 0; r < kScanPer; ++r) v[r] = in[min(base + r * kScanThreads + (int)threadIdx.x, count - 1)];
   uint32_t s = 0;
 #pragma unroll
-  for (int r = 0; r < kScanPer; ++r) {
-    const int p = base + r * kScanThreads + (int)threadIdx.x;
-    if (p < count) s += in[p];
-  }
+  for (int r = 0; r < kScanPer; ++r) s += base + r * kScanThreads + (int)threadIdx.x < count ? v[r] : 0u;
   s = block_sum_256(s, ws);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
@@ -223,17 +228,37 @@ __global__ __launch_bounds__(THREADS) void k_rs_scatter(const uint32_t* __restri
   const int t = threadIdx.x;
   const int lane = t & 63, w = t >> 6;
   const int blk = (int)blockIdx.x;
+  // This wavefront's keys and values, loaded first (clamped addresses, no
+  // branch around a load) so that their latency overlaps the digit starts.
+  const int base = blk * TILE + w * C;
+  uint32_t key[R], val[R], pos[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int p = min(base + r * 64 + lane, n - 1);
+    key[r] = kin[p];
+    val[r] = vin[p];
+  }
   if constexpr (RAW) {
     constexpr int Q = THREADS / 256;
+    // 16-byte loads per thread for the most tiles the raw form takes, all
+    // issued before the sums (entries past nblocks re-read entry 0 and count
+    // nothing)
+    constexpr int KMAX = (kRsRawMaxBlocks / 4 + Q - 1) / Q;
     const int d = t / Q, q = t - d * Q;
     const uint4* __restrict__ row = reinterpret_cast<const uint4*>(hist + (size_t)d * stride);
+    uint4 rv[KMAX];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      const int i = q + k * Q;
+      rv[k] = row[4 * i < nblocks ? i : 0];
+    }
     uint32_t pre = 0, tot = 0;
-    for (int k = q; 4 * k < nblocks; k += Q) {
-      const uint4 v = row[k];
-      const uint32_t e[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      const uint32_t e[4] = {rv[k].x, rv[k].y, rv[k].z, rv[k].w};
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int b = 4 * k + u;
+        const int b = 4 * (q + k * Q) + u;
         const uint32_t c = b < nblocks ? e[u] : 0u;
         tot += c;
         pre += b < blk ? c : 0u;
@@ -271,14 +296,9 @@ __global__ __launch_bounds__(THREADS) void k_rs_scatter(const uint32_t* __restri
   for (int i = t; i < W * 256; i += THREADS) (&wcnt[0][0])[i] = 0;
   __syncthreads();
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int base = blk * TILE + w * C;
-  uint32_t key[R], val[R], pos[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int p = base + r * 64 + lane;
-    const bool valid = p < n;
-    key[r] = valid ? kin[p] : 0u;
-    val[r] = valid ? vin[p] : 0u;
+    const bool valid = base + r * 64 + lane < n;
     const uint32_t d = (key[r] >> shift) & 255u;
     uint64_t m = __ballot(valid);
 #pragma unroll
@@ -487,8 +507,11 @@ __global__ __launch_bounds__(256) void k_above_census(int n, const int32_t* __re
     } else {
       const int g = t - (ng - 1);
       const int lo = gstart[g], hi = gstart[g + 1] - 1;
+      // a group has at most kGroup leaves: all their loads are issued at once
+      // (indices clamped to the group) rather than one round trip per leaf
       int cm = 0;
-      for (int k = lo; k <= hi; ++k) cm = max(cm, climb[k]);
+#pragma unroll
+      for (int k = 0; k < kGroup; ++k) cm = max(cm, climb[min(lo + k, hi)]);
       if (cm > 0) mx = max(mx, walk_depth(groot[lo], parent, flags) + cm - 1);
     }
   }
