@@ -162,15 +162,19 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(uint32_t* __restric
   __shared__ uint32_t ws[4];
   __shared__ uint32_t wsum[4];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  // the thread's entries are loaded first (clamped addresses), so their
+  // latency overlaps the sum of the earlier chunks' totals
+  const int base = blockIdx.x * kScanChunk + kScanPer * t;
+  uint32_t v[kScanPer];
+#pragma unroll
+  for (int r = 0; r < kScanPer; ++r) v[r] = data[min(base + r, count - 1)];
   uint32_t pre = 0;
   for (int b = t; b < (int)blockIdx.x; b += kScanThreads) pre += part[b];
   pre = block_sum_256(pre, ws);
-  const int base = blockIdx.x * kScanChunk + kScanPer * t;
-  uint32_t v[kScanPer];
   uint32_t s = 0;
 #pragma unroll
   for (int r = 0; r < kScanPer; ++r) {
-    v[r] = (base + r < count) ? data[base + r] : 0u;
+    if (base + r >= count) v[r] = 0u;
     s += v[r];
   }
   // inclusive scan of the per-thread sums across the wave, then the waves
