@@ -55,9 +55,9 @@ struct BatchArgs {
 template <bool PN, bool EXACT>
 __device__ __forceinline__ void batch_accel(int lane, int base, int cnt, int maxcnt, bool live, const double* lx,
                                             const double* ly, const double* lz, const double* lm, const double* lvx,
-                                            const double* lvy, const double* lvz, const double* X, const double* V,
-                                            double mi, double G, double ic2, double ic, int pnf, double soft,
-                                            double* A) {
+                                            const double* lvy, const double* lvz, const double* lv22,
+                                            const double* X, const double* V, double mi, double G, double ic2,
+                                            double ic, int pnf, double soft, double* A) {
   double ax = 0.0, ay = 0.0, az = 0.0;
   // fast mode, 1PN: this body's |v1|^2 and 5 G m1 (get_pn1's per-body parts)
   const double v11 = V[0] * V[0] + V[1] * V[1] + V[2] * V[2];
@@ -91,8 +91,10 @@ __device__ __forceinline__ void batch_accel(int lane, int base, int cnt, int max
       const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, soft)));
       const double inv = rsqrt64(valid ? r2 : 1.0);
       const double inv2 = inv * inv;
-      const double gm2 = G * m2;
-      const double f = valid ? gm2 * (inv2 * inv) : 0.0;
+      // a pair that does not count gets G m2 = 0 (its inv is finite), so
+      // both force terms vanish without selects of their own
+      const double gm2 = valid ? G * m2 : 0.0;
+      const double f = gm2 * (inv2 * inv);
       ax = fma(f, dx, ax);
       ay = fma(f, dy, ay);
       az = fma(f, dz, az);
@@ -105,13 +107,13 @@ __device__ __forceinline__ void batch_accel(int lane, int base, int cnt, int max
         const double nv1 = fma(nx, V[0], fma(ny, V[1], nz * V[2]));
         const double nv2 = fma(nx, v2x, fma(ny, v2y, nz * v2z));
         const double v12 = fma(V[0], v2x, fma(V[1], v2y, V[2] * v2z));
-        const double v22 = fma(v2x, v2x, fma(v2y, v2y, v2z * v2z));
+        const double v22 = lv22[src];  // |v2|^2, formed once per body when published
         double sc = fma(4.0, v12, -v11);
         sc = fma(-2.0, v22, sc);
         sc = fma(1.5 * nv2, nv2, sc);
         sc = fma(fma(4.0 * G, m2, g5m1), inv, sc);
         const double t = fma(4.0, nv1, -3.0 * nv2);
-        const double k = valid ? gm2 * inv2 * ic2 : 0.0;
+        const double k = gm2 * inv2 * ic2;
         ax = fma(fma(nx, sc, (V[0] - v2x) * t), k, ax);
         ay = fma(fma(ny, sc, (V[1] - v2y) * t), k, ay);
         az = fma(fma(nz, sc, (V[2] - v2z) * t), k, az);
@@ -218,7 +220,8 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs P) {
   // positions, masses (and velocities) of each wavefront's bodies, one plane
   // per quantity in a single array, so a source's values sit at one address
   // plus constant plane offsets
-  constexpr int kPlanes = PN ? 7 : 4;
+  // (the fast 1PN mode also keeps each body's |v|^2, a per-source term of get_pn1)
+  constexpr int kPlanes = PN ? (EXACT ? 7 : 8) : 4;
   __shared__ double planes[kPlanes][kBatchMaxWpb][64];
   auto sx = planes[0];
   auto sy = planes[1];
@@ -227,6 +230,7 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs P) {
   auto svx = planes[PN ? 4 : 0];
   auto svy = planes[PN ? 5 : 0];
   auto svz = planes[PN ? 6 : 0];
+  auto sv22 = planes[(PN && !EXACT) ? 7 : 0];
   constexpr int kE = (PN && EXACT) ? kBatchMaxWpb : 1;  // EIH potentials and Newtonian accelerations
   __shared__ double sphi[kE][64], sanx[kE][64], sany[kE][64], sanz[kE][64];
   const int wl = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -254,6 +258,7 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs P) {
   double* lvx = svx[PN ? wl : 0];
   double* lvy = svy[PN ? wl : 0];
   double* lvz = svz[PN ? wl : 0];
+  double* lv22 = sv22[wl];
   sm[wl][lane] = mi;
   auto publish = [&]() {
     lx[lane] = X[0];
@@ -263,6 +268,7 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs P) {
       lvx[lane] = V[0];
       lvy[lane] = V[1];
       lvz[lane] = V[2];
+      if (!EXACT) lv22[lane] = fma(V[0], V[0], fma(V[1], V[1], V[2] * V[2]));
     }
     wave_sync();
   };
@@ -273,8 +279,8 @@ __global__ __launch_bounds__(256) void k_batch(BatchArgs P) {
       batch_eih(lane, base, cnt, maxcnt, live, lx, ly, lz, sm[wl], lvx, lvy, lvz, X, V, P.G, P.ic, P.soft, sphi[e],
                 sanx[e], sany[e], sanz[e], A);
     } else {
-      batch_accel<PN, EXACT>(lane, base, cnt, maxcnt, live, lx, ly, lz, sm[wl], lvx, lvy, lvz, X, V, mi, P.G, P.ic2,
-                             P.ic, P.pnf, P.soft, A);
+      batch_accel<PN, EXACT>(lane, base, cnt, maxcnt, live, lx, ly, lz, sm[wl], lvx, lvy, lvz, lv22, X, V, mi, P.G,
+                             P.ic2, P.ic, P.pnf, P.soft, A);
     }
   };
   if (P.init) {
@@ -943,7 +949,7 @@ __global__ __launch_bounds__(256) void k_batch_traj_fwd(BatchArgs P, double* __r
   };
   publish();
   batch_accel<false, false>(lane, base, cnt, maxcnt, live, sx[wl], sy[wl], sz[wl], sm[wl], nullptr, nullptr,
-                            nullptr, X, V, mi, P.G, P.ic2, P.ic, P.pnf, P.soft, A);
+                            nullptr, nullptr, X, V, mi, P.G, P.ic2, P.ic, P.pnf, P.soft, A);
   wave_sync();
   store(0);
   const double dt = P.h[0];
@@ -952,7 +958,7 @@ __global__ __launch_bounds__(256) void k_batch_traj_fwd(BatchArgs P, double* __r
     kick_drift_reg(X, V, A, pending, dt);
     publish();
     batch_accel<false, false>(lane, base, cnt, maxcnt, live, sx[wl], sy[wl], sz[wl], sm[wl], nullptr, nullptr,
-                              nullptr, X, V, mi, P.G, P.ic2, P.ic, P.pnf, P.soft, A);
+                              nullptr, nullptr, X, V, mi, P.G, P.ic2, P.ic, P.pnf, P.soft, A);
     wave_sync();
     pending = dt;
     store(s + 1);
