@@ -38,10 +38,10 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X published FP64 vector peak
 # J^T w 12, w.d 5, mass gradient 2.
 ADJ_FLOP_PER_PAIR = 55
 # FP64 operations per ordered pair and force evaluation of the ensemble
-# kernel's fast mode with pairwise 1PN (what bench_batch runs): Newton 32
-# (separation 3, r^2 6, rsqrt estimate + two Newton steps 13, G m inv^3 4,
+# kernel's fast mode with pairwise 1PN (what bench_batch runs): Newton 26
+# (separation 3, r^2 6, rsqrt estimate + one Newton step 7, G m inv^3 4,
 # accumulation 6) + 1PN 87 (unit vector 3, get_pn1 78, accumulation 6).
-BATCH_FLOP_PER_PAIR = 119
+BATCH_FLOP_PER_PAIR = 113
 HBM_PEAK_GBPS = 8000.0    # MI355X published HBM3E bandwidth
 
 # Algorithmic HBM bytes per body of the bandwidth-bound phases of a BH step
