@@ -5,6 +5,19 @@
 
 namespace aster {
 
+// One coordinate of a body's kick/drift (update_leapfrog,
+// src/physics/tool_chain_impl.cpp:169-185): the previous stage's second
+// half-kick (dt_prev, when not 0), the drift and this stage's first
+// half-kick, all with the same acceleration, in the reference's operation
+// order and without contraction.
+__device__ __forceinline__ void kick_drift_1(double& x, double& v, double a, double dt_prev, double dt) {
+#pragma clang fp contract(off)
+  double vc = v;
+  if (dt_prev != 0.0) vc = vc + 0.5 * a * dt_prev;
+  x = x + (vc * dt + 0.5 * a * dt * dt);
+  v = vc + 0.5 * a * dt;
+}
+
 // get_pn1 (src/physics/tool_chain_impl.cpp:110-118) evaluated in the
 // reference's operation order; `n` is the caller's -norm (:133).
 __device__ __forceinline__ void pn1_term64(double m1, double m2, double v1x, double v1y, double v1z, double v2x,
@@ -298,6 +311,7 @@ __device__ inline void kepler_drift64(double* q, double* u, double mu, double dt
 __device__ __forceinline__ float uniform_f32(float x) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
 }
+
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -11,6 +11,7 @@
 #include <algorithm>
 
 #include "common.hpp"
+#include "device.hpp"
 
 namespace aster {
 
@@ -30,11 +31,8 @@ __global__ __launch_bounds__(256) void k_kick_drift(int n, double* __restrict__ 
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const size_t k = (size_t)c * n + i;
-      const double ac = a[k];
-      double vc = v[k];
-      if (dt_prev != 0.0) vc = vc + 0.5 * ac * dt_prev;
-      const double xc = x[k] + (vc * dt + 0.5 * ac * dt * dt);
-      vc = vc + 0.5 * ac * dt;
+      double xc = x[k], vc = v[k];
+      kick_drift_1(xc, vc, a[k], dt_prev, dt);
       x[k] = xc;
       v[k] = vc;
       mx[c] = fabs(xc);
@@ -182,10 +180,22 @@ __global__ __launch_bounds__(kBoxFinishThreads) void k_box_finish(const double* 
   __shared__ double wm[3][kBoxFinishThreads / 64];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   double r[3] = {0.0, 0.0, 0.0};
+  // 8 partials per thread and round, all loaded before the first is used
+  // (clamped indices), so a round costs one memory round trip
   if (!use_user)
-    for (int i = t; i < nparts; i += kBoxFinishThreads)
+    for (int i0 = t; i0 < nparts; i0 += 8 * kBoxFinishThreads) {
+      double q[8][3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) r[c] = fmax(r[c], part[3 * i + c]);
+      for (int u = 0; u < 8; ++u) {
+        const int i = min(i0 + u * kBoxFinishThreads, nparts - 1);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) q[u][c] = part[3 * i + c];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) r[c] = fmax(r[c], q[u][c]);
+    }
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     for (int off = 32; off > 0; off >>= 1) r[c] = fmax(r[c], __shfl_xor(r[c], off, 64));

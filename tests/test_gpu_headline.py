@@ -71,6 +71,33 @@ def test_bh_group_leapfrog_steps(ctx, orc, n, theta):
     assert np.quantile(e, 0.999) < 1e-4
 
 
+@pytest.mark.parametrize("integrator", [A.LEAPFROG, A.YOSHIDA4])
+@pytest.mark.parametrize("soft", [SOFT, 0.0])
+def test_bh_group_steps_split_same_bits(ctx, integrator, soft):
+    """Several steps in one aster_step call (each kick/drift pass finishes the
+    previous step's half-kick, and each drift leaves the next automatic box's
+    partials) against one step per call (a separate closing half-kick, the box
+    formed from scratch): positions, velocities and accelerations must be the
+    same bits (softening 0 takes the masked traversal)."""
+    dt, steps = 5e-4, 4
+    pos, vel, mass = A.ic_mixed(40000, 77)
+    vel = vel + 0.05 * np.random.default_rng(3).standard_normal(vel.shape)  # bodies cross Morton cells
+    out = []
+    for split in (False, True):
+        ctx.set_params(G=1.0, softening=soft, theta=0.5, precision=A.FP32, box=(0.0, 0.0, 0.0), lower=0, upper=-1)
+        ctx.set_state(pos, vel, mass)
+        if split:
+            for _ in range(steps):
+                ctx.step(A.BH_GROUP, integrator, dt, 1)
+        else:
+            ctx.step(A.BH_GROUP, integrator, dt, steps)
+        p, v = ctx.get_state()
+        out.append((p, v, ctx.get_acc()))
+    for q in range(3):
+        assert np.array_equal(out[0][q], out[1][q]), q
+    assert np.all(np.isfinite(out[0][0]))
+
+
 @pytest.fixture(scope="module")
 def full_state():
     """The headline state and its FP64 all-pairs accelerations (the device
