@@ -80,7 +80,6 @@ __device__ __forceinline__ void adj_tile(const double4* __restrict__ src, const 
     if (SELF) inv = (k == lane) ? 0.f : inv;
     const float inv2 = inv * inv;
     const float inv3 = inv2 * inv;
-    const float inv5 = inv3 * inv2;
     const float mj = unpack_mass_adj(q.w);
     const float f = mj * inv3;
     ax = fmaf(f, dx, ax);
@@ -90,10 +89,11 @@ __device__ __forceinline__ void adj_tile(const double4* __restrict__ src, const 
     const float uy = fmaf(mk, p.y, -(mj * wk.y));
     const float uz = fmaf(mk, p.z, -(mj * wk.z));
     const float du = fmaf(dx, ux, fmaf(dy, uy, dz * uz));
-    const float c3 = (du * inv5) * 3.f;
-    gx = fmaf(-c3, dx, fmaf(ux, inv3, gx));
-    gy = fmaf(-c3, dy, fmaf(uy, inv3, gy));
-    gz = fmaf(-c3, dz, fmaf(uz, inv3, gz));
+    // J^T w: inv3 u - 3 (d.u) inv5 d = inv3 (u - s d), s = 3 (d.u) inv^2
+    const float s3 = (du * inv2) * 3.f;
+    gx = fmaf(fmaf(-s3, dx, ux), inv3, gx);
+    gy = fmaf(fmaf(-s3, dy, uy), inv3, gy);
+    gz = fmaf(fmaf(-s3, dz, uz), inv3, gz);
     const float dw = fmaf(dx, p.x, fmaf(dy, p.y, dz * p.z));
     gm = fmaf(-dw, inv3, gm);
   }

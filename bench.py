@@ -33,10 +33,10 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak (packed FP32 FMAs)
 FP32_NONPACKED_TFLOPS = 78.6
 FP64_PEAK_TFLOPS = 78.6   # MI355X published FP64 vector peak
 # FP32 operations (an FMA counts 2) per source-target pair of the fused
-# adjoint kernel k_adj_fused: separation 3, r^2 6, rsqrt 1, inverse powers 3,
-# m inv^3 1, acceleration 6, u = m_k w_j - m_j w_k 9, d.u 5, 3 (d.u) inv^5 2,
-# J^T w 12, w.d 5, mass gradient 2.
-ADJ_FLOP_PER_PAIR = 55
+# adjoint kernel k_adj_fused: separation 3, r^2 6, rsqrt 1, inverse powers 2,
+# m inv^3 1, acceleration 6, u = m_k w_j - m_j w_k 9, d.u 5, s = 3 (d.u) inv^2 2,
+# J^T w = inv^3 (u - s d) 12, w.d 5, mass gradient 2.
+ADJ_FLOP_PER_PAIR = 54
 # FP64 operations per ordered pair and force evaluation of the ensemble
 # kernel's fast mode with pairwise 1PN (what bench_batch runs): Newton 26
 # (separation 3, r^2 6, rsqrt estimate + one Newton step 7, G m inv^3 4,
