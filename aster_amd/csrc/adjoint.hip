@@ -143,13 +143,29 @@ __global__ void k_adj_combine(int n, int S, const double* __restrict__ part, dou
                               double* __restrict__ a, double* __restrict__ gx, double* __restrict__ gm) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  // 4 slices per round, their 28 loads issued before the first add (clamped
+  // slice index); the adds stay in slice order
+  double s[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < S; k0 += 4) {
+    double v[4][7];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = min(k0 + u, S - 1);
+#pragma unroll
+      for (int q = 0; q < 7; ++q) v[u][q] = part[((size_t)k * 7 + q) * n + i];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (k0 + u < S)
+#pragma unroll
+        for (int q = 0; q < 7; ++q) s[q] += v[u][q];
+  }
+#pragma unroll
   for (int q = 0; q < 7; ++q) {
-    double s = 0.0;
-    for (int k = 0; k < S; ++k) s += part[((size_t)k * 7 + q) * n + i];
-    s *= q < 6 ? gsc : G;
-    if (q < 3) a[(size_t)q * n + i] = s;
-    else if (q < 6) gx[(size_t)(q - 3) * n + i] = s;
-    else gm[i] = s;
+    const double r = s[q] * (q < 6 ? gsc : G);
+    if (q < 3) a[(size_t)q * n + i] = r;
+    else if (q < 6) gx[(size_t)(q - 3) * n + i] = r;
+    else gm[i] = r;
   }
 }
 

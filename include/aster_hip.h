@@ -160,9 +160,12 @@ int aster_get_acc(aster_ctx* ctx, double* acc);
 /* Allocation-free all-pairs. No counterpart in the reference: for callers
  * that cannot allocate device memory (code under HIP graph capture, serving
  * loops, harnesses that forbid hipMalloc). The ASTER_ALLPAIRS_FP32_KAHAN
- * evaluation of aster_forces (the same kernels, source split and FP32 mass
- * scale, so the same bits as aster_forces with those G and softening, full
- * range, Newtonian) on caller-owned device arrays of the current device:
+ * evaluation of aster_forces (the same kernels and FP32 mass scale) with the
+ * source split S = min(64, npad / 1024, ceil(8192 / (npad / 64))), at least 1,
+ * npad = n rounded up to 256: the same bits as aster_forces with those G and
+ * softening, aster_params.split = S, full range, Newtonian. (aster_forces's
+ * own default split is finer; S bounds the static scratch below.) It works
+ * on caller-owned device arrays of the current device:
  * d_pos and d_acc AoS-d3 (3n doubles), d_mass n doubles. It allocates
  * nothing: its working memory is a static device array of about 17 MiB per
  * GPU, held until the process ends, shared by every call on that GPU, so

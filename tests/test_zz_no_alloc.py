@@ -1,8 +1,8 @@
 """aster_forces_noalloc (the allocation-free all-pairs entry point) against
-aster_forces with ASTER_ALLPAIRS_FP32_KAHAN, through the C ABI: the same bits
-on two calls in a row, the second with a smaller input (and an SI-scale
-system, whose mass scale the entry point forms on the device), and the FP32
-bar against the oracle."""
+aster_forces with ASTER_ALLPAIRS_FP32_KAHAN and the entry point's source split,
+through the C ABI: the same bits on two calls in a row, the second with a
+smaller input (and an SI-scale system, whose mass scale the entry point forms
+on the device), and the FP32 bar against the oracle."""
 import ctypes
 import os
 
@@ -58,10 +58,19 @@ CASES = [
 ]
 
 
+def noalloc_split(n):
+    """The entry point's source split as its declaration states it:
+    S = min(64, npad / 1024, ceil(8192 / (npad / 64))), at least 1, npad = n
+    rounded up to 256."""
+    npad = -(-n // 256) * 256
+    waves = npad // 64
+    return max(1, min(64, npad // 1024, -(-8192 // waves)))
+
+
 def test_noalloc_matches_aster_forces(ctx, orc):
     for n, G, soft, make in CASES:
         pos, vel, mass = make(n)
-        ctx.set_params(G=G, softening=soft, lower=0, upper=-1, pn_flags=A.NEWTON, split=0)
+        ctx.set_params(G=G, softening=soft, lower=0, upper=-1, pn_flags=A.NEWTON, split=noalloc_split(n))
         ctx.set_state(pos, vel, mass)
         ref = ctx.forces(A.ALLPAIRS_FP32_KAHAN)
         dp, dm, da = DeviceArray(pos), DeviceArray(mass), DeviceArray(np.full((n, 3), np.nan))
