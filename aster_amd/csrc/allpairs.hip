@@ -604,8 +604,8 @@ int launch_allpairs(aster_ctx* c, int method) {
 //
 // aster_forces_noalloc: the ASTER_ALLPAIRS_FP32_KAHAN evaluation (same
 // kernels and mass scale as aster_forces; the source split of about 8 waves
-// per SIMD, which aster_forces uses with aster_params.split set to it) on caller-owned device
-// arrays, with no device allocation. Its working memory is the file-scope
+// per SIMD, which aster_forces uses with aster_params.split set to it) on
+// caller-owned device arrays, with no device allocation. Its working memory is the file-scope
 // array `scratch` below: each GPU the library runs on holds one copy until
 // the process ends, and every call on that GPU uses it, so calls that
 // overlap on one GPU race. Nothing in it needs clearing: every call writes
