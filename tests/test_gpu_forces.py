@@ -129,6 +129,31 @@ def test_split_does_not_change_fp64_parity_of_combine(ctx, orc):
         assert np.median(rel_err(a, orc.allpairs(pos, vel, mass, soft=1e-4))) < 1e-6
 
 
+@pytest.mark.parametrize("n,lower,upper", [(1000, 0, -1), (5000, 0, -1), (100003, 0, -1), (100003, 1234, 77777)])
+def test_fp32_source_slices_at_the_split_rules_edges(ctx, n, lower, upper):
+    """The default source split (one slice below 2048 bodies, a short last
+    slice, 42 slices at 100003 bodies, a source range) against one slice and
+    against the FP64 path: every split must give the FP32 bars (Kahan: 1e-5,
+    plain 1e-4, per body relative to max(|a_i|, median |a|), since a source
+    range leaves some bodies with nearly cancelled forces), and the splits
+    differ only by FP32 rounding."""
+    pos, vel, mass = plummer(n, seed=n % 97, mass_mode=1)
+    ctx.set_params(G=1.0, softening=2.5e-5, lower=lower, upper=upper, split=0)
+    ctx.set_state(pos, vel, mass)
+    ref = ctx.forces(A.ALLPAIRS_FP64)
+    scale = np.linalg.norm(ref, axis=1)
+    scale = np.maximum(scale, np.median(scale))
+    for method, bar in ((A.ALLPAIRS_FP32_KAHAN, 1e-5), (A.ALLPAIRS_FP32, 1e-4)):
+        out = []
+        for split in (0, 1):
+            ctx.set_params(G=1.0, softening=2.5e-5, lower=lower, upper=upper, split=split)
+            a = ctx.forces(method)
+            e = np.linalg.norm(a - ref, axis=1) / scale
+            assert np.all(np.isfinite(a)) and e.max() <= bar, (n, method, split, e.max())
+            out.append(a)
+        assert (np.linalg.norm(out[0] - out[1], axis=1) / scale).max() <= bar, (n, method)
+
+
 def test_leapfrog_trajectory_bit_identical(ctx, orc):
     # Config 1 shape (Plummer, eps = 0.01 -> softening 1e-4), FP64 all-pairs.
     pos, vel, mass = plummer(1024, seed=8)
