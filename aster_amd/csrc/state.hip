@@ -72,12 +72,28 @@ __global__ __launch_bounds__(256) void k_com_partial(int64_t n, const double* __
   if (threadIdx.x < kComQ) part[kComQ * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
-__global__ void k_com_final(int nparts, const double* __restrict__ part, double* __restrict__ out) {
-  if (threadIdx.x != 0) return;
+// One block of 64 threads: the partials are staged in LDS with one load
+// round per thread, then thread 0 sums them in block order (the order, and so
+// the bits, of a single thread reading them one by one).
+__global__ __launch_bounds__(64) void k_com_final(int nparts, const double* __restrict__ part,
+                                                  double* __restrict__ out) {
+  __shared__ double sp[kComQ * kComBlocks];
+  const int t = threadIdx.x;
+  const int nb = min(nparts, kComBlocks);  // callers pass at most kComBlocks partials
+  const int total = kComQ * nb;
+  constexpr int kPer = (kComQ * kComBlocks + 63) / 64;
+  double v[kPer];
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) v[r] = part[min(t + 64 * r, max(total - 1, 0))];
+#pragma unroll
+  for (int r = 0; r < kPer; ++r)
+    if (t + 64 * r < total) sp[t + 64 * r] = v[r];
+  __syncthreads();
+  if (t != 0) return;
   double s[kComQ] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int b = 0; b < nparts; ++b) {
-    for (int q = 0; q < 4; ++q) s[q] += part[kComQ * b + q];
-    s[4] = fmax(s[4], part[kComQ * b + 4]);
+  for (int b = 0; b < nb; ++b) {
+    for (int q = 0; q < 4; ++q) s[q] += sp[kComQ * b + q];
+    s[4] = fmax(s[4], sp[kComQ * b + 4]);
   }
   for (int q = 0; q < kComQ; ++q) out[q] = s[q];
 }
